@@ -1,0 +1,1006 @@
+/*
+ * fame_oracle.cpp -- FP64 CPU restatement of the FAME-Isotropic-GPU eigensolver.
+ *
+ * TEST INFRASTRUCTURE ONLY: this is the parity checker (and the CPU baseline
+ * timed by bench.py).  The product library never links or loads it.
+ *
+ * It follows the reference algorithm step by step (same tables, same operator
+ * order, x0 = 0 CG with absolute tolerance, e_n start vector, one-pass modified
+ * Gram-Schmidt, exact-shift lock/purge restart, omega = sqrt(1/theta)).  The
+ * only replaced third-party pieces are cuFFT (own Stockham + Bluestein FFT,
+ * published algorithms) and LAPACKE_zpteqr (own implicit-QL tridiagonal
+ * eigensolver, textbook method); both are pinned by tests against numpy.
+ */
+#include "fame_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef std::complex<double> cplx;
+static const double PI = 3.141592653589793;   /* include/FAME_Internal_Common.h:24 */
+
+static int g_threads = 1;
+extern "C" void orc_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+extern "C" int  orc_get_threads(void) { return g_threads; }
+
+/* ========================================================================== */
+/*  FFT: mixed-radix Stockham autosort (radices 2..13) + Bluestein chirp-z     */
+/* ========================================================================== */
+struct Plan {
+    int n = 0;
+    std::vector<int> radices;       /* Stockham factorisation                 */
+    std::vector<cplx> Wm, Wp;       /* exp(-+2 pi i m / n)                    */
+    bool blue = false;              /* Bluestein for large prime factors      */
+    int M = 0;
+    Plan* sub = nullptr;            /* length-M plan                          */
+    std::vector<cplx> chirp_m, chirp_p;   /* exp(-+ i pi m^2 / n)             */
+    std::vector<cplx> bhat_m, bhat_p;     /* FFT_M of conj-chirp kernel        */
+};
+
+static std::mutex g_plan_mtx;
+static std::map<int, Plan*> g_plans;
+void fft_line(Plan* P, int sign, cplx* x, cplx* w1, cplx* w2);
+
+static bool smooth13(int n, std::vector<int>* rad) {
+    std::vector<int> r;
+    int m = n;
+    while (m % 4 == 0) { r.push_back(4); m /= 4; }
+    while (m % 2 == 0) { r.push_back(2); m /= 2; }
+    const int ps[] = {3, 5, 7, 11, 13};
+    for (int p : ps) while (m % p == 0) { r.push_back(p); m /= p; }
+    if (m != 1) return false;
+    if (rad) *rad = r;
+    return true;
+}
+
+static Plan* get_plan(int n);
+
+static Plan* build_plan(int n) {
+    Plan* P = new Plan;
+    P->n = n;
+    P->Wm.resize(n); P->Wp.resize(n);
+    for (int m = 0; m < n; ++m) {
+        long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        double c = (double)cosl(ang), s = (double)sinl(ang);
+        P->Wm[m] = cplx(c, -s);
+        P->Wp[m] = cplx(c, s);
+    }
+    if (n == 1) return P;
+    if (smooth13(n, &P->radices)) return P;
+    /* Bluestein: X[k] = w[k] sum_j (x[j] w[j]) conj(w[k-j]),  w[m] = exp(s i pi m^2/n) */
+    P->blue = true;
+    int M = 2 * n - 1;
+    while (!smooth13(M, nullptr)) ++M;
+    P->M = M;
+    P->chirp_m.resize(n); P->chirp_p.resize(n);
+    for (int m = 0; m < n; ++m) {
+        long long q = ((long long)m * m) % (2LL * n);
+        long double ang = 3.14159265358979323846264338327950288L * (long double)q / (long double)n;
+        double c = (double)cosl(ang), s = (double)sinl(ang);
+        P->chirp_m[m] = cplx(c, -s);
+        P->chirp_p[m] = cplx(c, s);
+    }
+    P->sub = get_plan(M);
+    return P;
+}
+
+static Plan* get_plan(int n) {
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mtx);
+        auto it = g_plans.find(n);
+        if (it != g_plans.end()) return it->second;
+    }
+    Plan* P = build_plan(n);
+    if (P->blue) {
+        /* kernel b[m] = conj(w[m]) for |m| < n (wrapped), transformed once per sign */
+        int M = P->M, n2 = P->n;
+        std::vector<cplx> bm(M, 0.0), bp(M, 0.0);
+        for (int m = 0; m < n2; ++m) {
+            bm[m] = std::conj(P->chirp_m[m]); bp[m] = std::conj(P->chirp_p[m]);
+            if (m) { bm[M - m] = bm[m]; bp[M - m] = bp[m]; }
+        }
+        std::vector<cplx> w1(M);
+        fft_line(P->sub, -1, bm.data(), w1.data(), nullptr);
+        fft_line(P->sub, -1, bp.data(), w1.data(), nullptr);
+        P->bhat_m = bm; P->bhat_p = bp;
+    }
+    std::lock_guard<std::mutex> lk(g_plan_mtx);
+    auto it = g_plans.find(n);
+    if (it != g_plans.end()) { delete P; return it->second; }
+    g_plans[n] = P;
+    return P;
+}
+
+/* small DFT of radix r: out[q] = sum_t v[t] W_r^{sign q t}, using the plan table */
+static inline void dft_r(int r, cplx* v, int sign, const std::vector<cplx>& W, int n) {
+    if (r == 2) { cplx a = v[0], b = v[1]; v[0] = a + b; v[1] = a - b; return; }
+    if (r == 4) {
+        cplx a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = v[1] - v[3];
+        cplx j3 = (sign < 0) ? cplx(a3.imag(), -a3.real()) : cplx(-a3.imag(), a3.real());
+        v[0] = a0 + a2; v[2] = a0 - a2; v[1] = a1 + j3; v[3] = a1 - j3; return;
+    }
+    cplx out[13];
+    const std::vector<cplx>& T = W;
+    int step = n / r;
+    for (int q = 0; q < r; ++q) {
+        cplx acc = 0.0;
+        for (int t = 0; t < r; ++t) acc += v[t] * T[((q * t) % r) * step];
+        out[q] = acc;
+    }
+    for (int q = 0; q < r; ++q) v[q] = out[q];
+}
+
+/* one line, length P->n; x in/out; w1, w2 scratch of length max(n, M) */
+void fft_line(Plan* P, int sign, cplx* x, cplx* w1, cplx* w2) {
+    int n = P->n;
+    if (n == 1) return;
+    if (!P->blue) {
+        const std::vector<cplx>& W = sign < 0 ? P->Wm : P->Wp;
+        cplx* a = x; cplx* b = w1;
+        int Ns = 1;
+        for (int r : P->radices) {
+            int nb = n / r, tstep = n / (Ns * r);
+            for (int j = 0; j < nb; ++j) {
+                int k = j % Ns;
+                cplx v[13];
+                for (int t = 0; t < r; ++t) {
+                    v[t] = a[j + t * nb];
+                    if (t && k) v[t] *= W[((long long)t * k * tstep) % n];
+                }
+                /* small DFT uses W_r = W_n^(n/r) */
+                dft_r(r, v, sign, W, n);
+                int base = (j / Ns) * Ns * r + k;
+                for (int t = 0; t < r; ++t) b[base + t * Ns] = v[t];
+            }
+            std::swap(a, b);
+            Ns *= r;
+        }
+        if (a != x) std::memcpy(x, a, sizeof(cplx) * n);
+        return;
+    }
+    int M = P->M;
+    const std::vector<cplx>& ch = sign < 0 ? P->chirp_m : P->chirp_p;
+    const std::vector<cplx>& bh = sign < 0 ? P->bhat_m : P->bhat_p;
+    cplx* A = w1;               /* length M; w2 is the sub-transform's ping-pong buffer */
+    for (int j = 0; j < n; ++j) A[j] = x[j] * ch[j];
+    for (int j = n; j < M; ++j) A[j] = 0.0;
+    fft_line(P->sub, -1, A, w2, nullptr);
+    for (int j = 0; j < M; ++j) A[j] *= bh[j];
+    fft_line(P->sub, +1, A, w2, nullptr);
+    double inv = 1.0 / M;
+    for (int kk = 0; kk < n; ++kk) x[kk] = A[kk] * ch[kk] * inv;
+}
+
+/* transform along one axis of a 3-D array dims (n0 fastest, n1, n2) */
+static void fft_axis(cplx* data, int n0, int n1, int n2, int axis, int sign) {
+    int n = axis == 0 ? n0 : (axis == 1 ? n1 : n2);
+    Plan* P = get_plan(n);
+    int nlines = (n0 * n1 * n2) / n;
+    size_t stride = axis == 0 ? 1 : (axis == 1 ? (size_t)n0 : (size_t)n0 * n1);
+    int W = std::max(n, P->blue ? P->M : n);
+#pragma omp parallel num_threads(g_threads)
+    {
+        std::vector<cplx> line(W), w1(W), w2(W);
+#pragma omp for schedule(static)
+        for (int l = 0; l < nlines; ++l) {
+            size_t base;
+            if (axis == 0) base = (size_t)l * n0;
+            else if (axis == 1) { int i0 = l % n0, i2 = l / n0; base = (size_t)i2 * n0 * n1 + i0; }
+            else base = (size_t)l;
+            for (int j = 0; j < n; ++j) line[j] = data[base + j * stride];
+            fft_line(P, sign, line.data(), w1.data(), w2.data());
+            for (int j = 0; j < n; ++j) data[base + j * stride] = line[j];
+        }
+    }
+}
+
+extern "C" void orc_fft(int n, int sign, double* data, int howmany) {
+    Plan* P = get_plan(n);
+    int W = std::max(n, P->blue ? P->M : n);
+    std::vector<cplx> w1(W), w2(W), line(W);
+    cplx* d = reinterpret_cast<cplx*>(data);
+    for (int h = 0; h < howmany; ++h) {
+        std::memcpy(line.data(), d + (size_t)h * n, sizeof(cplx) * n);
+        fft_line(P, sign, line.data(), w1.data(), w2.data());
+        std::memcpy(d + (size_t)h * n, line.data(), sizeof(cplx) * n);
+    }
+}
+
+/* ========================================================================== */
+/*  Per-k tables  (FAME_Matrix_Lambdas.cu)                                     */
+/* ========================================================================== */
+struct orc_tables {
+    orc_lattice lat;
+    int N, Nd, general;
+    double k[3];
+    std::vector<cplx> Lx, Ly, Lz;           /* Nd */
+    std::vector<double> Lq;                  /* Nd  (Lambda_q_sqrt) */
+    std::vector<cplx> PiQr, PiPr, PiQrs, PiPrs;   /* 6 Nd */
+    std::vector<cplx> Dk, Dks;               /* Simple: N */
+    std::vector<cplx> Dkx, Dky, Dkz;         /* General: Nx, Nx*Ny, N */
+};
+
+static inline cplx cexpi(double arg) { return cplx(std::cos(arg), std::sin(arg)); }
+
+static void build_pi(orc_tables* T, const std::vector<cplx>& Ldx, const std::vector<cplx>& Ldy,
+                     const std::vector<cplx>& Ldz) {
+    int N = T->N, Nd = T->Nd, off = N - Nd;
+    T->Lx.assign(Ldx.begin() + off, Ldx.end());
+    T->Ly.assign(Ldy.begin() + off, Ldy.end());
+    T->Lz.assign(Ldz.begin() + off, Ldz.end());
+    T->Lq.resize(Nd);
+    T->PiQr.resize(6 * (size_t)Nd); T->PiPr.resize(6 * (size_t)Nd);
+    T->PiQrs.resize(6 * (size_t)Nd); T->PiPrs.resize(6 * (size_t)Nd);
+    for (int i = 0; i < Nd; ++i) {
+        cplx lx = T->Lx[i], ly = T->Ly[i], lz = T->Lz[i];
+        double Lq = std::real(std::conj(lx) * lx + std::conj(ly) * ly + std::conj(lz) * lz);
+        cplx p1[3], p2[3];
+        if (!T->general) {
+            /* Construct_Lambdas_Simple, FAME_Matrix_Lambdas.cu:184-215 (alpha=3, beta=5) */
+            const double al = 3., be = 5.;
+            cplx t1 = be * lz - ly, t2 = lx - al * lz, t3 = al * ly - be * lx;
+            cplx n2 = std::conj(t1) * t1 + std::conj(t2) * t2 + std::conj(t3) * t3;
+            cplx n1 = n2 * Lq;
+            cplx s1 = std::sqrt(n1), s2 = std::sqrt(n2);
+            p1[0] = ((al * ly - be * lx) * std::conj(ly) - (lx - al * lz) * std::conj(lz)) / s1;
+            p1[1] = ((be * lz - ly) * std::conj(lz) - (al * ly - be * lx) * std::conj(lx)) / s1;
+            p1[2] = ((lx - al * lz) * std::conj(lx) - (be * lz - ly) * std::conj(ly)) / s1;
+            p2[0] = (be * std::conj(lz) - std::conj(ly)) / s2;
+            p2[1] = (std::conj(lx) - al * std::conj(lz)) / s2;
+            p2[2] = (al * std::conj(ly) - be * std::conj(lx)) / s2;
+        } else {
+            /* Construct_Lambdas_General, FAME_Matrix_Lambdas.cu:441-461 */
+            cplx Ls = lx + ly + lz;
+            cplx Lp = std::conj(Ls) * Ls;
+            cplx n1 = std::sqrt(cplx(3 * Lq * Lq) - Lq * Lp);
+            cplx n2 = std::sqrt(cplx(3 * Lq) - Lp);
+            p1[0] = (Lq - lx * std::conj(Ls)) / n1;
+            p1[1] = (Lq - ly * std::conj(Ls)) / n1;
+            p1[2] = (Lq - lz * std::conj(Ls)) / n1;
+            p2[0] = (std::conj(lz) - std::conj(ly)) / n2;
+            p2[1] = (std::conj(lx) - std::conj(lz)) / n2;
+            p2[2] = (std::conj(ly) - std::conj(lx)) / n2;
+        }
+        T->Lq[i] = std::sqrt(Lq);
+        size_t n = Nd;
+        for (int c = 0; c < 3; ++c) {
+            T->PiQr[c * n + i] = p1[c];
+            T->PiQr[(c + 3) * n + i] = p2[c];
+            T->PiPr[c * n + i] = -std::conj(p2[c]);       /* :224-229 */
+            T->PiPr[(c + 3) * n + i] = std::conj(p1[c]);
+        }
+        /* permuted conjugates, :231-243 */
+        const int perm[6] = {0, 3, 1, 4, 2, 5};
+        for (int q = 0; q < 6; ++q) {
+            T->PiQrs[q * n + i] = std::conj(T->PiQr[perm[q] * n + i]);
+            T->PiPrs[q * n + i] = std::conj(T->PiPr[perm[q] * n + i]);
+        }
+    }
+}
+
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+extern "C" orc_tables* orc_tables_create(const orc_lattice* lat, const double k[3]) {
+    orc_tables* T = new orc_tables;
+    T->lat = *lat;
+    T->general = lat->general;
+    T->k[0] = k[0]; T->k[1] = k[1]; T->k[2] = k[2];
+    const int Nx = lat->N[0], Ny = lat->N[1], Nz = lat->N[2];
+    const int N = Nx * Ny * Nz;
+    T->N = N;
+    const double* a = lat->a;
+    const double* h = lat->h;
+    double knorm = std::sqrt(dot3(k, k));
+    std::vector<cplx> Ldx(N), Ldy(N), Ldz(N);
+    if (!lat->general) {
+        /* FAME_Matrix_Lambdas.cu:43-45, 77-181 */
+        double th[3] = {dot3(k, a), dot3(k, a + 3), dot3(k, a + 6)};
+        std::vector<cplx> L1(Nx), L2(Ny), L3(Nz), D1(Nx), D2(Ny), D3(Nz);
+        for (int i = 0; i < Nx; ++i) { L1[i] = (cexpi(2 * PI * (i + th[0]) / Nx) - 1.0) / h[0]; D1[i] = cexpi(th[0] * 2 * PI * i / Nx); }
+        for (int i = 0; i < Ny; ++i) { L2[i] = (cexpi(2 * PI * (i + th[1]) / Ny) - 1.0) / h[1]; D2[i] = cexpi(th[1] * 2 * PI * i / Ny); }
+        for (int i = 0; i < Nz; ++i) { L3[i] = (cexpi(2 * PI * (i + th[2]) / Nz) - 1.0) / h[2]; D3[i] = cexpi(th[2] * 2 * PI * i / Nz); }
+        T->Dk.resize(N); T->Dks.resize(N);
+        for (int z = 0; z < Nz; ++z)
+            for (int y = 0; y < Ny; ++y)
+                for (int x = 0; x < Nx; ++x) {
+                    size_t r = x + (size_t)Nx * (y + (size_t)Ny * z);
+                    Ldx[r] = L1[x]; Ldy[r] = L2[y]; Ldz[r] = L3[z];
+                    T->Dk[r] = D3[z] * (D2[y] * D1[x]);
+                    T->Dks[r] = std::conj(T->Dk[r]);
+                }
+        T->Nd = knorm < 1e-4 ? N - 1 : N;                /* :154-181 */
+    } else {
+        /* FAME_Matrix_Lambdas.cu:53-72 */
+        double c1;
+        if (a[3] < 0) c1 = 1. - (double)lat->m1 / (double)Nx;
+        else c1 = -(double)lat->m1 / (double)Nx;        /* a[3] > 0; a[3]==0 treated as > 0 */
+        double c2 = (double)lat->m3 / (double)Ny;
+        double c3 = (double)lat->m2 / (double)Nx;
+        double a2h[3], a3h[3];
+        for (int i = 0; i < 3; ++i) a2h[i] = c1 * a[i] + a[3 + i];
+        for (int i = 0; i < 3; ++i) a3h[i] = a[6 + i] - c2 * a2h[i] - c3 * a[i] + lat->t1[i];
+        double thx = dot3(k, a), thxy = dot3(k, a2h), thxyz = dot3(k, a3h);
+        /* :302-375 */
+        std::vector<cplx> L1(Nx);
+        for (int i = 0; i < Nx; ++i) L1[i] = (cexpi(2 * PI * (i + thx) / Nx) - 1.0) / h[0];
+        double cz1 = ((double)lat->m1 * (double)lat->m3) / ((double)Ny * Nx) - ((double)lat->m2 / (double)Nx);
+        double cz2 = -((double)lat->m3 / (double)Ny);
+        for (int ix = 0; ix < Nx; ++ix)
+            for (int iy = 0; iy < Ny; ++iy) {
+                double mxy = (-(double)lat->m1 / (double)Nx) * ix + iy;
+                cplx l2 = (cexpi(2 * PI * (mxy + thxy) / Ny) - 1.0) / h[1];
+                for (int iz = 0; iz < Nz; ++iz) {
+                    size_t f = ((size_t)ix * Ny + iy) * Nz + iz;
+                    double mxyz = (cz1 * ix + cz2 * iy) + iz;
+                    Ldx[f] = L1[ix];
+                    Ldy[f] = l2;
+                    Ldz[f] = (cexpi(2 * PI * (mxyz + thxyz) / Nz) - 1.0) / h[2];
+                }
+            }
+        /* D_kx, D_ky, D_kz: :377-400 */
+        T->Dkx.resize(Nx); T->Dky.resize((size_t)Nx * Ny); T->Dkz.resize(N);
+        for (int i = 0; i < Nx; ++i) T->Dkx[i] = cexpi(thx * 2 * PI * i / Nx);
+        for (int i = 0; i < Nx; ++i) {
+            double phi = 2 * PI * (thxy + i * (-(double)lat->m1 / (double)Nx)) / (double)Ny;
+            for (int j = 0; j < Ny; ++j) T->Dky[(size_t)i * Ny + j] = cexpi(phi * j);
+        }
+        for (int i = 0; i < Nx; ++i)
+            for (int j = 0; j < Ny; ++j) {
+                double phi = 2 * PI * (thxyz + j * (-(double)lat->m3 / (double)Ny) +
+                                       i * ((double)(lat->m1 * lat->m3) / (double)(Nx * Ny) - (double)lat->m2 / (double)Nx)) / (double)Nz;
+                for (int kk = 0; kk < Nz; ++kk) T->Dkz[kk + (size_t)j * Nz + (size_t)i * Ny * Nz] = cexpi(phi * kk);
+            }
+        T->Nd = knorm < 1e-3 ? N - 1 : N;                /* :402-430 */
+    }
+    build_pi(T, Ldx, Ldy, Ldz);
+    return T;
+}
+
+extern "C" void orc_tables_destroy(orc_tables* t) { delete t; }
+extern "C" int orc_tables_nd(const orc_tables* t) { return t->Nd; }
+
+extern "C" void orc_tables_get(const orc_tables* t, int which, double* out) {
+    const std::vector<cplx>* v = nullptr;
+    switch (which) {
+        case 0: v = &t->Lx; break;
+        case 1: v = &t->Ly; break;
+        case 2: v = &t->Lz; break;
+        case 3: std::memcpy(out, t->Lq.data(), sizeof(double) * t->Nd); return;
+        case 4: v = &t->PiQr; break;
+        case 5: v = &t->PiPr; break;
+        case 6: v = &t->PiQrs; break;
+        case 7: v = &t->PiPrs; break;
+        default: return;
+    }
+    std::memcpy(out, v->data(), sizeof(cplx) * v->size());
+}
+
+/* ========================================================================== */
+/*  Transforms  (FAME_FFT_CUDA.cu)                                             */
+/* ========================================================================== */
+/* General T (spMV_fastT_gpu flag +1, FAME_FFT_CUDA.cu:104-126):
+ * IFFT_z, .D_kz, IFFT_y, .D_ky, IFFT_x, .D_kx / n ; Fourier (z fastest) -> real (x fastest) */
+static void T_general(const orc_tables* T, const cplx* u, cplx* y) {
+    const int Nx = T->lat.N[0], Ny = T->lat.N[1], Nz = T->lat.N[2];
+    const size_t N = T->N;
+    std::vector<cplx> w(u, u + N);            /* layout [x][y][z] z fastest */
+    fft_axis(w.data(), Nz, Ny, Nx, 0, +1);
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) w[f] *= T->Dkz[f];
+    fft_axis(w.data(), Nz, Ny, Nx, 1, +1);
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) {
+        int iy = (f / Nz) % Ny, ix = f / ((size_t)Ny * Nz);
+        w[f] *= T->Dky[(size_t)ix * Ny + iy];
+    }
+    fft_axis(w.data(), Nz, Ny, Nx, 2, +1);
+    const double n = (double)N;
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) {
+        int iz = f % Nz, iy = (f / Nz) % Ny, ix = f / ((size_t)Ny * Nz);
+        y[ix + (size_t)Nx * (iy + (size_t)Ny * iz)] = (T->Dkx[ix] * w[f]) / n;
+    }
+}
+
+/* General T* (flag -1, FAME_FFT_CUDA.cu:128-151): exact adjoint order, unnormalised */
+static void Tstar_general(const orc_tables* T, const cplx* v, cplx* y) {
+    const int Nx = T->lat.N[0], Ny = T->lat.N[1], Nz = T->lat.N[2];
+    const size_t N = T->N;
+    std::vector<cplx> w(N);                   /* layout [x][y][z] */
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) {
+        int iz = f % Nz, iy = (f / Nz) % Ny, ix = f / ((size_t)Ny * Nz);
+        w[f] = std::conj(T->Dkx[ix]) * v[ix + (size_t)Nx * (iy + (size_t)Ny * iz)];
+    }
+    fft_axis(w.data(), Nz, Ny, Nx, 2, -1);
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) {
+        int iy = (f / Nz) % Ny, ix = f / ((size_t)Ny * Nz);
+        w[f] *= std::conj(T->Dky[(size_t)ix * Ny + iy]);
+    }
+    fft_axis(w.data(), Nz, Ny, Nx, 1, -1);
+#pragma omp parallel for num_threads(g_threads)
+    for (long f = 0; f < (long)N; ++f) w[f] *= std::conj(T->Dkz[f]);
+    fft_axis(w.data(), Nz, Ny, Nx, 0, -1);
+    std::memcpy(y, w.data(), sizeof(cplx) * N);
+}
+
+/* Simple IFFT_CUDA (:38-60): y = D_k .* IDFT3(x) / N ; FFT_CUDA (:18-36): y = DFT3(D_ks .* x) */
+static void T_simple(const orc_tables* T, const cplx* u, cplx* y) {
+    const int Nx = T->lat.N[0], Ny = T->lat.N[1], Nz = T->lat.N[2];
+    const size_t N = T->N;
+    std::vector<cplx> w(u, u + N);
+    fft_axis(w.data(), Nx, Ny, Nz, 0, +1);
+    fft_axis(w.data(), Nx, Ny, Nz, 1, +1);
+    fft_axis(w.data(), Nx, Ny, Nz, 2, +1);
+    const double alpha = 1.0 / (double)N;
+#pragma omp parallel for num_threads(g_threads)
+    for (long r = 0; r < (long)N; ++r) y[r] = (T->Dk[r] * w[r]) * alpha;
+}
+static void Tstar_simple(const orc_tables* T, const cplx* v, cplx* y) {
+    const int Nx = T->lat.N[0], Ny = T->lat.N[1], Nz = T->lat.N[2];
+    const size_t N = T->N;
+    std::vector<cplx> w(N);
+#pragma omp parallel for num_threads(g_threads)
+    for (long r = 0; r < (long)N; ++r) w[r] = T->Dks[r] * v[r];
+    fft_axis(w.data(), Nx, Ny, Nz, 0, -1);
+    fft_axis(w.data(), Nx, Ny, Nz, 1, -1);
+    fft_axis(w.data(), Nx, Ny, Nz, 2, -1);
+    std::memcpy(y, w.data(), sizeof(cplx) * N);
+}
+
+extern "C" void orc_T(const orc_tables* t, const double* u, double* y) {
+    if (t->general) T_general(t, (const cplx*)u, (cplx*)y);
+    else T_simple(t, (const cplx*)u, (cplx*)y);
+}
+extern "C" void orc_Tstar(const orc_tables* t, const double* v, double* y) {
+    if (t->general) Tstar_general(t, (const cplx*)v, (cplx*)y);
+    else Tstar_simple(t, (const cplx*)v, (cplx*)y);
+}
+
+/* Qr / Pr: Pi-expansion into the tail [N-Nd, N) of each component, then T
+ * (FAME_Matrix_Vector_Production_Qr.cu:8-78, ..._Pr.cu:11-65)               */
+static void expand_and_T(const orc_tables* t, const std::vector<cplx>& Pi, const cplx* x, cplx* y) {
+    const size_t N = t->N, Nd = t->Nd, off = N - Nd;
+    std::vector<cplx> u(N);
+    for (int c = 0; c < 3; ++c) {
+        std::fill(u.begin(), u.end(), cplx(0.0));
+        const cplx* L1 = &Pi[c * Nd];
+        const cplx* L2 = &Pi[(c + 3) * Nd];
+        for (size_t i = 0; i < Nd; ++i) u[off + i] = L1[i] * x[i] + L2[i] * x[Nd + i];
+        if (t->general) T_general(t, u.data(), y + c * N);
+        else T_simple(t, u.data(), y + c * N);
+    }
+}
+/* Qrs / Prs: T* of each component, then the Pi^H contraction
+ * (FAME_Matrix_Vector_Production_Qrs.cu:7-76, ..._Prs.cu:9-63)               */
+static void Tstar_and_contract(const orc_tables* t, const std::vector<cplx>& PiS, const cplx* v, cplx* y) {
+    const size_t N = t->N, Nd = t->Nd, off = N - Nd;
+    std::vector<cplx> w(3 * N);
+    for (int c = 0; c < 3; ++c) {
+        if (t->general) Tstar_general(t, v + c * N, w.data() + c * N);
+        else Tstar_simple(t, v + c * N, w.data() + c * N);
+    }
+    for (size_t i = 0; i < Nd; ++i) {
+        cplx y1 = 0.0, y2 = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            y1 += PiS[(2 * c) * Nd + i] * w[c * N + off + i];
+            y2 += PiS[(2 * c + 1) * Nd + i] * w[c * N + off + i];
+        }
+        y[i] = y1; y[Nd + i] = y2;
+    }
+}
+
+extern "C" void orc_qr(const orc_tables* t, const double* x, double* y) { expand_and_T(t, t->PiQr, (const cplx*)x, (cplx*)y); }
+extern "C" void orc_pr(const orc_tables* t, const double* x, double* y) { expand_and_T(t, t->PiPr, (const cplx*)x, (cplx*)y); }
+extern "C" void orc_qrs(const orc_tables* t, const double* v, double* y) { Tstar_and_contract(t, t->PiQrs, (const cplx*)v, (cplx*)y); }
+extern "C" void orc_prs(const orc_tables* t, const double* v, double* y) { Tstar_and_contract(t, t->PiPrs, (const cplx*)v, (cplx*)y); }
+
+/* QBQ = Qrs(invB .* Qr(x))  (FAME_Matrix_Vector_Production_Isotropic_QBQ.cu:8-67) */
+extern "C" void orc_qbq(const orc_tables* t, const double* invB, const double* x, double* y) {
+    const size_t N3 = 3 * (size_t)t->N;
+    std::vector<cplx> f(N3);
+    orc_qr(t, x, (double*)f.data());
+#pragma omp parallel for num_threads(g_threads)
+    for (long i = 0; i < (long)N3; ++i) f[i] *= invB[i];
+    orc_qrs(t, (const double*)f.data(), y);
+}
+
+/* C E = T (Lambda x T* E): the factorisation checked at FAME_Main_Code.cu:250-316 */
+static void curl_impl(const orc_tables* t, const cplx* E, cplx* out, bool herm) {
+    const size_t N = t->N;
+    std::vector<cplx> Ehat(3 * N);
+    for (int c = 0; c < 3; ++c) {
+        if (t->general) Tstar_general(t, E + c * N, Ehat.data() + c * N);
+        else Tstar_simple(t, E + c * N, Ehat.data() + c * N);
+    }
+    /* full-length Lambda: at Gamma mode 0 is dropped from the tables and is exactly 0 */
+    std::vector<cplx> L[3];
+    if (t->Nd == (int)N) { L[0] = t->Lx; L[1] = t->Ly; L[2] = t->Lz; }
+    else {
+        for (int c = 0; c < 3; ++c) { L[c].assign(N, cplx(0.0)); }
+        /* mode 0 at Gamma: Lambda = (exp(2 pi i theta/N)-1)/h with theta = 0 -> 0 */
+        for (size_t i = 0; i + 1 < N; ++i) { L[0][i + 1] = t->Lx[i]; L[1][i + 1] = t->Ly[i]; L[2][i + 1] = t->Lz[i]; }
+    }
+    std::vector<cplx> R(3 * N);
+    for (size_t f = 0; f < N; ++f) {
+        cplx l[3] = {L[0][f], L[1][f], L[2][f]};
+        if (herm) for (int c = 0; c < 3; ++c) l[c] = -std::conj(l[c]);
+        cplx e[3] = {Ehat[f], Ehat[N + f], Ehat[2 * N + f]};
+        R[f] = l[1] * e[2] - l[2] * e[1];
+        R[N + f] = l[2] * e[0] - l[0] * e[2];
+        R[2 * N + f] = l[0] * e[1] - l[1] * e[0];
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (t->general) T_general(t, R.data() + c * N, out + c * N);
+        else T_simple(t, R.data() + c * N, out + c * N);
+    }
+}
+extern "C" void orc_curl(const orc_tables* t, const double* E, double* CE) { curl_impl(t, (const cplx*)E, (cplx*)CE, false); }
+extern "C" void orc_curlh(const orc_tables* t, const double* F, double* CHF) { curl_impl(t, (const cplx*)F, (cplx*)CHF, true); }
+
+/* ========================================================================== */
+/*  CG and invAr  (CG.cu:5-87, ..._Isotropic_invAr.cu:8-48)                    */
+/* ========================================================================== */
+static cplx zdotc(size_t n, const cplx* a, const cplx* b) {
+    double re = 0, im = 0;
+#pragma omp parallel for reduction(+ : re, im) num_threads(g_threads)
+    for (long i = 0; i < (long)n; ++i) { cplx p = std::conj(a[i]) * b[i]; re += p.real(); im += p.imag(); }
+    return cplx(re, im);
+}
+static double dznrm2(size_t n, const cplx* a) { return std::sqrt(zdotc(n, a, a).real()); }
+static void zaxpy(size_t n, cplx alpha, const cplx* x, cplx* y) {
+#pragma omp parallel for num_threads(g_threads)
+    for (long i = 0; i < (long)n; ++i) y[i] += alpha * x[i];
+}
+static void zdscal(size_t n, double a, cplx* x) {
+#pragma omp parallel for num_threads(g_threads)
+    for (long i = 0; i < (long)n; ++i) x[i] *= a;
+}
+
+extern "C" int orc_cg(const orc_tables* t, const double* invB, const double* rhs_, double* x_,
+                      int maxit, double tol, double* final_res) {
+    const size_t dim = 2 * (size_t)t->Nd;
+    const cplx* rhs = (const cplx*)rhs_;
+    cplx* x = (cplx*)x_;
+    std::vector<cplx> r(rhs, rhs + dim), p(dim), Ap(dim);
+    std::fill(x, x + dim, cplx(0.0));
+    double r1 = zdotc(dim, r.data(), r.data()).real(), r0 = 0;
+    int k = 1;
+    while (r1 > tol * tol && k <= maxit) {
+        if (k > 1) {
+            double b = r1 / r0;
+            zdscal(dim, b, p.data());
+            zaxpy(dim, 1.0, r.data(), p.data());
+        } else {
+            p = r;
+        }
+        orc_qbq(t, invB, (const double*)p.data(), (double*)Ap.data());
+        cplx dot = zdotc(dim, p.data(), Ap.data());
+        double temp = dot.real() * dot.real() + dot.imag() * dot.imag();
+        cplx a(r1 * dot.real() / temp, -r1 * dot.imag() / temp);
+        zaxpy(dim, a, p.data(), x);
+        zaxpy(dim, -a, Ap.data(), r.data());
+        r0 = r1;
+        r1 = zdotc(dim, r.data(), r.data()).real();
+        ++k;
+    }
+    if (final_res) *final_res = std::sqrt(r1);
+    return k;
+}
+
+extern "C" int orc_invar(const orc_tables* t, const double* invB, const double* x_, double* y_,
+                         int maxit, double tol) {
+    const size_t Nd = t->Nd, dim = 2 * Nd;
+    const cplx* x = (const cplx*)x_;
+    cplx* y = (cplx*)y_;
+    std::vector<cplx> tmp(dim);
+    for (size_t i = 0; i < dim; ++i) tmp[i] = x[i] / t->Lq[i % Nd];
+    int it = orc_cg(t, invB, (const double*)tmp.data(), y_, maxit, tol, nullptr);
+    for (size_t i = 0; i < dim; ++i) y[i] /= t->Lq[i % Nd];
+    return it;
+}
+
+/* ========================================================================== */
+/*  Sparse curl  (FAME_Matrix_Curl.cpp:90-443)                                 */
+/* ========================================================================== */
+extern "C" int orc_curl_coo(const orc_lattice* lat, const double k[3], int* Cr, int* Cc, double* Cv_) {
+    const int n1 = lat->N[0], n2 = lat->N[1], n3 = lat->N[2];
+    const int N = n1 * n2 * n3;
+    cplx* Cv = (cplx*)Cv_;
+    std::vector<int> r1(2 * N), c1(2 * N), r2(2 * N), c2(2 * N), r3(2 * N), c3(2 * N);
+    std::vector<cplx> v1(2 * N), v2(2 * N), v3(2 * N);
+    const double* a = lat->a;
+    const double* h = lat->h;
+    cplx ika1(0, 2 * PI * dot3(a, k)), ika2(0, 2 * PI * dot3(a + 3, k)), ika3(0, 2 * PI * dot3(a + 6, k));
+    /* 1-D quasi-periodic difference, spMTX_FDM_1d_quasiperiodic :378-397 */
+    auto fdm1 = [](int dim, double hh, cplx th, std::vector<int>& R, std::vector<int>& C, std::vector<cplx>& V) {
+        R.resize(2 * dim); C.resize(2 * dim); V.resize(2 * dim);
+        double tmp = 1.0 / hh;
+        for (int i = 0; i < dim; ++i) { R[i] = i; C[i] = i; V[i] = -tmp; }
+        for (int i = dim; i < 2 * dim - 1; ++i) { R[i] = i - dim; C[i] = i - dim + 1; V[i] = tmp; }
+        R[2 * dim - 1] = dim - 1; C[2 * dim - 1] = 0; V[2 * dim - 1] = tmp * std::exp(th);
+    };
+    /* C1 = kron(I_zy, K_x)  (both lattice classes, :149-159 / :243-253) */
+    {
+        std::vector<int> R, C; std::vector<cplx> V;
+        fdm1(n1, h[0], ika1, R, C, V);
+        int idx = 0;
+        for (int i = 0; i < n3 * n2; ++i)
+            for (int j = 0; j < 2 * n1; ++j, ++idx) { r1[idx] = i * n1 + R[j]; c1[idx] = i * n1 + C[j]; v1[idx] = V[j]; }
+    }
+    if (!lat->general) {
+        /* C2 = kron(I_z, kron(K_y, I_x)) :165-196 */
+        std::vector<int> R, C; std::vector<cplx> V;
+        fdm1(n2, h[1], ika2, R, C, V);
+        std::vector<int> tr(2 * n1 * n2), tc(2 * n1 * n2); std::vector<cplx> tv(2 * n1 * n2);
+        int idx = 0;
+        for (int i = 0; i < 2 * n2; ++i)
+            for (int j = 0; j < n1; ++j, ++idx) { tr[idx] = R[i] * n1 + j; tc[idx] = C[i] * n1 + j; tv[idx] = V[i]; }
+        idx = 0;
+        for (int i = 0; i < n3; ++i)
+            for (int j = 0; j < 2 * n1 * n2; ++j, ++idx) { r2[idx] = i * n1 * n2 + tr[j]; c2[idx] = i * n1 * n2 + tc[j]; v2[idx] = tv[j]; }
+        /* C3 = kron(K_z, I_yx) :202-216 */
+        fdm1(n3, h[2], ika3, R, C, V);
+        idx = 0;
+        for (int i = 0; i < 2 * n3; ++i)
+            for (int j = 0; j < n1 * n2; ++j, ++idx) { r3[idx] = R[i] * n1 * n2 + j; c3[idx] = C[i] * n1 * n2 + j; v3[idx] = V[i]; }
+    } else {
+        cplx ikt1(0, 2 * PI * dot3(lat->t1, k)), ikt2(0, 2 * PI * dot3(lat->t2, k));
+        cplx ikt3(0, 2 * PI * dot3(lat->t3, k)), ikt4(0, 2 * PI * dot3(lat->t4, k));
+        const int m1 = lat->m1, m2 = lat->m2, m3 = lat->m3, m4 = lat->m4;
+        /* K_2: spMTX_FDM_2d_quasiperiodic(n2, n1, hy, 0) + J2 block, :259-280 */
+        int L = n2 * n1;
+        std::vector<int> R(2 * L), C(2 * L); std::vector<cplx> V(2 * L);
+        double tmp = 1.0 / h[1];
+        for (int i = 0; i < L; ++i) { R[i] = i; C[i] = i; V[i] = -tmp; }
+        for (int i = L; i < 2 * L - n1; ++i) { R[i] = i - L; C[i] = i - L + n1; V[i] = tmp; }
+        for (int i = 2 * L - n1; i < 2 * L; ++i) { R[i] = i - (2 * L - n1) + (L - n1); C[i] = i - (2 * L - n1); V[i] = tmp; }
+        /* rho_1 = (theta_3 > pi/2); the computational a2 = |a2|(cos theta_3, sin theta_3, 0),
+         * so theta_3 > pi/2 exactly when a2(1) < 0 (:265-267) */
+        int rho_1 = lat->a[3] < 0 ? 1 : 0;
+        for (int i = 2 * L - n1; i < 2 * L - n1 + m1; ++i) {
+            R[i] = i - (2 * L - n1) + (L - n1);
+            C[i] = i - (2 * L - n1) + (n1 - m1);
+            V[i] = std::exp(ika2 + ika1 * (double)(rho_1 - 1)) / h[1];
+        }
+        for (int i = 2 * L - n1 + m1; i < 2 * L; ++i) {
+            R[i] = i - (2 * L - n1 + m1) + (L - n1) + m1;
+            C[i] = i - (2 * L - n1 + m1);
+            V[i] = std::exp(ika2 + ika1 * (double)rho_1) / h[1];
+        }
+        int idx = 0;
+        for (int i = 0; i < n3; ++i)
+            for (int j = 0; j < 2 * L; ++j, ++idx) { r2[idx] = i * L + R[j]; c2[idx] = i * L + C[j]; v2[idx] = V[j]; }
+        /* C3: spMTX_FDM_3d_quasiperiodic(n3, n2, n1, hz, 0) then J31/J32, :296-373 */
+        int NN = n3 * n2 * n1, P = n2 * n1;
+        double tz = 1.0 / h[2];
+        for (int i = 0; i < NN; ++i) { r3[i] = i; c3[i] = i; v3[i] = -tz; }
+        for (int i = NN; i < 2 * NN - P; ++i) { r3[i] = i - NN; c3[i] = i - NN + P; v3[i] = tz; }
+        std::vector<int> J31r, J31c, J32r, J32c; std::vector<cplx> J31v, J32v;
+        std::vector<int> tr(n1), tc(n1); std::vector<cplx> tv(n1);
+        for (int i = 0; i < m4; ++i) { tr[i] = i; tc[i] = i + n1 - m4; tv[i] = std::exp(ika3 + ikt3) / h[2]; }
+        for (int i = m4; i < n1; ++i) { tr[i] = i; tc[i] = i - m4; tv[i] = std::exp(ika3 + ikt4) / h[2]; }
+        for (int b = 0; b < m3; ++b)
+            for (int j = 0; j < n1; ++j) { J31r.push_back(b * n1 + tr[j]); J31c.push_back(b * n1 + tc[j]); J31v.push_back(tv[j]); }
+        for (int i = 0; i < m2; ++i) { tr[i] = i; tc[i] = i + n1 - m2; tv[i] = std::exp(ika3 + ikt2) / h[2]; }
+        for (int i = m2; i < n1; ++i) { tr[i] = i; tc[i] = i - m2; tv[i] = std::exp(ika3 + ikt1) / h[2]; }
+        for (int b = 0; b < n2 - m3; ++b)
+            for (int j = 0; j < n1; ++j) { J32r.push_back(b * n1 + tr[j]); J32c.push_back(b * n1 + tc[j]); J32v.push_back(tv[j]); }
+        int s0 = 2 * NN - P;
+        for (int i = s0; i < s0 + m3 * n1; ++i) {
+            r3[i] = J31r[i - s0] + NN - P;
+            c3[i] = J31c[i - s0] + (n2 - m3) * n1;
+            v3[i] = J31v[i - s0];
+        }
+        for (int i = s0 + m3 * n1; i < 2 * NN; ++i) {
+            r3[i] = J32r[i - (s0 + m3 * n1)] + NN - P + m3 * n1;
+            c3[i] = J32c[i - (s0 + m3 * n1)];
+            v3[i] = J32v[i - (s0 + m3 * n1)];
+        }
+    }
+    /* block assembly :105-129 */
+    for (int i = 0; i < 2 * N; ++i) {
+        Cr[i] = r1[i] + N;          Cc[i] = c1[i] + 2 * N;      Cv[i] = -v1[i];
+        Cr[i + 2 * N] = r1[i] + 2 * N; Cc[i + 2 * N] = c1[i] + N; Cv[i + 2 * N] = v1[i];
+        Cr[i + 4 * N] = r2[i];      Cc[i + 4 * N] = c2[i] + 2 * N; Cv[i + 4 * N] = v2[i];
+        Cr[i + 6 * N] = r2[i] + 2 * N; Cc[i + 6 * N] = c2[i];   Cv[i + 6 * N] = -v2[i];
+        Cr[i + 8 * N] = r3[i];      Cc[i + 8 * N] = c3[i] + N;  Cv[i + 8 * N] = -v3[i];
+        Cr[i + 10 * N] = r3[i] + N; Cc[i + 10 * N] = c3[i];     Cv[i + 10 * N] = v3[i];
+    }
+    return 12 * N;
+}
+
+/* ========================================================================== */
+/*  Tridiagonal eigensolver (replaces LAPACKE_zpteqr, Lanczos_Isotropic.cu:53) */
+/*  Implicit QL with Wilkinson-type shift (textbook method).                   */
+/* ========================================================================== */
+extern "C" int orc_tridiag_eig(int n, const double* d_in, const double* e_in, double* w, double* z) {
+    std::vector<double> d(d_in, d_in + n), e(n, 0.0);
+    for (int i = 0; i + 1 < n; ++i) e[i] = e_in[i];
+    std::vector<double> Z((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) Z[(size_t)i * n + i] = 1.0;
+    for (int l = 0; l < n; ++l) {
+        int iter = 0, m;
+        do {
+            for (m = l; m < n - 1; ++m) {
+                double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= 1e-300 || std::fabs(e[m]) <= 2.220446049250313e-16 * dd) break;
+            }
+            if (m != l) {
+                if (iter++ == 200) return -1;
+                double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+                double r = std::hypot(g, 1.0);
+                g = d[m] - d[l] + e[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
+                double s = 1.0, c = 1.0, p = 0.0;
+                int i;
+                bool underflow = false;
+                for (i = m - 1; i >= l; --i) {
+                    double f = s * e[i], b = c * e[i];
+                    r = std::hypot(f, g);
+                    e[i + 1] = r;
+                    if (r == 0.0) { d[i + 1] -= p; e[m] = 0.0; underflow = true; break; }
+                    s = f / r; c = g / r;
+                    g = d[i + 1] - p;
+                    r = (d[i] - g) * s + 2.0 * c * b;
+                    p = s * r;
+                    d[i + 1] = g + p;
+                    g = c * r - b;
+                    double* zi = &Z[(size_t)i * n];
+                    double* zi1 = &Z[(size_t)(i + 1) * n];
+                    for (int kk = 0; kk < n; ++kk) {
+                        double fz = zi1[kk];
+                        zi1[kk] = s * zi[kk] + c * fz;
+                        zi[kk] = c * zi[kk] - s * fz;
+                    }
+                }
+                if (underflow) continue;
+                d[l] -= p; e[l] = g; e[m] = 0.0;
+            }
+        } while (m != l);
+    }
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] > d[y]; });
+    for (int j = 0; j < n; ++j) {
+        w[j] = d[idx[j]];
+        for (int i = 0; i < n; ++i) z[(size_t)j * n + i] = Z[(size_t)idx[j] * n + i];
+    }
+    return 0;
+}
+
+/* Givens with the reference's sign convention, Lanczos_LockPurge_gpu.cu:245-266 */
+static void given_rotation(double* c, double* s, double* r, double a, double b) {
+    double t, u;
+    if (std::fabs(a) > std::fabs(b)) {
+        t = b / a; u = std::fabs(a) / a * std::sqrt(1 + t * t);
+        *c = 1.0 / u; *s = *c * t; *r = a * u; *s = -*s;
+    } else {
+        t = a / b; u = std::fabs(b) / b * std::sqrt(1 + t * t);
+        *s = 1.0 / u; *c = *s * t; *r = b * u; *s = -*s;
+    }
+}
+
+/* GVqrrq_g, Lanczos_LockPurge_gpu.cu:95-190 */
+extern "C" void orc_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s,
+                           double shift, int n) {
+    double a, b, r, R[6];
+    std::memcpy(T3, T1, (n - 1) * sizeof(double));
+    for (int i = 0; i < n; ++i) T0[i] -= shift;
+    for (int i = 0; i < n - 1; ++i) {
+        a = T0[i]; b = T1[i];
+        given_rotation(&c[i], &s[i], &r, a, b);
+        T0[i] = r; T1[i] = 0.0;
+        a = T3[i]; b = T0[i + 1];
+        T3[i] = c[i] * a - s[i] * b;
+        T0[i + 1] = s[i] * a + c[i] * b;
+        if (i < n - 2) { T2[i] = -s[i] * T3[i + 1]; T3[i + 1] = c[i] * T3[i + 1]; }
+    }
+    R[0] = T0[0]; R[1] = T3[0]; R[2] = T1[0]; R[3] = T0[1];
+    T0[0] = c[0] * R[0] - s[0] * R[1];
+    T3[0] = s[0] * R[0] + c[0] * R[1];
+    T1[0] = c[0] * R[2] - s[0] * R[3];
+    T0[1] = s[0] * R[2] + c[0] * R[3];
+    for (int i = 1; i < n - 1; ++i) {
+        R[0] = T3[i - 1]; R[1] = T2[i - 1]; R[2] = T0[i]; R[3] = T3[i]; R[4] = T1[i]; R[5] = T0[i + 1];
+        T3[i - 1] = c[i] * R[0] - s[i] * R[1];
+        T2[i - 1] = s[i] * R[0] + c[i] * R[1];
+        T0[i] = c[i] * R[2] - s[i] * R[3];
+        T3[i] = s[i] * R[2] + c[i] * R[3];
+        T1[i] = c[i] * R[4] - s[i] * R[5];
+        T0[i + 1] = s[i] * R[4] + c[i] * R[5];
+    }
+    for (int i = 0; i < n - 1; ++i) T1[i] = (T1[i] + T3[i]) / 2.0;
+    for (int i = 0; i < n; ++i) T0[i] += shift;
+}
+
+/* ========================================================================== */
+/*  Lanczos  (Lanczos_decomp_gpu.cu, Lanczos_LockPurge_gpu.cu, Lanczos_Isotropic.cu) */
+/* ========================================================================== */
+struct LanczosState {
+    const orc_tables* t;
+    const double* invB;
+    orc_ls ls;
+    size_t Asize;
+    std::vector<cplx> U;     /* Asize x (nstep+1) column-major */
+    std::vector<double> T0, T1, T2, T3, LT0, LT1, c, s, z;
+    orc_profile* prof;
+};
+
+static void invar_col(LanczosState& S, int from, int to) {
+    cplx* U = S.U.data();
+    int it = orc_invar(S.t, S.invB, (const double*)(U + S.Asize * from), (double*)(U + S.Asize * to),
+                       S.ls.maxit, S.ls.tol);
+    S.prof->ls_iter += it;
+    S.prof->es_iter += 1;
+}
+
+static void lanczos_decomp(LanczosState& S, bool isInit, int nwant, int nstep) {
+    const size_t A = S.Asize;
+    cplx* U = S.U.data();
+    int loopStart;
+    if (!isInit) {
+        /* Lanczos_init_kernel: e_n, Lanczos_decomp_gpu.cu:93-124 */
+        std::fill(U, U + A, cplx(0.0));
+        U[A - 1] = 1.0;
+        invar_col(S, 0, 1);
+        cplx al = zdotc(A, U + A, U);
+        S.T0[0] = al.real();
+        zaxpy(A, cplx(-S.T0[0], 0.0), U, U + A);
+        S.T1[0] = dznrm2(A, U + A);
+        zdscal(A, 1.0 / S.T1[0], U + A);
+        loopStart = 1;
+    } else {
+        loopStart = nwant;
+    }
+    for (int j = loopStart; j < nstep; ++j) {
+        invar_col(S, j, j + 1);
+        cplx* uj1 = U + A * (j + 1);
+        cplx al = zdotc(A, uj1, U + A * j);
+        S.T0[j] = al.real();
+        zaxpy(A, cplx(-al.real(), 0.0), U + A * j, uj1);
+        zaxpy(A, cplx(-S.T1[j - 1], 0.0), U + A * (j - 1), uj1);
+        S.T1[j] = dznrm2(A, uj1);
+        zdscal(A, 1.0 / S.T1[j], uj1);
+        for (int i = 0; i <= j; ++i) {                 /* :177-183 */
+            cplx loss = zdotc(A, U + A * i, uj1);
+            zaxpy(A, -loss, U + A * i, uj1);
+        }
+    }
+}
+
+/* cublasZdrot(x, y, c, s): x' = c x + s y ; y' = -s x + c y */
+static void zdrot(size_t n, cplx* x, cplx* y, double c, double s) {
+#pragma omp parallel for num_threads(g_threads)
+    for (long i = 0; i < (long)n; ++i) {
+        cplx xv = x[i], yv = y[i];
+        x[i] = c * xv + s * yv;
+        y[i] = -s * xv + c * yv;
+    }
+}
+
+static void lock_purge(LanczosState& S, int nwant, int nstep) {
+    const size_t A = S.Asize;
+    cplx* U = S.U.data();
+    double *T0 = S.T0.data(), *T1 = S.T1.data(), *T2 = S.T2.data(), *T3 = S.T3.data();
+    double *c = S.c.data(), *s = S.s.data();
+    for (int j = 0; j < nstep - nwant; ++j) {
+        orc_gvqrrq(T0, T1, T2, T3, c, s, S.LT0[nwant + j], nstep - j);
+        int jj;
+        for (jj = 0; jj < nstep - 1 - j; ++jj) zdrot(A, U + A * (jj + 1), U + A * jj, c[jj], s[jj]);
+        s[jj - 1] *= T1[jj];
+        zdrot(A, U + A * (jj + 1), U + A * jj, T1[jj - 1], s[jj - 1]);
+        T1[jj - 1] = dznrm2(A, U + A * jj);
+        zdscal(A, 1.0 / T1[jj - 1], U + A * jj);
+        T0[jj] = 0.0; T1[jj] = 0.0;
+        ++jj;
+        std::fill(U + A * jj, U + A * (jj + 1), cplx(0.0));
+    }
+}
+
+/* Lanczos_Isotropic.cu:7-96 ; returns iteration count, ev = U z (Asize x nwant) */
+static int lanczos_isotropic(LanczosState& S, int nwant, int nstep, int maxit, double tol,
+                             double* freq, std::vector<cplx>& ev) {
+    lanczos_decomp(S, false, nwant, nstep);
+    int iter;
+    for (iter = 1; iter <= maxit; ++iter) {
+        std::vector<double> w(nstep);
+        if (orc_tridiag_eig(nstep, S.T0.data(), S.T1.data(), w.data(), S.z.data()) != 0) {
+            std::fprintf(stderr, "oracle: tridiagonal eigensolver failed\n");
+            std::abort();
+        }
+        S.LT0 = w;
+        int conv = 0;
+        for (int i = 0; i < nwant; ++i) {
+            double res = S.T1[nstep - 1] * std::fabs(S.z[(size_t)(i + 1) * nstep - 1]);
+            if (res < tol) conv++; else break;
+        }
+        if (conv == nwant) break;
+        lock_purge(S, nwant, nstep);
+        lanczos_decomp(S, true, nwant, nstep);
+    }
+    for (int i = 0; i < nwant; ++i) freq[i] = std::sqrt(1.0 / S.LT0[i]);
+    const size_t A = S.Asize;
+    ev.assign(A * nwant, cplx(0.0));
+    for (int j = 0; j < nwant; ++j)
+        for (int l = 0; l < nstep; ++l) {
+            double zz = S.z[(size_t)j * nstep + l];
+            const cplx* ul = S.U.data() + A * l;
+            cplx* e = ev.data() + A * j;
+#pragma omp parallel for num_threads(g_threads)
+            for (long i = 0; i < (long)A; ++i) e[i] += ul[i] * zz;
+        }
+    return iter;
+}
+
+extern "C" double orc_rel_residual(const orc_tables* t, const double* eps, const double* E_, double omega) {
+    const size_t N3 = 3 * (size_t)t->N;
+    const cplx* E = (const cplx*)E_;
+    std::vector<cplx> CE(N3), CHCE(N3);
+    curl_impl(t, E, CE.data(), false);
+    curl_impl(t, CE.data(), CHCE.data(), true);
+    double num = 0, den = 0, w2 = omega * omega;
+    for (size_t i = 0; i < N3; ++i) {
+        cplx r = CHCE[i] - w2 * eps[i] * E[i];
+        num += std::norm(r); den += std::norm(E[i]);
+    }
+    return den > 0 ? std::sqrt(num / den) : 0.0;
+}
+
+extern "C" int orc_eigs(const orc_lattice* lat, const double* eps, const double k[3],
+                        const orc_es* es, const orc_ls* ls, double* freq, double* ev_out,
+                        orc_profile* prof) {
+    orc_profile dummy{0, 0, 0, 0};
+    if (!prof) prof = &dummy;
+    prof->ls_iter = prof->es_iter = prof->restarts = 0;
+    orc_tables* t = orc_tables_create(lat, k);
+    const size_t N = t->N, N3 = 3 * N, Nd = t->Nd;
+    std::vector<double> invB(N3);
+    for (size_t i = 0; i < N3; ++i) invB[i] = 1.0 / eps[i];
+    LanczosState S;
+    S.t = t; S.invB = invB.data(); S.ls = *ls; S.Asize = 2 * Nd; S.prof = prof;
+    const int nstep = es->nstep, nwant = es->nwant;
+    S.U.assign(S.Asize * (nstep + 1), cplx(0.0));
+    S.T0.assign(nstep + 1, 0.0); S.T1.assign(nstep + 1, 0.0); S.T2.assign(nstep + 1, 0.0);
+    S.T3.assign(nstep + 1, 0.0); S.LT0.assign(nstep, 0.0); S.LT1.assign(nstep, 0.0);
+    S.c.assign(nstep + 1, 0.0); S.s.assign(nstep + 1, 0.0); S.z.assign((size_t)nstep * nstep, 0.0);
+    std::vector<cplx> ev;
+    prof->restarts = lanczos_isotropic(S, nwant, nstep, es->maxit, es->tol, freq, ev);
+    prof->nd = (int)Nd;
+    /* Eigen_Restoration_Isotropic + Gamma rule, FAME_Fast_Algorithms_Isotropic.cu:57-125 */
+    if (ev_out) {
+        cplx* E = (cplx*)ev_out;
+        std::vector<cplx> y(2 * Nd);
+        for (int i = 0; i < nwant; ++i) {
+            for (size_t q = 0; q < 2 * Nd; ++q) y[q] = ev[(size_t)i * 2 * Nd + q] * t->Lq[q % Nd];
+            orc_qr(t, (const double*)y.data(), (double*)(E + N3 * i));
+            for (size_t q = 0; q < N3; ++q) E[N3 * i + q] *= invB[q];
+            double nrm = dznrm2(N3, E + N3 * i);
+            zdscal(N3, nrm, E + N3 * i);                 /* reference quirk: times the norm */
+        }
+        if ((int)Nd == (int)N - 1) {
+            for (int i = nwant - 1; i >= 2; --i) {
+                std::swap_ranges(E + N3 * i, E + N3 * (i + 1), E + N3 * (i - 2));
+                freq[i] = freq[i - 2];
+            }
+            freq[0] = freq[1] = 0.0;
+            double c0 = 1.0 / std::sqrt((double)N3);
+            for (size_t q = 0; q < 2 * N3; ++q) E[q] = cplx(c0, 0.0);
+        }
+    } else if ((int)Nd == (int)N - 1) {
+        for (int i = nwant - 1; i >= 2; --i) freq[i] = freq[i - 2];
+        freq[0] = freq[1] = 0.0;
+    }
+    orc_tables_destroy(t);
+    return 0;
+}
