@@ -1,0 +1,536 @@
+/*
+ * fame_blas.hip -- batched vector, reduction and small-GEMM kernels for the
+ * CG (CG.cu:5-87) and Lanczos (Lanczos_decomp_gpu.cu, Lanczos_LockPurge_gpu.cu,
+ * Lanczos_Isotropic.cu:93) layers.  Every kernel works on a batch of problems
+ * (blockIdx.y) and honours a per-problem mask, so converged problems cost
+ * nothing.  Scalars (alpha, beta, norms) stay on the device: no host round trip
+ * per BLAS-1 call as with CUBLAS_POINTER_MODE_HOST in the reference.
+ *
+ * Reductions are two-stage: blocks write partial sums to `partials`
+ * [problem][block][ncomp], one block per problem then sums them in a fixed
+ * order (deterministic results run to run).
+ */
+#include "fame_kernels.h"
+
+using namespace fame;
+
+#define RB 256   /* threads per block for vector kernels */
+
+template <typename T> struct V2;
+template <> struct V2<double> { typedef double2 t; };
+template <> struct V2<float> { typedef float2 t; };
+
+__device__ __forceinline__ double2 d2(double2 v) { return v; }
+__device__ __forceinline__ double2 d2(float2 v) { return make_double2(v.x, v.y); }
+
+/* block sum of `ncomp` doubles per thread -> partials[p][blockIdx.x][c] */
+template <int NC>
+__device__ __forceinline__ void block_partials(double (&v)[NC], double* partials, int p) {
+    __shared__ double red[NC][RB];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) red[c][threadIdx.x] = v[c];
+    __syncthreads();
+    for (int s = RB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        double* o = partials + ((size_t)p * gridDim.x + blockIdx.x) * NC;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) o[c] = red[c][0];
+    }
+}
+
+/* ---------------------------------------------------------------- CG ---- */
+template <typename T>
+__global__ void __launch_bounds__(RB) k_cg_begin(typename V2<T>::t* x, typename V2<T>::t* r, typename V2<T>::t* pv,
+                                                 const typename V2<T>::t* b, double* partials, size_t n, size_t stride,
+                                                 const int* mask) {
+    typedef typename V2<T>::t T2;
+    const int p = blockIdx.y;
+    double acc[1] = {0.0};
+    if (!mask || mask[p]) {
+        const size_t o = (size_t)p * stride;
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+            T2 v = b[o + i];
+            T2 z; z.x = 0; z.y = 0;
+            x[o + i] = z; r[o + i] = v; pv[o + i] = v;
+            T s = v.x * v.x + v.y * v.y;
+            acc[0] += (double)s;
+        }
+    }
+    block_partials<1>(acc, partials, p);
+}
+
+__global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int* active,
+                               const int* mask) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
+    CgScal c;
+    c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
+    c.tol2 = tol * tol; c.iters = 1; c.maxit = maxit;
+    c.active = (!mask || mask[p]) && (s > c.tol2) && (1 <= maxit);
+    c.pad = 0;
+    sc[p] = c;
+    active[p] = c.active;
+}
+
+__global__ void k_cg_alpha(CgScal* sc, const double* partials, int nblk, const int* active) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0 || !active[p]) return;
+    double re = 0, im = 0;
+    for (int b = 0; b < nblk; ++b) { re += partials[((size_t)p * nblk + b) * 2]; im += partials[((size_t)p * nblk + b) * 2 + 1]; }
+    CgScal& c = sc[p];
+    c.pap_re = re; c.pap_im = im;
+    /* a = r1 / dot  (CG.cu:60-63) */
+    double temp = re * re + im * im;
+    c.alpha_re = c.rr * re / temp;
+    c.alpha_im = -c.rr * im / temp;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RB) k_cg_update_xr(typename V2<T>::t* x, typename V2<T>::t* r,
+                                                     const typename V2<T>::t* pv, const typename V2<T>::t* Ap,
+                                                     const CgScal* sc, double* partials, size_t n, size_t stride,
+                                                     const int* active) {
+    typedef typename V2<T>::t T2;
+    const int p = blockIdx.y;
+    T acc = 0;
+    if (active[p]) {
+        const T ar = (T)sc[p].alpha_re, ai = (T)sc[p].alpha_im;
+        const size_t o = (size_t)p * stride;
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+            T2 pp = pv[o + i], ap = Ap[o + i], xx = x[o + i], rr = r[o + i];
+            xx.x += ar * pp.x - ai * pp.y; xx.y += ar * pp.y + ai * pp.x;
+            rr.x -= ar * ap.x - ai * ap.y; rr.y -= ar * ap.y + ai * ap.x;
+            x[o + i] = xx; r[o + i] = rr;
+            acc += rr.x * rr.x + rr.y * rr.y;
+        }
+    }
+    double a2[1] = {(double)acc};
+    block_partials<1>(a2, partials, p);
+}
+
+__global__ void k_cg_beta(CgScal* sc, const double* partials, int nblk, int* active) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0 || !active[p]) return;
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
+    CgScal& c = sc[p];
+    c.beta = s / c.rr;
+    c.rr = s;
+    c.iters += 1;
+    c.active = (s > c.tol2) && (c.iters <= c.maxit);
+    active[p] = c.active;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RB) k_cg_update_p(typename V2<T>::t* pv, const typename V2<T>::t* r, const CgScal* sc,
+                                                    size_t n, size_t stride, const int* active) {
+    typedef typename V2<T>::t T2;
+    const int p = blockIdx.y;
+    if (!active[p]) return;
+    const T be = (T)sc[p].beta;
+    const size_t o = (size_t)p * stride;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        T2 pp = pv[o + i], rr = r[o + i];
+        pp.x = rr.x + be * pp.x; pp.y = rr.y + be * pp.y;
+        pv[o + i] = pp;
+    }
+}
+
+/* ------------------------------------------------------------ helpers ---- */
+template <typename T>
+__global__ void k_scale_real(typename V2<T>::t* y, const typename V2<T>::t* x, const double* s, size_t n,
+                             size_t xstride, size_t sstride, size_t smod) {
+    typedef typename V2<T>::t T2;
+    const int p = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        T2 v = x[(size_t)p * xstride + i];
+        T f = (T)s[(size_t)p * sstride + (i % smod)];
+        v.x *= f; v.y *= f;
+        y[(size_t)p * xstride + i] = v;
+    }
+}
+
+template <typename TD, typename TS>
+__global__ void k_convert(typename V2<TD>::t* d, const typename V2<TS>::t* s, size_t n, size_t dstride, size_t sstride) {
+    const int p = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        typename V2<TS>::t v = s[(size_t)p * sstride + i];
+        typename V2<TD>::t w; w.x = (TD)v.x; w.y = (TD)v.y;
+        d[(size_t)p * dstride + i] = w;
+    }
+}
+
+template <typename TS>
+__global__ void k_axpy_mixed(double2* y, const typename V2<TS>::t* x, double alpha, size_t n, size_t ystride,
+                             size_t xstride, const int* mask) {
+    const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        double2 v = d2(x[(size_t)p * xstride + i]);
+        double2 w = y[(size_t)p * ystride + i];
+        w.x += alpha * v.x; w.y += alpha * v.y;
+        y[(size_t)p * ystride + i] = w;
+    }
+}
+
+__global__ void k_sub(double2* r, const double2* b, const double2* ax, size_t n, size_t stride, const int* mask) {
+    const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        double2 u = b[(size_t)p * stride + i], v = ax[(size_t)p * stride + i];
+        r[(size_t)p * stride + i] = make_double2(u.x - v.x, u.y - v.y);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RB) k_norm2(const typename V2<T>::t* x, double* partials, size_t n, size_t stride) {
+    const int p = blockIdx.y;
+    double acc[1] = {0.0};
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        double2 v = d2(x[(size_t)p * stride + i]);
+        acc[0] += v.x * v.x + v.y * v.y;
+    }
+    block_partials<1>(acc, partials, p);
+}
+
+__global__ void k_reduce_partials(const double* partials, int nblk, int ncomp, double* out) {
+    const int p = blockIdx.x;
+    for (int c = threadIdx.x; c < ncomp; c += blockDim.x) {
+        double s = 0;
+        for (int b = 0; b < nblk; ++b) s += partials[((size_t)p * nblk + b) * ncomp + c];
+        out[(size_t)p * ncomp + c] = s;
+    }
+}
+
+/* ------------------------------------------------------------ Lanczos ---- */
+/* h[j] = sum_i conj(V[j][i]) w[i], j < m  (partials of 2m doubles per block) */
+__global__ void __launch_bounds__(RB) k_gemv_h(const double2* V, const double2* w, double* partials, int m, size_t n,
+                                               size_t cstride, size_t pstride, size_t wstride, const int* mask) {
+    const int p = blockIdx.y;
+    __shared__ double red[RB];
+    const bool on = !mask || mask[p];
+    const double2* Vp = V + (size_t)p * pstride;
+    const double2* wp = w + (size_t)p * wstride;
+    for (int j = 0; j < m; ++j) {
+        double re = 0, im = 0;
+        if (on) {
+            for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+                double2 v = Vp[(size_t)j * cstride + i], x = wp[i];
+                re += v.x * x.x + v.y * x.y;
+                im += v.x * x.y - v.y * x.x;
+            }
+        }
+        for (int c = 0; c < 2; ++c) {
+            red[threadIdx.x] = c == 0 ? re : im;
+            __syncthreads();
+            for (int s = RB / 2; s > 0; s >>= 1) {
+                if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) partials[((size_t)p * gridDim.x + blockIdx.x) * 2 * m + 2 * j + c] = red[0];
+            __syncthreads();
+        }
+    }
+}
+
+/* w[i] -= sum_j V[j][i] h[j]  (h complex, 2m doubles per problem) */
+__global__ void k_gemv_n(const double2* V, double2* w, const double* h, int m, size_t n, size_t cstride,
+                         size_t pstride, size_t wstride, size_t hstride, const int* mask) {
+    const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
+    const double2* Vp = V + (size_t)p * pstride;
+    double2* wp = w + (size_t)p * wstride;
+    const double* hp = h + (size_t)p * hstride;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        double2 acc = wp[i];
+        for (int j = 0; j < m; ++j) {
+            double2 v = Vp[(size_t)j * cstride + i];
+            double hr = hp[2 * j], hi = hp[2 * j + 1];
+            acc.x -= v.x * hr - v.y * hi;
+            acc.y -= v.x * hi + v.y * hr;
+        }
+        wp[i] = acc;
+    }
+}
+
+/* Gram matrix G[j][l] = sum_i conj(V[j][i]) V[l][i] over the upper triangle.
+ * A block stages GR rows of all m columns in LDS; each thread owns a fixed set
+ * of (j,l) pairs and accumulates over the staged rows.  Partial per block:
+ * 2 m^2 doubles (lower triangle left zero).                                  */
+#define GR 32
+__global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials, int m, size_t n, size_t cstride,
+                                             size_t pstride, const int* mask) {
+    const int p = blockIdx.y;
+    extern __shared__ double2 gtile[];   /* m x GR */
+    const bool on = !mask || mask[p];
+    const double2* Vp = V + (size_t)p * pstride;
+    const int npair = m * (m + 1) / 2;
+    const int PPT = 8;                   /* pairs per thread (m <= 64 -> 2080 pairs <= 8*256) */
+    int pj[PPT], pl[PPT];
+    double accr[PPT], acci[PPT];
+    for (int q = 0; q < PPT; ++q) {
+        int t = threadIdx.x + q * RB;
+        pj[q] = -1; pl[q] = -1; accr[q] = 0; acci[q] = 0;
+        if (t < npair) {
+            int j = 0, rem = t;
+            while (rem >= m - j) { rem -= m - j; ++j; }
+            pj[q] = j; pl[q] = j + rem;
+        }
+    }
+    const size_t chunk = (size_t)gridDim.x * GR;
+    for (size_t base = (size_t)blockIdx.x * GR; on && base < n; base += chunk) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < m * GR; t += RB) {
+            int j = t / GR, r = t - j * GR;
+            size_t i = base + r;
+            gtile[t] = (i < n) ? Vp[(size_t)j * cstride + i] : make_double2(0, 0);
+        }
+        __syncthreads();
+        for (int q = 0; q < PPT; ++q) {
+            if (pj[q] < 0) continue;
+            const double2* a = gtile + pj[q] * GR;
+            const double2* b = gtile + pl[q] * GR;
+            double re = accr[q], im = acci[q];
+#pragma unroll 8
+            for (int r = 0; r < GR; ++r) {
+                double2 x = a[r], y = b[r];
+                re += x.x * y.x + x.y * y.y;
+                im += x.x * y.y - x.y * y.x;
+            }
+            accr[q] = re; acci[q] = im;
+        }
+    }
+    double* o = partials + ((size_t)p * gridDim.x + blockIdx.x) * 2 * m * m;
+    for (int t = threadIdx.x; t < 2 * m * m; t += RB) o[t] = 0;
+    __syncthreads();
+    for (int q = 0; q < PPT; ++q) {
+        if (pj[q] < 0) continue;
+        o[2 * (pj[q] * m + pl[q])] = accr[q];
+        o[2 * (pj[q] * m + pl[q]) + 1] = acci[q];
+    }
+}
+
+/* W[l][i] = sum_j V[j][i] Q[j][l], Q real m x q (column-major, leading dim m).
+ * Output columns are produced 16 at a time; Q is staged in LDS.              */
+__global__ void __launch_bounds__(RB) k_gemm_vq(const double2* V, const double* Q, double2* W, int m, int q, size_t n,
+                                                size_t cstride, size_t pstride, size_t wcstride, size_t wpstride,
+                                                size_t qstride, const int* mask) {
+    const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
+    extern __shared__ double qs[];
+    const double2* Vp = V + (size_t)p * pstride;
+    double2* Wp = W + (size_t)p * wpstride;
+    const double* Qp = Q + (size_t)p * qstride;
+    for (int t = threadIdx.x; t < m * q; t += RB) qs[t] = Qp[t];
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        for (int l0 = 0; l0 < q; l0 += 16) {
+            double2 acc[16];
+#pragma unroll
+            for (int l = 0; l < 16; ++l) acc[l] = make_double2(0, 0);
+            for (int j = 0; j < m; ++j) {
+                double2 v = Vp[(size_t)j * cstride + i];
+#pragma unroll
+                for (int l = 0; l < 16; ++l) {
+                    if (l0 + l < q) {
+                        double qq = qs[(l0 + l) * m + j];
+                        acc[l].x += v.x * qq; acc[l].y += v.y * qq;
+                    }
+                }
+            }
+#pragma unroll
+            for (int l = 0; l < 16; ++l)
+                if (l0 + l < q) Wp[(size_t)(l0 + l) * wcstride + i] = acc[l];
+        }
+    }
+}
+
+/* ================================================================ host ==== */
+static inline dim3 vgrid(size_t n, int nprob, int nblk) { return dim3(nblk, nprob); }
+
+hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* partials, int nblk, size_t n, size_t stride,
+                         int nprob, int prec, const int* mask, hipStream_t s) {
+    if (prec == 8)
+        hipLaunchKernelGGL(k_cg_begin<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)x, (double2*)r, (double2*)p,
+                           (const double2*)b, partials, n, stride, mask);
+    else
+        hipLaunchKernelGGL(k_cg_begin<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x, (float2*)r, (float2*)p,
+                           (const float2*)b, partials, n, stride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob, int* active,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
+    return hipGetLastError();
+}
+hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob, const int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, active);
+    return hipGetLastError();
+}
+hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, const CgScal* sc, double* partials, int nblk,
+                             size_t n, size_t stride, int nprob, int prec, const int* active, hipStream_t s) {
+    if (prec == 8)
+        hipLaunchKernelGGL(k_cg_update_xr<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)x, (double2*)r,
+                           (const double2*)p, (const double2*)Ap, sc, partials, n, stride, active);
+    else
+        hipLaunchKernelGGL(k_cg_update_xr<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x, (float2*)r,
+                           (const float2*)p, (const float2*)Ap, sc, partials, n, stride, active);
+    return hipGetLastError();
+}
+hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, active);
+    return hipGetLastError();
+}
+hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
+                            const int* active, hipStream_t s) {
+    const int nblk = 1024;
+    if (prec == 8)
+        hipLaunchKernelGGL(k_cg_update_p<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)p, (const double2*)r, sc, n,
+                           stride, active);
+    else
+        hipLaunchKernelGGL(k_cg_update_p<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)p, (const float2*)r, sc, n,
+                           stride, active);
+    return hipGetLastError();
+}
+
+hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
+                               size_t smod, int nprob, int prec, hipStream_t s) {
+    dim3 g(1024, nprob);
+    if (prec == 8)
+        hipLaunchKernelGGL(k_scale_real<double>, g, dim3(RB), 0, s, (double2*)y, (const double2*)x, sreal, n, xstride, sstride, smod);
+    else
+        hipLaunchKernelGGL(k_scale_real<float>, g, dim3(RB), 0, s, (float2*)y, (const float2*)x, sreal, n, xstride, sstride, smod);
+    return hipGetLastError();
+}
+hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
+                           int nprob, int prec, hipStream_t s) {
+    return fame_scale_real_mod(y, x, sreal, n, xstride, sstride, n, nprob, prec, s);
+}
+hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t n, size_t dstride, size_t sstride,
+                        int nprob, hipStream_t s) {
+    dim3 g(1024, nprob);
+    if (dprec == 4 && sprec == 8)
+        hipLaunchKernelGGL((k_convert<float, double>), g, dim3(RB), 0, s, (float2*)dst, (const double2*)src, n, dstride, sstride);
+    else if (dprec == 8 && sprec == 4)
+        hipLaunchKernelGGL((k_convert<double, float>), g, dim3(RB), 0, s, (double2*)dst, (const float2*)src, n, dstride, sstride);
+    else if (dprec == 8)
+        hipLaunchKernelGGL((k_convert<double, double>), g, dim3(RB), 0, s, (double2*)dst, (const double2*)src, n, dstride, sstride);
+    else
+        hipLaunchKernelGGL((k_convert<float, float>), g, dim3(RB), 0, s, (float2*)dst, (const float2*)src, n, dstride, sstride);
+    return hipGetLastError();
+}
+hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
+                           int nprob, const int* mask, hipStream_t s) {
+    dim3 g(1024, nprob);
+    if (xprec == 8)
+        hipLaunchKernelGGL(k_axpy_mixed<double>, g, dim3(RB), 0, s, (double2*)y64, (const double2*)x, alpha, n, ystride, xstride, mask);
+    else
+        hipLaunchKernelGGL(k_axpy_mixed<float>, g, dim3(RB), 0, s, (double2*)y64, (const float2*)x, alpha, n, ystride, xstride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
+                    hipStream_t s) {
+    dim3 g(1024, nprob);
+    hipLaunchKernelGGL(k_sub, g, dim3(RB), 0, s, (double2*)r, (const double2*)b, (const double2*)ax, n, stride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,
+                               hipStream_t s) {
+    if (prec == 8)
+        hipLaunchKernelGGL(k_norm2<double>, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)x, partials, n, stride);
+    else
+        hipLaunchKernelGGL(k_norm2<float>, dim3(nblk, nprob), dim3(RB), 0, s, (const float2*)x, partials, n, stride);
+    return hipGetLastError();
+}
+hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, double* out, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nprob), dim3(256), 0, s, partials, nblk, ncomp, out);
+    return hipGetLastError();
+}
+hipError_t fame_gemv_h(const void* V, const void* w, double* partials, int nblk, int m, size_t n, size_t cstride,
+                       size_t pstride, size_t wstride, int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_gemv_h, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, (const double2*)w, partials, m, n,
+                       cstride, pstride, wstride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n, size_t cstride, size_t pstride,
+                       size_t wstride, size_t hstride, int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_gemv_n, dim3(1024, nprob), dim3(RB), 0, s, (const double2*)V, (double2*)w, h, m, n, cstride,
+                       pstride, wstride, hstride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_gram(const void* V, double* partials, int nblk, int m, size_t n, size_t cstride, size_t pstride, int nprob,
+                     const int* mask, hipStream_t s) {
+    size_t lds = sizeof(double2) * m * GR;
+    hipLaunchKernelGGL(k_gram, dim3(nblk, nprob), dim3(RB), lds, s, (const double2*)V, partials, m, n, cstride, pstride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_gemm_vq(const void* V, const double* Q, void* W, int m, int q, size_t n, size_t cstride, size_t pstride,
+                        size_t wcstride, size_t wpstride, size_t qstride, int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_gemm_vq, dim3(1024, nprob), dim3(RB), sizeof(double) * m * q, s, (const double2*)V, Q, (double2*)W, m, q, n, cstride,
+                       pstride, wcstride, wpstride, qstride, mask);
+    return hipGetLastError();
+}
+
+/* Lanczos three-term coefficients (Lanczos_decomp_gpu.cu:161-169): alpha_j is
+ * the real part of <U_j, U_{j+1}>; h receives the column weights that
+ * gemv_n subtracts: [beta_{j-1}, alpha_j] for columns (j-1, j), or [alpha_0]. */
+__global__ void k_lz_coeffs(const double* hdot, double* T0d, const double* T1d, int ld, int j, double* hout,
+                            const int* mask) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0 || (mask && !mask[p])) return;
+    const double a = hdot[2 * p];
+    T0d[(size_t)p * ld + j] = a;
+    double* h = hout + 4 * (size_t)p;
+    if (j >= 1) { h[0] = T1d[(size_t)p * ld + j - 1]; h[1] = 0; h[2] = a; h[3] = 0; }
+    else { h[0] = a; h[1] = 0; h[2] = 0; h[3] = 0; }
+}
+
+/* beta_j = ||U_{j+1}|| and its inverse (Lanczos_decomp_gpu.cu:171-174) */
+__global__ void k_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, const int* mask) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0 || (mask && !mask[p])) return;
+    const double b = sqrt(nrm2[p]);
+    T1d[(size_t)p * ld + j] = b;
+    invb[p] = 1.0 / b;
+}
+
+/* CG start scalars with an absolute tolerance and an optional relative one:
+ * stop when <r,r> <= max(tol^2, reltol^2 <b,b>)                              */
+__global__ void k_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
+                                int* active, const int* mask) {
+    const int p = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
+    CgScal c;
+    c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
+    c.tol2 = fmax(tol * tol, reltol * reltol * s); c.iters = 1; c.maxit = maxit;
+    c.active = (!mask || mask[p]) && (s > c.tol2) && (1 <= maxit);
+    c.pad = 0;
+    sc[p] = c;
+    active[p] = c.active;
+}
+
+hipError_t fame_lz_coeffs(const double* hdot, double* T0d, const double* T1d, int ld, int j, double* hout, int nprob,
+                          const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_lz_coeffs, dim3(nprob), dim3(64), 0, s, hdot, T0d, T1d, ld, j, hout, mask);
+    return hipGetLastError();
+}
+hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, int nprob, const int* mask,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_lz_beta, dim3(nprob), dim3(64), 0, s, nrm2, T1d, ld, j, invb, mask);
+    return hipGetLastError();
+}
+hipError_t fame_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
+                              int nprob, int* active, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_init_scal2, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, tol, reltol, maxit, active, mask);
+    return hipGetLastError();
+}

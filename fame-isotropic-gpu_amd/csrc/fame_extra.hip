@@ -1,0 +1,234 @@
+/*
+ * fame_extra.hip -- residual check and Hellmann-Feynman adjoint.
+ *
+ * fame_residuals: ||C^H C E - omega^2 B E|| / ||E|| on the GPU through the Fourier
+ *   factorisation C_i = T Lambda_i T* (the identity Check_Eigendecomp asserts,
+ *   FAME_Main_Code.cu:250-362): C^H C E = T( -conj(Lambda) x (Lambda x T* E) ).
+ *   It replaces the host sparse-matrix Check_Residual (FAME_Main_Code.cu:383-421)
+ *   and reports the residual relative to ||E||.
+ * fame_eig_grad: d(sum_kj w_kj lambda_kj)/d eps_e with
+ *   d lambda / d eps_e = -lambda |E_e|^2 / (E^H B E) for A E = lambda B E,
+ *   A = C^H C independent of eps, B = diag(eps).  Within a degenerate cluster
+ *   the weights are replaced by their mean, i.e. the derivative of the cluster
+ *   trace, which is the only basis-independent quantity there.
+ */
+#include "fame_hip.h"
+#include "fame_host.h"
+#include "fame_kernels.h"
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+using namespace fame;
+
+std::string& fame_err_ref();
+#define FAIL(code, msg) do { fame_err_ref() = (msg); return (code); } while (0)
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    fame_err_ref() = std::string("HIP error: ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" + std::to_string(__LINE__); \
+    return FAME_ERR_HIP; } } while (0)
+#define CHK(x) do { fame_status s_ = (x); if (s_ != FAME_OK) return s_; } while (0)
+
+/* inner pass of the curl-curl operator: F (after T* stages c and b) ->
+ * conj(twid).F -> FFT_a -> -conj(L) x (L x .) -> IFFT_a -> twid . -> F        */
+__global__ void __launch_bounds__(256) k_curlcurl(KInner args) {
+    typedef double2 T2;
+    const int p = blockIdx.y;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const GridDev g = args.g;
+    const ProbDev pr = args.prob[p];
+    const FftPlanDev& pl = args.plan;
+    const int na = g.na, L = args.L, pitch = args.pitch;
+    T2* Wl = reinterpret_cast<T2*>(smem_raw);
+    T2* A = Wl + pl.M;
+    T2* B = A + 3 * L * pitch;
+    {
+        const T2* s = reinterpret_cast<const T2*>(pl.W);
+        for (int i = threadIdx.x; i < pl.M; i += blockDim.x) Wl[i] = s[i];
+    }
+    const int nlines = g.nb * g.nc;
+    const int l0 = blockIdx.x * L;
+    const size_t N = (size_t)g.N;
+    T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
+    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+        const int line = idx / na, a = idx - line * na, gl = l0 + line;
+        T2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+        if (gl < nlines) {
+            const int bf = gl % g.nb, cf = gl / g.nb;
+            const T2 w = expipi(2.0 * (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a / (double)na);
+            for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + (size_t)gl * na + a], w);
+        }
+        for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
+    }
+    __syncthreads();
+    T2* R = fft_lines<T2, double>(A, B, 3 * L, pitch, pl, Wl, -1);
+    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+        const int line = idx / na, a = idx - line * na, gl = l0 + line;
+        if (gl >= nlines) continue;
+        const int bf = gl % g.nb, cf = gl / g.nb;
+        double2 l[3];
+        lambdas(g, pr, a, bf, cf, l[0], l[1], l[2]);
+        double2 e[3];
+        for (int c = 0; c < 3; ++c) e[c] = R[(c * L + line) * pitch + a];
+        double2 u[3];   /* u = L x e */
+        u[0] = csub(cmul(l[1], e[2]), cmul(l[2], e[1]));
+        u[1] = csub(cmul(l[2], e[0]), cmul(l[0], e[2]));
+        u[2] = csub(cmul(l[0], e[1]), cmul(l[1], e[0]));
+        double2 m[3] = {make_double2(-l[0].x, l[0].y), make_double2(-l[1].x, l[1].y), make_double2(-l[2].x, l[2].y)};
+        double2 r[3];   /* r = (-conj L) x u */
+        r[0] = csub(cmul(m[1], u[2]), cmul(m[2], u[1]));
+        r[1] = csub(cmul(m[2], u[0]), cmul(m[0], u[2]));
+        r[2] = csub(cmul(m[0], u[1]), cmul(m[1], u[0]));
+        for (int c = 0; c < 3; ++c) R[(c * L + line) * pitch + a] = r[c];
+    }
+    __syncthreads();
+    T2* S = (R == A) ? B : A;
+    R = fft_lines<T2, double>(R, S, 3 * L, pitch, pl, Wl, +1);
+    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+        const int line = idx / na, a = idx - line * na, gl = l0 + line;
+        if (gl >= nlines) continue;
+        const int bf = gl % g.nb, cf = gl / g.nb;
+        const T2 w = expipi(2.0 * (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a / (double)na);
+        for (int c = 0; c < 3; ++c) F[c * N + (size_t)gl * na + a] = cmul(R[(c * L + line) * pitch + a], w);
+    }
+}
+
+/* partials of ||F - w2 eps E||^2 and ||E||^2 (kernel layout, one problem per blockIdx.y) */
+__global__ void __launch_bounds__(256) k_resid(const double2* F, const double2* E, const double* eps, const double* w2,
+                                               double* partials, size_t n3, size_t stride) {
+    const int p = blockIdx.y;
+    double a = 0, b = 0;
+    const double om2 = w2[p];
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n3; i += (size_t)gridDim.x * 256) {
+        double2 f = F[p * stride + i], e = E[p * stride + i];
+        double s = om2 * eps[i];
+        double rx = f.x - s * e.x, ry = f.y - s * e.y;
+        a += rx * rx + ry * ry;
+        b += e.x * e.x + e.y * e.y;
+    }
+    __shared__ double ra[256], rb[256];
+    ra[threadIdx.x] = a; rb[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { ra[threadIdx.x] += ra[threadIdx.x + s]; rb[threadIdx.x] += rb[threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[((size_t)p * gridDim.x + blockIdx.x) * 2] = ra[0];
+        partials[((size_t)p * gridDim.x + blockIdx.x) * 2 + 1] = rb[0];
+    }
+}
+
+/* partials of sum eps |E|^2 for one field (reference layout) */
+__global__ void __launch_bounds__(256) k_wnorm(const double2* E, const double* eps, double* partials, size_t n3) {
+    double a = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n3; i += (size_t)gridDim.x * 256) {
+        double2 e = E[i];
+        a += eps[i] * (e.x * e.x + e.y * e.y);
+    }
+    __shared__ double ra[256];
+    ra[threadIdx.x] = a;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) ra[threadIdx.x] += ra[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = ra[0];
+}
+
+/* g += coef * |E|^2 */
+__global__ void k_grad_acc(double* g, const double2* E, double coef, size_t n3) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n3; i += (size_t)gridDim.x * 256) {
+        double2 e = E[i];
+        g[i] += coef * (e.x * e.x + e.y * e.y);
+    }
+}
+
+extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq, const void* E_dev, double* res) {
+    if (!c || !freq || !E_dev || !res || nwant < 1) FAIL(FAME_ERR_ARG, "fame_residuals: bad arguments");
+    if (!c->have_eps) FAIL(FAME_ERR_STATE, "fame_residuals: permittivity not set");
+    HIPCHK(hipSetDevice(c->device));
+    const int nk = c->nk;
+    const size_t N3 = 3 * (size_t)c->N;
+    double2* Ecopy = nullptr;
+    HIPCHK(hipMalloc(&Ecopy, 16 * N3 * nk));
+    std::vector<double> w2(nk), part;
+    const int NB = 256;
+    std::vector<double> hp((size_t)nk * NB * 2);
+    double* w2d = c->scal;
+    fame_status st = FAME_OK;
+    for (int i = 0; i < nwant && st == FAME_OK; ++i) {
+        for (int p = 0; p < nk; ++p) {
+            const double2* src = (const double2*)E_dev + ((size_t)p * nwant + i) * N3;
+            if ((st = c->field_from_ref(src, c->F64 + (size_t)p * N3, 1)) != FAME_OK) break;
+            if ((st = c->field_from_ref(src, Ecopy + (size_t)p * N3, 1)) != FAME_OK) break;
+            w2[p] = freq[(size_t)p * nwant + i] * freq[(size_t)p * nwant + i];
+        }
+        if (st != FAME_OK) break;
+        KAxis ko = c->kout(8, 0, nullptr);
+        KAxis km = c->kmid(8, 0, nullptr);
+        KInner ki = c->kin(8, 0, nullptr);
+        if (hipMemcpyAsync(w2d, w2.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            fame_launch_outer(ko, OUTER_QRS, 8, nk, c->stream) != hipSuccess ||
+            fame_launch_mid(km, -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+        dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, nk);
+        size_t lds = 16 * (ki.plan.M + 2 * 3 * ki.L * ki.pitch);
+        hipLaunchKernelGGL(k_curlcurl, grid, dim3(256), lds, c->stream, ki);
+        if (hipGetLastError() != hipSuccess || fame_launch_mid(km, +1, 8, nk, c->stream) != hipSuccess ||
+            fame_launch_outer(ko, OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+        hipLaunchKernelGGL(k_resid, dim3(NB, nk), dim3(256), 0, c->stream, c->F64, Ecopy, c->eps_int, w2d, c->partials,
+                           N3, N3);
+        if (hipMemcpyAsync(hp.data(), c->partials, sizeof(double) * nk * NB * 2, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+        for (int p = 0; p < nk; ++p) {
+            double a = 0, b = 0;
+            for (int q = 0; q < NB; ++q) { a += hp[((size_t)p * NB + q) * 2]; b += hp[((size_t)p * NB + q) * 2 + 1]; }
+            res[(size_t)p * nwant + i] = b > 0 ? std::sqrt(a / b) : 0.0;
+        }
+    }
+    (void)hipFree(Ecopy);
+    if (st == FAME_ERR_HIP) FAIL(FAME_ERR_HIP, std::string("fame_residuals: HIP error: ") + hipGetErrorString(hipGetLastError()));
+    return st;
+}
+
+extern "C" fame_status fame_eig_grad(fame_ctx* c, int nwant, const double* freq, const void* E_dev, const double* weights,
+                                     void* grad_dev) {
+    if (!c || !freq || !E_dev || !weights || !grad_dev || nwant < 1) FAIL(FAME_ERR_ARG, "fame_eig_grad: bad arguments");
+    if (!c->have_eps) FAIL(FAME_ERR_STATE, "fame_eig_grad: permittivity not set");
+    HIPCHK(hipSetDevice(c->device));
+    const int nk = c->nk;
+    const size_t N3 = 3 * (size_t)c->N;
+    const int NB = 256;
+    std::vector<double> hp(NB);
+    HIPCHK(hipMemsetAsync(grad_dev, 0, sizeof(double) * N3, c->stream));
+    for (int p = 0; p < nk; ++p) {
+        const double* f = freq + (size_t)p * nwant;
+        /* degenerate clusters: mean weight over bands whose lambda agree to 1e-8 relative */
+        std::vector<double> w(weights + (size_t)p * nwant, weights + (size_t)(p + 1) * nwant);
+        for (int i = 0; i < nwant;) {
+            int j = i + 1;
+            const double li = f[i] * f[i];
+            while (j < nwant && std::fabs(f[j] * f[j] - li) <= 1e-8 * std::max(li, 1e-300)) ++j;
+            double m = 0;
+            for (int q = i; q < j; ++q) m += w[q];
+            m /= (j - i);
+            for (int q = i; q < j; ++q) w[q] = m;
+            i = j;
+        }
+        for (int i = 0; i < nwant; ++i) {
+            const double lam = f[i] * f[i];
+            if (w[i] == 0.0 || lam == 0.0) continue;
+            const double2* E = (const double2*)E_dev + ((size_t)p * nwant + i) * N3;
+            hipLaunchKernelGGL(k_wnorm, dim3(NB), dim3(256), 0, c->stream, E, c->eps_ref, c->partials, N3);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hp.data(), c->partials, sizeof(double) * NB, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            double s = 0;
+            for (int q = 0; q < NB; ++q) s += hp[q];
+            hipLaunchKernelGGL(k_grad_acc, dim3(1024), dim3(256), 0, c->stream, (double*)grad_dev, E, -w[i] * lam / s, N3);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FAME_OK;
+}

@@ -1,0 +1,75 @@
+/*
+ * fame_host.h -- internal host-side state of a solver context (not public).
+ */
+#pragma once
+#include <vector>
+#include <hip/hip_runtime.h>
+#include "fame_hip.h"
+#include "fame_kernels.h"
+
+#define NBLK_VEC 1024   /* blocks per problem of the vector/reduction kernels */
+
+struct AxisPlan {
+    int n = 0, M = 0, blue = 0;
+    std::vector<int> radices;
+    fame::FftPlanDev dev[2];      /* [0] FP64 tables, [1] FP32 tables */
+    std::vector<void*> bufs;
+    fame_status build(int n);
+    ~AxisPlan();
+};
+
+struct fame_ctx {
+    int device = 0;
+    hipStream_t stream = 0;
+    fame_lattice lat;
+    fame::GridDev g;
+    int N = 0, maxb = 0;
+    AxisPlan pa, pb, pc;
+    int Lin[2] = {0, 0}, pin[2] = {0, 0}, Lmid[2] = {0, 0}, pmid[2] = {0, 0}, Lout[2] = {0, 0}, pout[2] = {0, 0};
+    int nblk_in[2] = {0, 0};
+    /* permittivity, kernel layout */
+    double* invB64 = nullptr; float* invB32 = nullptr; double* eps_int = nullptr; double* eps_ref = nullptr;
+    bool have_eps = false;
+    /* wave vectors */
+    int nk = 0;
+    std::vector<fame::ProbDev> prob_h;
+    std::vector<int> nd;
+    fame::ProbDev* prob_d = nullptr;
+    double* sig = nullptr;       /* nk x N  |Lambda|          */
+    double* sig_inv = nullptr;   /* nk x N  1/|Lambda| (0 at the removed Gamma mode) */
+    /* work */
+    double2* F64 = nullptr; float2* F32 = nullptr;        /* fields, 3N per problem        */
+    void* cg64[4] = {nullptr, nullptr, nullptr, nullptr};   /* r, p, Ap, spare (2N/problem) */
+    void* cg32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  /* r, p, Ap, rhs32, sol32 */
+    void* ref64[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double* partials = nullptr;   /* [problem][NBLK_VEC][<=4]   */
+    double* partials2 = nullptr;  /* [problem][inner blocks][2] */
+    CgScal* cgsc = nullptr;
+    int* active = nullptr;
+    int* mask = nullptr;
+    double* scal = nullptr;       /* 64 doubles per problem of small device scalars */
+    int* active_h = nullptr;      /* pinned */
+    std::vector<void*> allocs;
+    std::vector<void*> big;       /* Lanczos basis etc., reallocated per solve shape */
+    /* tuning / state */
+    int cg_predict[2] = {40, 40};
+    int refine_max_sweeps = 8;
+    double refine_reltol = 1e-6;
+    fame_ls ls_last = {0, 0.0};
+    long long cnt_qbq64 = 0, cnt_qbq32 = 0;
+
+    KInner kin(int prec, int p0, const int* act) const;
+    KAxis kmid(int prec, int p0, const int* act) const;
+    KAxis kout(int prec, int p0, const int* act) const;
+    fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec);
+    fame_status qr_field(const void* x, int p0, int nprob, int pikind, const int* act);
+    fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
+    fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);
+    fame_status field_from_ref(const void* fref, void* fint, int nprob);
+    fame_status cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
+                   const int* mask_dev, int* iters);
+    fame_status invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls, int* iters);
+    ~fame_ctx();
+};
+
+void fame_phases(const fame_lattice& L, const double* k, fame::ProbDev* p);

@@ -1,0 +1,108 @@
+/*
+ * fame_kernels.h -- launch interface between the host solver and the gfx950
+ * kernels (not part of the public C-ABI; see include/fame_hip.h for that).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include "fame_device.hpp"
+
+enum { OUTER_EPS = 0, OUTER_QR = 1, OUTER_QRS = 2 };
+
+struct KInner {
+    fame::GridDev g;
+    const fame::ProbDev* prob;   /* device, one per problem                    */
+    fame::FftPlanDev plan;       /* transform along the inner axis              */
+    const int* active;           /* device flags or nullptr                    */
+    const void* x;               /* EXPAND: reduced input (padded, 2N/problem) */
+    void* y;                     /* !EXPAND: reduced output                    */
+    void* f;                     /* field, 3N per problem                      */
+    const void* dotvec;          /* !EXPAND: optional d for partial conj(d).y  */
+    double* partials;            /* [problem][block][2]                        */
+    size_t xstride, fstride;
+    int L, pitch;                /* lines per block, LDS line pitch            */
+    int pikind;                  /* 0 Pi_Qr, 1 Pi_Pr                           */
+};
+
+struct KAxis {
+    fame::GridDev g;
+    const fame::ProbDev* prob;
+    fame::FftPlanDev plan;
+    const int* active;
+    void* f;
+    const void* invB;            /* outer EPS mode: 3N reals, internal layout  */
+    size_t fstride;
+    int L, pitch;
+};
+
+hipError_t fame_launch_inner(const KInner& a, int expand, int prec, int nprob, hipStream_t s);
+hipError_t fame_launch_mid(const KAxis& a, int dir, int prec, int nprob, hipStream_t s);
+hipError_t fame_launch_outer(const KAxis& a, int mode, int prec, int nprob, hipStream_t s);
+hipError_t fame_launch_xz_transpose(const void* in, void* out, int nx, int ny, int nz, int dir, size_t stride,
+                                    int ncomp, int nprob, int prec, hipStream_t s);
+hipError_t fame_launch_pack(const void* in, void* out, int N, int Nd, int dir, int prec, hipStream_t s);
+
+/* ---- vector / reduction kernels (fame_blas.hip) ------------------------- */
+struct CgScal {
+    double rr;        /* current <r,r>                                         */
+    double rr0;       /* <b,b> of the solve                                    */
+    double pap_re, pap_im;
+    double alpha_re, alpha_im;
+    double beta;
+    double tol2;      /* absolute tolerance squared                           */
+    int iters;        /* reference-style counter k (starts at 1)              */
+    int maxit;
+    int active;       /* 1 while iterating                                      */
+    int pad;
+};
+
+/* y = alpha * x (+ y if accumulate), complex vectors of n per problem, real alpha per element table */
+hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* partials, int nblk,
+                         size_t n, size_t stride, int nprob, int prec, const int* mask, hipStream_t s);
+hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob,
+                             int* active, hipStream_t s);
+hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob, const int* active, hipStream_t s);
+hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, const CgScal* sc, double* partials,
+                             int nblk, size_t n, size_t stride, int nprob, int prec, const int* active, hipStream_t s);
+hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
+hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
+                            const int* active, hipStream_t s);
+
+/* generic helpers (complex double unless prec says float) */
+hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
+                           int nprob, int prec, hipStream_t s);
+hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t n, size_t dstride, size_t sstride,
+                        int nprob, hipStream_t s);
+hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
+                           int nprob, const int* mask, hipStream_t s);
+hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
+                    hipStream_t s);
+hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,
+                               hipStream_t s);
+hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, double* out, int nprob, hipStream_t s);
+
+/* Lanczos block kernels: V columns of length n (complex double), column stride cstride,
+ * problem stride pstride */
+hipError_t fame_gemv_h(const void* V, const void* w, double* partials, int nblk, int m, size_t n, size_t cstride,
+                       size_t pstride, size_t wstride, int nprob, const int* mask, hipStream_t s);
+hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n, size_t cstride, size_t pstride,
+                       size_t wstride, size_t hstride, int nprob, const int* mask, hipStream_t s);
+hipError_t fame_gram(const void* V, double* partials, int nblk, int m, size_t n, size_t cstride, size_t pstride,
+                     int nprob, const int* mask, hipStream_t s);
+hipError_t fame_gemm_vq(const void* V, const double* Q, void* W, int m, int q, size_t n, size_t cstride,
+                        size_t pstride, size_t wcstride, size_t wpstride, size_t qstride, int nprob, const int* mask,
+                        hipStream_t s);
+hipError_t fame_lanczos_step(void* V, int j, const double* alpha_beta_prev, double* partials, int nblk, size_t n,
+                             size_t cstride, size_t pstride, int nprob, const int* mask, hipStream_t s);
+hipError_t fame_launch_sigma(const fame::GridDev& g, const fame::ProbDev* prob, double* sig, double* sig_inv, int nprob,
+                             hipStream_t s);
+
+/* Lanczos scalar helpers (fame_blas.hip) */
+hipError_t fame_lz_coeffs(const double* hdot, double* T0d, const double* T1d, int ld, int j, double* hout,
+                          int nprob, const int* mask, hipStream_t s);
+hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, int nprob, const int* mask,
+                        hipStream_t s);
+hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
+                               size_t smod, int nprob, int prec, hipStream_t s);
+hipError_t fame_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
+                              int nprob, int* active, const int* mask, hipStream_t s);

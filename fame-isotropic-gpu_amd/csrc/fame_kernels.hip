@@ -1,0 +1,439 @@
+/*
+ * fame_kernels.hip -- gfx950 kernels of the null-space-free Maxwell band solver.
+ *
+ * One QBQ = Q_r^H B^{-1} Q_r apply (FAME_Matrix_Vector_Production_Isotropic_QBQ.cu)
+ * is five axis passes over a batch of problems:
+ *   P1 inner  : Pi expansion of the reduced vector + inverse FFT along a + inner twiddle
+ *   P2 middle : inverse FFT along b + middle twiddle
+ *   P3 outer  : inverse FFT along c, 1/eps and 1/N, forward FFT along c (the outer
+ *               twiddle D and its conjugate cancel around the real diagonal 1/eps)
+ *   P4 middle : conjugate middle twiddle + forward FFT along b
+ *   P5 inner  : conjugate inner twiddle + forward FFT along a + Pi^H contraction
+ * Every pass reads and writes each field element once; twiddles and Pi bases are
+ * recomputed from closed forms instead of being read from N-sized tables.
+ * All kernels take the problem index from blockIdx.y and skip problems whose
+ * `active` flag is 0 (per-problem convergence masks of the batched solvers).
+ */
+#include "fame_device.hpp"
+#include "fame_kernels.h"
+
+namespace fame {
+
+template <typename T> struct Vec;
+template <> struct Vec<double> { typedef double2 T2; };
+template <> struct Vec<float>  { typedef float2 T2; };
+
+template <typename T2> __device__ __forceinline__ T2 from_d(double2 v);
+template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
+template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ double2 to_d(double2 v) { return v; }
+__device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+
+template <typename T2>
+__device__ __forceinline__ void load_table(T2* dst, const void* src, int n) {
+    const T2* s = reinterpret_cast<const T2*>(src);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = s[i];
+}
+
+/* ------------------------------------------------------------------------- */
+/*  P1 / P5: inner axis                                                       */
+/* ------------------------------------------------------------------------- */
+/* expand = 1 : Y(field) = twid .* IFFT_a( Pi x )          (Qr / Pr first stage)
+ * expand = 0 : y(reduced) = Pi^H FFT_a( conj(twid) .* F ) (Qrs / Prs last stage)
+ * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
+ *              (FAME_Matrix_Lambdas.cu:217-243)                                 */
+template <typename T, int EXPAND>
+__global__ void __launch_bounds__(256) k_inner(KInner args) {
+    typedef typename Vec<T>::T2 T2;
+    const int p = blockIdx.y;
+    if (args.active && !args.active[p]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const GridDev g = args.g;
+    const ProbDev pr = args.prob[p];
+    const FftPlanDev& pl = args.plan;
+    const int na = g.na, L = args.L, pitch = args.pitch;
+    T2* Wl = reinterpret_cast<T2*>(smem_raw);
+    T2* A = Wl + pl.M;
+    T2* B = A + 3 * L * pitch;
+    load_table<T2>(Wl, pl.W, pl.M);
+    const int nlines = g.nb * g.nc;
+    const int l0 = blockIdx.x * L;
+    const size_t N = (size_t)g.N;
+    const T2* X = reinterpret_cast<const T2*>(args.x) + (size_t)p * args.xstride;
+    T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
+
+    if (EXPAND) {
+        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+            const int line = idx / na, a = idx - line * na;
+            const int gl = l0 + line;
+            double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+            if (gl < nlines) {
+                const size_t f = (size_t)gl * na + a;
+                if (!(pr.gamma && f == 0)) {
+                    const int bf = gl % g.nb, cf = gl / g.nb;
+                    double2 lx, ly, lz, p1[3], p2[3];
+                    double lq;
+                    lambdas(g, pr, a, bf, cf, lx, ly, lz);
+                    pi_basis(g, lx, ly, lz, p1, p2, lq);
+                    const double2 x1 = to_d(X[f]), x2 = to_d(X[N + f]);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
+                        else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = from_d<T2>(u[c]);
+        }
+        __syncthreads();
+        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, Wl, +1);
+        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+            const int line = idx / na, a = idx - line * na;
+            const int gl = l0 + line;
+            if (gl >= nlines) continue;
+            const int bf = gl % g.nb, cf = gl / g.nb;
+            const double ph = (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a;
+            const T2 w = from_d<T2>(expipi(2.0 * ph / (double)na));
+            const size_t f = (size_t)gl * na + a;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) F[c * N + f] = cmul(R[(c * L + line) * pitch + a], w);
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+            const int line = idx / na, a = idx - line * na;
+            const int gl = l0 + line;
+            T2 v[3];
+            if (gl < nlines) {
+                const int bf = gl % g.nb, cf = gl / g.nb;
+                const double ph = (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a;
+                const T2 w = from_d<T2>(expipi(2.0 * ph / (double)na));
+                const size_t f = (size_t)gl * na + a;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + f], w);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) { v[c].x = 0; v[c].y = 0; }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
+        }
+        __syncthreads();
+        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, Wl, -1);
+        T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
+        const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
+        double dre = 0.0, dim = 0.0;
+        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
+            const int line = idx / na, a = idx - line * na;
+            const int gl = l0 + line;
+            if (gl >= nlines) continue;
+            const size_t f = (size_t)gl * na + a;
+            double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
+            if (!(pr.gamma && f == 0)) {
+                const int bf = gl % g.nb, cf = gl / g.nb;
+                double2 lx, ly, lz, p1[3], p2[3];
+                double lq;
+                lambdas(g, pr, a, bf, cf, lx, ly, lz);
+                pi_basis(g, lx, ly, lz, p1, p2, lq);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double2 w = to_d(R[(c * L + line) * pitch + a]);
+                    if (args.pikind == 0) {
+                        y1 = cadd(y1, cmul(cconj(p1[c]), w));
+                        y2 = cadd(y2, cmul(cconj(p2[c]), w));
+                    } else {
+                        y1 = csub(y1, cmul(p2[c], w));
+                        y2 = cadd(y2, cmul(p1[c], w));
+                    }
+                }
+            }
+            const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
+            Y[f] = o1;
+            Y[N + f] = o2;
+            if (D) {
+                const double2 d1 = to_d(D[f]), d2 = to_d(D[N + f]);
+                const double2 q1 = to_d(o1), q2 = to_d(o2);
+                dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
+                dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+            }
+        }
+        if (D) {
+            /* block reduction of conj(d) . y into partials[p][blockIdx.x] */
+            __shared__ double red[2][256];
+            __syncthreads();
+            red[0][threadIdx.x] = dre; red[1][threadIdx.x] = dim;
+            __syncthreads();
+            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+                if ((int)threadIdx.x < s) {
+                    red[0][threadIdx.x] += red[0][threadIdx.x + s];
+                    red[1][threadIdx.x] += red[1][threadIdx.x + s];
+                }
+                __syncthreads();
+            }
+            if (threadIdx.x == 0) {
+                double* part = args.partials + ((size_t)p * gridDim.x + blockIdx.x) * 2;
+                part[0] = red[0][0]; part[1] = red[1][0];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/*  P2 / P4: middle axis, per component                                      */
+/* ------------------------------------------------------------------------- */
+/* dir = +1: F = twid_b .* IFFT_b(F) ; dir = -1: F = FFT_b(conj(twid_b) .* F)   */
+template <typename T, int DIR>
+__global__ void __launch_bounds__(256) k_mid(KAxis args) {
+    typedef typename Vec<T>::T2 T2;
+    const int p = blockIdx.y / 3, comp = blockIdx.y % 3;
+    if (args.active && !args.active[p]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const GridDev g = args.g;
+    const ProbDev pr = args.prob[p];
+    const FftPlanDev& pl = args.plan;
+    const int na = g.na, nb = g.nb, TA = args.L, pitch = args.pitch;
+    T2* Wl = reinterpret_cast<T2*>(smem_raw);
+    T2* A = Wl + pl.M;
+    T2* B = A + TA * pitch;
+    load_table<T2>(Wl, pl.W, pl.M);
+    const int ntile = (na + TA - 1) / TA;
+    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x % ntile) * TA;
+    T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
+    const double phc = pr.s0 + pr.s1 * (double)cf;
+    for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
+        const int ta = idx % TA, b = idx / TA, a = a0 + ta;
+        T2 v;
+        if (a < na) {
+            v = F[(size_t)b * na + a];
+            if (DIR < 0) v = cmulc(v, from_d<T2>(expipi(2.0 * phc * (double)b / (double)nb)));
+        } else { v.x = 0; v.y = 0; }
+        A[ta * pitch + b] = v;
+    }
+    __syncthreads();
+    T2* R = fft_lines<T2, T>(A, B, TA, pitch, pl, Wl, DIR > 0 ? +1 : -1);
+    for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
+        const int ta = idx % TA, b = idx / TA, a = a0 + ta;
+        if (a >= na) continue;
+        T2 v = R[ta * pitch + b];
+        if (DIR > 0) v = cmul(v, from_d<T2>(expipi(2.0 * phc * (double)b / (double)nb)));
+        F[(size_t)b * na + a] = v;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/*  P3: outer axis, per component                                            */
+/* ------------------------------------------------------------------------- */
+/* mode OUTER_EPS   : F = FFT_c( invB/N .* IFFT_c(F) )
+ * mode OUTER_QR    : F = D_c/N .* IFFT_c(F)            (end of Qr / Pr)
+ * mode OUTER_QRS   : F = FFT_c( conj(D_c) .* F )       (start of Qrs / Prs)   */
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) k_outer(KAxis args) {
+    typedef typename Vec<T>::T2 T2;
+    const int p = blockIdx.y / 3, comp = blockIdx.y % 3;
+    if (args.active && !args.active[p]) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const GridDev g = args.g;
+    const ProbDev pr = args.prob[p];
+    const FftPlanDev& pl = args.plan;
+    const int nc = g.nc, TQ = args.L, pitch = args.pitch;
+    const int plane = g.na * g.nb;
+    T2* Wl = reinterpret_cast<T2*>(smem_raw);
+    T2* A = Wl + pl.M;
+    T2* B = A + TQ * pitch;
+    load_table<T2>(Wl, pl.W, pl.M);
+    const int q0 = blockIdx.x * TQ;
+    T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
+    for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
+        const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+        T2 v;
+        if (q < plane) {
+            v = F[(size_t)c * plane + q];
+            if (MODE == OUTER_QRS) v = cmulc(v, from_d<T2>(expipi(2.0 * pr.r0 * (double)c / (double)nc)));
+        } else { v.x = 0; v.y = 0; }
+        A[tq * pitch + c] = v;
+    }
+    __syncthreads();
+    if (MODE == OUTER_EPS) {
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, +1);
+        T2* S = (R == A) ? B : A;
+        const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
+        const T invN = (T)(1.0 / (double)g.N);
+        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
+            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            T s = (q < plane) ? invB[(size_t)c * plane + q] * invN : (T)0;
+            R[tq * pitch + c] = cscale(R[tq * pitch + c], s);
+        }
+        __syncthreads();
+        R = fft_lines<T2, T>(R, S, TQ, pitch, pl, Wl, -1);
+        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
+            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
+        }
+    } else if (MODE == OUTER_QR) {
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, +1);
+        const double invN = 1.0 / (double)g.N;
+        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
+            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            if (q >= plane) continue;
+            double2 w = expipi(2.0 * pr.r0 * (double)c / (double)nc);
+            F[(size_t)c * plane + q] = cmul(R[tq * pitch + c], from_d<T2>(cscale(w, invN)));
+        }
+    } else {
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, -1);
+        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
+            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/*  Layout helpers                                                            */
+/* ------------------------------------------------------------------------- */
+/* internal [c][b][a] <-> reference real-space x-fastest for General lattices,
+ * i.e. [x][y][z] (z fastest) <-> [z][y][x] (x fastest).  dir=+1: int -> ref.  */
+template <typename T2>
+__global__ void k_xz_transpose(const T2* __restrict__ in, T2* __restrict__ out, int nx, int ny, int nz, int dir,
+                               size_t stride, int ncomp) {
+    __shared__ T2 tile[32][33];
+    const int comp = blockIdx.z % ncomp, p = blockIdx.z / ncomp;
+    const int y = blockIdx.y;
+    const int tilesz = (nz + 31) / 32;
+    const int tx0 = (blockIdx.x / tilesz) * 32, tz0 = (blockIdx.x % tilesz) * 32;
+    const size_t N = (size_t)nx * ny * nz;
+    const T2* I = in + p * stride + comp * N;
+    T2* O = out + p * stride + comp * N;
+    /* threads: 32 x 8 */
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    if (dir > 0) {
+        for (int r = ly; r < 32; r += 8) {
+            int x = tx0 + r, z = tz0 + lx;
+            if (x < nx && z < nz) tile[r][lx] = I[((size_t)x * ny + y) * nz + z];
+        }
+        __syncthreads();
+        for (int r = ly; r < 32; r += 8) {
+            int z = tz0 + r, x = tx0 + lx;
+            if (x < nx && z < nz) O[x + (size_t)nx * (y + (size_t)ny * z)] = tile[lx][r];
+        }
+    } else {
+        for (int r = ly; r < 32; r += 8) {
+            int z = tz0 + r, x = tx0 + lx;
+            if (x < nx && z < nz) tile[r][lx] = I[x + (size_t)nx * (y + (size_t)ny * z)];
+        }
+        __syncthreads();
+        for (int r = ly; r < 32; r += 8) {
+            int x = tx0 + r, z = tz0 + lx;
+            if (x < nx && z < nz) O[((size_t)x * ny + y) * nz + z] = tile[lx][r];
+        }
+    }
+}
+
+/* reduced vector: reference layout [x1(Nd); x2(Nd)] <-> padded [x1(N); x2(N)] */
+template <typename T2>
+__global__ void k_pack(const T2* __restrict__ in, T2* __restrict__ out, int N, int Nd, int dir) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int off = N - Nd;
+    if (i >= 2 * N) return;
+    const int half = i / N, f = i - half * N;
+    if (dir > 0) {           /* ref -> padded */
+        T2 v; v.x = 0; v.y = 0;
+        if (f >= off) v = in[half * Nd + (f - off)];
+        out[i] = v;
+    } else {                 /* padded -> ref */
+        if (f >= off) out[half * Nd + (f - off)] = in[i];
+    }
+}
+
+}  // namespace fame
+
+/* ========================================================================= */
+/*  Host launchers                                                            */
+/* ========================================================================= */
+using namespace fame;
+
+template <typename T>
+static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
+    const int nlines = a.g.nb * a.g.nc;
+    dim3 grid((nlines + a.L - 1) / a.L, nprob);
+    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * 3 * a.L * a.pitch);
+    if (expand) hipLaunchKernelGGL((k_inner<T, 1>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((k_inner<T, 0>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) {
+    const int ntile = (a.g.na + a.L - 1) / a.L;
+    dim3 grid(ntile * a.g.nc, 3 * nprob);
+    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * a.L * a.pitch);
+    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1>), grid, dim3(256), lds, s, a);
+    else hipLaunchKernelGGL((k_mid<T, -1>), grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t launch_outer(const KAxis& a, int mode, int nprob, hipStream_t s) {
+    const int plane = a.g.na * a.g.nb;
+    dim3 grid((plane + a.L - 1) / a.L, 3 * nprob);
+    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * a.L * a.pitch);
+    switch (mode) {
+        case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS>), grid, dim3(256), lds, s, a); break;
+        case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR>), grid, dim3(256), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_outer<T, OUTER_QRS>), grid, dim3(256), lds, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t fame_launch_inner(const KInner& a, int expand, int prec, int nprob, hipStream_t s) {
+    return prec == 8 ? launch_inner<double>(a, expand, nprob, s) : launch_inner<float>(a, expand, nprob, s);
+}
+hipError_t fame_launch_mid(const KAxis& a, int dir, int prec, int nprob, hipStream_t s) {
+    return prec == 8 ? launch_mid<double>(a, dir, nprob, s) : launch_mid<float>(a, dir, nprob, s);
+}
+hipError_t fame_launch_outer(const KAxis& a, int mode, int prec, int nprob, hipStream_t s) {
+    return prec == 8 ? launch_outer<double>(a, mode, nprob, s) : launch_outer<float>(a, mode, nprob, s);
+}
+
+hipError_t fame_launch_xz_transpose(const void* in, void* out, int nx, int ny, int nz, int dir, size_t stride,
+                                    int ncomp, int nprob, int prec, hipStream_t s) {
+    const int tiles = ((nx + 31) / 32) * ((nz + 31) / 32);
+    dim3 grid(tiles, ny, ncomp * nprob), block(32, 8);
+    if (prec == 8)
+        hipLaunchKernelGGL(k_xz_transpose<double2>, grid, block, 0, s, (const double2*)in, (double2*)out, nx, ny, nz, dir, stride, ncomp);
+    else
+        hipLaunchKernelGGL(k_xz_transpose<float2>, grid, block, 0, s, (const float2*)in, (float2*)out, nx, ny, nz, dir, stride, ncomp);
+    return hipGetLastError();
+}
+
+hipError_t fame_launch_pack(const void* in, void* out, int N, int Nd, int dir, int prec, hipStream_t s) {
+    dim3 grid((2 * N + 255) / 256);
+    if (prec == 8) hipLaunchKernelGGL(k_pack<double2>, grid, dim3(256), 0, s, (const double2*)in, (double2*)out, N, Nd, dir);
+    else hipLaunchKernelGGL(k_pack<float2>, grid, dim3(256), 0, s, (const float2*)in, (float2*)out, N, Nd, dir);
+    return hipGetLastError();
+}
+
+/* Sigma_r = |Lambda| per Fourier mode and its inverse (0 for the removed Gamma
+ * mode), FAME_Matrix_Lambdas.cu:193-205 / :442-454, duplicated over [x1;x2] by
+ * the callers through a modulo-N index.                                       */
+__global__ void k_sigma(GridDev g, const ProbDev* prob, double* sig, double* sig_inv) {
+    const int p = blockIdx.y;
+    const ProbDev pr = prob[p];
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < g.N; f += gridDim.x * blockDim.x) {
+        const int a = f % g.na, bc = f / g.na, b = bc % g.nb, c = bc / g.nb;
+        double s = 0.0, si = 0.0;
+        if (!(pr.gamma && f == 0)) {
+            double2 lx, ly, lz;
+            lambdas(g, pr, a, b, c, lx, ly, lz);
+            double Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
+            s = sqrt(Lq);
+            si = 1.0 / s;
+        }
+        sig[(size_t)p * g.N + f] = s;
+        sig_inv[(size_t)p * g.N + f] = si;
+    }
+}
+
+hipError_t fame_launch_sigma(const GridDev& g, const ProbDev* prob, double* sig, double* sig_inv, int nprob, hipStream_t s) {
+    hipLaunchKernelGGL(k_sigma, dim3(512, nprob), dim3(256), 0, s, g, prob, sig, sig_inv);
+    return hipGetLastError();
+}

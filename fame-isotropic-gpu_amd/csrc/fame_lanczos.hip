@@ -1,0 +1,356 @@
+/*
+ * fame_lanczos.hip -- batched restarted Lanczos on A_r^{-1} (host orchestration).
+ *
+ * Follows Lanczos_Isotropic.cu:7-96 / Lanczos_decomp_gpu.cu:7-91 /
+ * Lanczos_LockPurge_gpu.cu:7-48 of the reference, re-planned for one GPU and
+ * many wave vectors at once:
+ *  - all problems of the context advance in lock-step; a per-problem mask
+ *    freezes converged ones;
+ *  - alpha, beta, norms and reorthogonalisation coefficients never leave the
+ *    device; T (tridiagonal) comes back once per restart;
+ *  - full reorthogonalisation is classical Gram-Schmidt applied twice (the
+ *    reference applies one modified Gram-Schmidt pass, one dot+axpy per column);
+ *  - the exact-shift lock/purge restart is accumulated on the host as an
+ *    (nstep+1)x(nstep+1) rotation matrix Q, using the Gram matrix G = U^H U for
+ *    the normalisations, and applied to the basis as one U*Q product instead
+ *    of sum(nstep-j) column rotations;
+ *  - the tridiagonal eigensolver (LAPACKE_zpteqr in the reference) is an own
+ *    implicit-QL routine (fame_tridiag.cpp).
+ */
+#include "fame_hip.h"
+#include "fame_host.h"
+#include "fame_kernels.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fame;
+
+extern "C" const char* fame_last_error(void);
+std::string& fame_err_ref();
+
+#define FAIL(code, msg) do { fame_err_ref() = (msg); return (code); } while (0)
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    fame_err_ref() = std::string("HIP error: ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" + std::to_string(__LINE__); \
+    return FAME_ERR_HIP; } } while (0)
+#define CHK(x) do { fame_status s_ = (x); if (s_ != FAME_OK) return s_; } while (0)
+
+int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z);
+void fame_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s, double shift, int n);
+
+#define GEMV_BLK 256
+#define GRAM_BLK 64
+
+namespace {
+
+struct LzBuffers {
+    size_t n2 = 0;          /* 2N */
+    int nk = 0, nstep = 0, nwant = 0;
+    double2* U = nullptr;   /* nk x (nstep+1) x 2N */
+    double2* W = nullptr;   /* nk x max(nwant+1, nwant) x 2N */
+    double* T0d = nullptr;  /* nk x (nstep+1) */
+    double* T1d = nullptr;
+    double* hdot = nullptr; /* nk x 2 */
+    double* h = nullptr;    /* nk x 4 */
+    double* hh = nullptr;   /* nk x 2(nstep+1) */
+    double* nrm = nullptr;  /* nk */
+    double* invb = nullptr; /* nk */
+    double* gpart = nullptr;/* nk x GEMV_BLK x 2(nstep+1) ; also gram partials */
+    double* G = nullptr;    /* nk x 2 m^2 */
+    double* Q = nullptr;    /* nk x m x (nwant+1) */
+    int* mask = nullptr;    /* nk */
+};
+
+fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
+    for (void* p : c->big) if (p) (void)hipFree(p);
+    c->big.clear();
+    b.n2 = 2 * (size_t)c->N; b.nk = nk; b.nstep = nstep; b.nwant = nwant;
+    const size_t m = nstep + 1;
+    auto al = [&](void** p, size_t bytes) -> fame_status {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+        if (e != hipSuccess) { *p = nullptr; FAIL(FAME_ERR_NOMEM, std::string("Lanczos allocation of ") + std::to_string(bytes) + " bytes failed"); }
+        c->big.push_back(*p);
+        return FAME_OK;
+    };
+    CHK(al((void**)&b.U, 16 * b.n2 * m * nk));
+    CHK(al((void**)&b.W, 16 * b.n2 * (nwant + 1) * nk));
+    CHK(al((void**)&b.T0d, sizeof(double) * m * nk));
+    CHK(al((void**)&b.T1d, sizeof(double) * m * nk));
+    CHK(al((void**)&b.hdot, sizeof(double) * 2 * nk));
+    CHK(al((void**)&b.h, sizeof(double) * 4 * nk));
+    CHK(al((void**)&b.hh, sizeof(double) * 2 * m * nk));
+    CHK(al((void**)&b.nrm, sizeof(double) * nk));
+    CHK(al((void**)&b.invb, sizeof(double) * nk));
+    size_t gp = std::max((size_t)GEMV_BLK * 2 * m, (size_t)GRAM_BLK * 2 * m * m);
+    CHK(al((void**)&b.gpart, sizeof(double) * gp * nk));
+    CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
+    CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
+    CHK(al((void**)&b.mask, sizeof(int) * nk));
+    return FAME_OK;
+}
+
+}  // namespace
+
+/* column j of problem 0 (problem stride is (nstep+1)*2N) */
+static inline double2* col(const LzBuffers& b, int j) { return b.U + (size_t)j * b.n2; }
+
+/* U_{j+1} = invAr(U_j) for the masked problems, through contiguous staging */
+static fame_status lz_invar(fame_ctx* c, LzBuffers& b, int j, int mode, const fame_ls& ls, const int* mask_dev,
+                            std::vector<int>& ls_iter, std::vector<int>& es_iter, const std::vector<int>& mask_h) {
+    const size_t ps = b.n2 * (b.nstep + 1);
+    double2* xin = (double2*)c->ref64[5];
+    double2* yout = (double2*)c->ref64[0];
+    HIPCHK(fame_convert(xin, 8, col(b, j), 8, b.n2, b.n2, ps, b.nk, c->stream));
+    std::vector<int> it(b.nk, 0);
+    CHK(c->invar(mode, xin, yout, 0, b.nk, mask_dev, ls, it.data()));
+    HIPCHK(fame_convert(col(b, j + 1), 8, yout, 8, b.n2, ps, b.n2, b.nk, c->stream));
+    for (int p = 0; p < b.nk; ++p) if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; }
+    return FAME_OK;
+}
+
+/* alpha_j, subtract the three-term recurrence, beta_j, normalise (Lanczos_decomp_gpu.cu:161-174) */
+static fame_status lz_three_term(fame_ctx* c, LzBuffers& b, int j, const int* mask_dev) {
+    const size_t ps = b.n2 * (b.nstep + 1);
+    const int ld = b.nstep + 1;
+    HIPCHK(fame_gemv_h(col(b, j), col(b, j + 1), b.gpart, GEMV_BLK, 1, b.n2, b.n2, ps, ps, b.nk, mask_dev, c->stream));
+    HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 2, b.hdot, b.nk, c->stream));
+    HIPCHK(fame_lz_coeffs(b.hdot, b.T0d, b.T1d, ld, j, b.h, b.nk, mask_dev, c->stream));
+    if (j >= 1) HIPCHK(fame_gemv_n(col(b, j - 1), col(b, j + 1), b.h, 2, b.n2, b.n2, ps, ps, 4, b.nk, mask_dev, c->stream));
+    else HIPCHK(fame_gemv_n(col(b, j), col(b, j + 1), b.h, 1, b.n2, b.n2, ps, ps, 4, b.nk, mask_dev, c->stream));
+    HIPCHK(fame_norm2_partials(col(b, j + 1), 8, b.gpart, GEMV_BLK, b.n2, ps, b.nk, c->stream));
+    HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 1, b.nrm, b.nk, c->stream));
+    HIPCHK(fame_lz_beta(b.nrm, b.T1d, ld, j, b.invb, b.nk, mask_dev, c->stream));
+    HIPCHK(fame_scale_real_mod(col(b, j + 1), col(b, j + 1), b.invb, b.n2, ps, 1, 1, b.nk, 8, c->stream));
+    return FAME_OK;
+}
+
+/* classical Gram-Schmidt twice against U_0..U_j */
+static fame_status lz_reorth(fame_ctx* c, LzBuffers& b, int j, const int* mask_dev) {
+    const size_t ps = b.n2 * (b.nstep + 1);
+    const int m = j + 1;
+    for (int rep = 0; rep < 2; ++rep) {
+        HIPCHK(fame_gemv_h(col(b, 0), col(b, j + 1), b.gpart, GEMV_BLK, m, b.n2, b.n2, ps, ps, b.nk, mask_dev, c->stream));
+        HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 2 * m, b.hh, b.nk, c->stream));
+        HIPCHK(fame_gemv_n(col(b, 0), col(b, j + 1), b.hh, m, b.n2, b.n2, ps, ps, 2 * m, b.nk, mask_dev, c->stream));
+    }
+    return FAME_OK;
+}
+
+static fame_status lz_decomp(fame_ctx* c, LzBuffers& b, bool init, int mode, const fame_ls& ls,
+                             const std::vector<int>& mask_h, std::vector<int>& ls_iter, std::vector<int>& es_iter) {
+    HIPCHK(hipMemcpyAsync(b.mask, mask_h.data(), sizeof(int) * b.nk, hipMemcpyHostToDevice, c->stream));
+    int start = b.nwant;
+    if (init) {
+        /* U_0 = e_n (Lanczos_init_kernel, Lanczos_decomp_gpu.cu:93-124) */
+        const size_t ps = b.n2 * (b.nstep + 1);
+        const double one[2] = {1.0, 0.0};
+        for (int p = 0; p < b.nk; ++p) {
+            if (!mask_h[p]) continue;
+            HIPCHK(hipMemsetAsync(b.U + p * ps, 0, 16 * b.n2, c->stream));
+            HIPCHK(hipMemcpyAsync(b.U + p * ps + (b.n2 - 1), one, 16, hipMemcpyHostToDevice, c->stream));
+        }
+        HIPCHK(hipStreamSynchronize(c->stream));
+        CHK(lz_invar(c, b, 0, mode, ls, b.mask, ls_iter, es_iter, mask_h));
+        CHK(lz_three_term(c, b, 0, b.mask));
+        start = 1;
+    }
+    for (int j = start; j < b.nstep; ++j) {
+        CHK(lz_invar(c, b, j, mode, ls, b.mask, ls_iter, es_iter, mask_h));
+        CHK(lz_three_term(c, b, j, b.mask));
+        CHK(lz_reorth(c, b, j, b.mask));
+    }
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
+                                      void* evr_dev, void* E_dev, fame_profile* prof);
+
+extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
+                                 void* evr_dev, void* E_dev, fame_profile* prof) {
+    return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
+}
+
+fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev_padded, size_t ev_pstride, void* E_dev);
+
+extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
+                                      void* evr_dev, void* E_dev, fame_profile* prof) {
+    if (!c || !es || !ls || !freq) FAIL(FAME_ERR_ARG, "fame_eigs: null argument");
+    if (!c->have_eps || c->nk < 1) FAIL(FAME_ERR_STATE, "fame_eigs: permittivity and wave vectors must be set first");
+    if (es->nwant < 1 || es->nstep <= es->nwant || es->nstep > 62 || es->maxit < 1 || ls->maxit < 1)
+        FAIL(FAME_ERR_ARG, "fame_eigs: need 1 <= nwant < nstep <= 62 and positive maxit");
+    if (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE) FAIL(FAME_ERR_ARG, "fame_eigs: unknown precision");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    c->ls_last = *ls;
+    const int nk = c->nk, nstep = es->nstep, nwant = es->nwant, m = nstep + 1;
+    LzBuffers b;
+    CHK(lz_alloc(c, b, nk, nstep, nwant));
+    long long q64_0 = c->cnt_qbq64, q32_0 = c->cnt_qbq32;
+    std::vector<int> ls_iter(nk, 0), es_iter(nk, 0), restarts(nk, 0), conv(nk, 0);
+    std::vector<int> lzmask(nk, 1);
+    CHK(lz_decomp(c, b, true, precision, *ls, lzmask, ls_iter, es_iter));
+
+    std::vector<double> T0((size_t)m * nk), T1((size_t)m * nk);
+    std::vector<double> LT0((size_t)nstep * nk), Z((size_t)nstep * nstep * nk);
+    std::vector<double> G((size_t)2 * m * m * nk), Qh((size_t)m * (nwant + 1) * nk);
+    std::vector<double> T2(m), T3(m), cc(m), ss(m), Qf((size_t)m * m);
+    for (int iter = 1; iter <= es->maxit; ++iter) {
+        HIPCHK(hipMemcpyAsync(T0.data(), b.T0d, sizeof(double) * m * nk, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpyAsync(T1.data(), b.T1d, sizeof(double) * m * nk, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<int> rmask(nk, 0);
+        int nrest = 0;
+        for (int p = 0; p < nk; ++p) {
+            if (!lzmask[p]) continue;
+            double* t0 = &T0[(size_t)p * m];
+            double* t1 = &T1[(size_t)p * m];
+            double* lt0 = &LT0[(size_t)p * nstep];
+            double* z = &Z[(size_t)p * nstep * nstep];
+            if (fame_tridiag_eig(nstep, t0, t1, lt0, z) != 0) FAIL(FAME_ERR_HIP, "tridiagonal eigensolver did not converge");
+            restarts[p] = iter;
+            int cv = 0;
+            for (int i = 0; i < nwant; ++i) {     /* Lanczos_Isotropic.cu:59-68 */
+                double res = t1[nstep - 1] * std::fabs(z[(size_t)(i + 1) * nstep - 1]);
+                if (res < es->tol) cv++; else break;
+            }
+            if (cv == nwant) { conv[p] = 1; lzmask[p] = 0; continue; }
+            rmask[p] = 1; nrest++;
+        }
+        if (!nrest) break;
+        /* Gram matrices of the bases that restart */
+        HIPCHK(hipMemcpyAsync(b.mask, rmask.data(), sizeof(int) * nk, hipMemcpyHostToDevice, c->stream));
+        const size_t ps = b.n2 * m;
+        HIPCHK(fame_gram(b.U, b.gpart, GRAM_BLK, m, b.n2, b.n2, ps, nk, b.mask, c->stream));
+        HIPCHK(fame_reduce_partials(b.gpart, GRAM_BLK, 2 * m * m, b.G, nk, c->stream));
+        HIPCHK(hipMemcpyAsync(G.data(), b.G, sizeof(double) * 2 * m * m * nk, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < nk; ++p) {
+            if (!rmask[p]) continue;
+            double* t0 = &T0[(size_t)p * m];
+            double* t1 = &T1[(size_t)p * m];
+            const double* lt0 = &LT0[(size_t)p * nstep];
+            const double* Gp = &G[(size_t)p * 2 * m * m];
+            /* Q = I ; columns follow the reference's Zdrot sequence (Lanczos_LockPurge_gpu.cu:23-45) */
+            std::fill(Qf.begin(), Qf.end(), 0.0);
+            for (int i = 0; i < m; ++i) Qf[(size_t)i * m + i] = 1.0;
+            auto rot = [&](int xcol, int ycol, double cs, double sn) {   /* x = c x + s y ; y = -s x + c y */
+                double* x = &Qf[(size_t)xcol * m];
+                double* y = &Qf[(size_t)ycol * m];
+                for (int r = 0; r < m; ++r) { double xv = x[r], yv = y[r]; x[r] = cs * xv + sn * yv; y[r] = -sn * xv + cs * yv; }
+            };
+            auto gnorm = [&](int qc) {   /* ||U q|| with G = U^H U (upper triangle stored) */
+                const double* q = &Qf[(size_t)qc * m];
+                double s = 0.0;
+                for (int r = 0; r < m; ++r) {
+                    if (q[r] == 0.0) continue;
+                    s += q[r] * q[r] * Gp[2 * (r * m + r)];
+                    for (int t = r + 1; t < m; ++t) s += 2.0 * q[r] * q[t] * Gp[2 * (r * m + t)];
+                }
+                return std::sqrt(std::max(s, 0.0));
+            };
+            for (int j = 0; j < nstep - nwant; ++j) {
+                fame_gvqrrq(t0, t1, T2.data(), T3.data(), cc.data(), ss.data(), lt0[nwant + j], nstep - j);
+                int jj;
+                for (jj = 0; jj < nstep - 1 - j; ++jj) rot(jj + 1, jj, cc[jj], ss[jj]);
+                ss[jj - 1] *= t1[jj];
+                rot(jj + 1, jj, t1[jj - 1], ss[jj - 1]);
+                t1[jj - 1] = gnorm(jj);
+                double inv = 1.0 / t1[jj - 1];
+                for (int r = 0; r < m; ++r) Qf[(size_t)jj * m + r] *= inv;
+                t0[jj] = 0.0; t1[jj] = 0.0;
+                ++jj;
+                for (int r = 0; r < m; ++r) Qf[(size_t)jj * m + r] = 0.0;
+            }
+            std::memcpy(&Qh[(size_t)p * m * (nwant + 1)], Qf.data(), sizeof(double) * m * (nwant + 1));
+        }
+        HIPCHK(hipMemcpyAsync(b.Q, Qh.data(), sizeof(double) * m * (nwant + 1) * nk, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b.T0d, T0.data(), sizeof(double) * m * nk, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(b.T1d, T1.data(), sizeof(double) * m * nk, hipMemcpyHostToDevice, c->stream));
+        const size_t wps = b.n2 * (nwant + 1);
+        HIPCHK(fame_gemm_vq(b.U, b.Q, b.W, m, nwant + 1, b.n2, b.n2, ps, b.n2, wps, (size_t)m * (nwant + 1), nk, b.mask,
+                            c->stream));
+        for (int p = 0; p < nk; ++p)
+            if (rmask[p]) HIPCHK(hipMemcpyAsync(b.U + p * ps, b.W + p * wps, 16 * wps, hipMemcpyDeviceToDevice, c->stream));
+        CHK(lz_decomp(c, b, false, precision, *ls, rmask, ls_iter, es_iter));
+        if (iter == es->maxit) for (int p = 0; p < nk; ++p) if (rmask[p]) restarts[p] = es->maxit + 1;
+    }
+    /* Ritz values and vectors: omega = sqrt(1/theta), ev = U z (Lanczos_Isotropic.cu:87-93) */
+    std::vector<double> Zq((size_t)nstep * nwant * nk);
+    for (int p = 0; p < nk; ++p) {
+        for (int i = 0; i < nwant; ++i) freq[(size_t)p * nwant + i] = std::sqrt(1.0 / LT0[(size_t)p * nstep + i]);
+        std::memcpy(&Zq[(size_t)p * nstep * nwant], &Z[(size_t)p * nstep * nstep], sizeof(double) * nstep * nwant);
+    }
+    HIPCHK(hipMemcpyAsync(b.Q, Zq.data(), sizeof(double) * nstep * nwant * nk, hipMemcpyHostToDevice, c->stream));
+    const size_t ps = b.n2 * m, wps = b.n2 * (nwant + 1);
+    HIPCHK(fame_gemm_vq(b.U, b.Q, b.W, nstep, nwant, b.n2, b.n2, ps, b.n2, wps, (size_t)nstep * nwant, nk, nullptr,
+                        c->stream));
+    if (evr_dev) {
+        /* reduced eigenvectors, reference layout (2Nd) with a 2N stride per band */
+        for (int p = 0; p < nk; ++p)
+            for (int i = 0; i < nwant; ++i)
+                HIPCHK(fame_launch_pack(b.W + p * wps + (size_t)i * b.n2,
+                                        (double2*)evr_dev + ((size_t)p * nwant + i) * b.n2, c->N, c->nd[p], -1, 8, c->stream));
+    }
+    if (E_dev) CHK(fame_restore_impl(c, nwant, b.W, wps, E_dev));
+    /* Gamma rule on the frequencies (FAME_Fast_Algorithms_Isotropic.cu:61-70) */
+    for (int p = 0; p < nk; ++p) {
+        if (!c->prob_h[p].gamma) continue;
+        double* f = freq + (size_t)p * nwant;
+        for (int i = nwant - 1; i >= 2; --i) f[i] = f[i - 2];
+        f[0] = 0.0;
+        if (nwant > 1) f[1] = 0.0;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (prof) {
+        for (int p = 0; p < nk; ++p) {
+            fame_profile& q = prof[p];
+            q.ls_iter = ls_iter[p]; q.es_iter = es_iter[p]; q.restarts = restarts[p]; q.nd = c->nd[p];
+            q.converged = conv[p]; q.pad = 0; q.es_time = secs;
+            q.qbq64 = (int)((c->cnt_qbq64 - q64_0) / nk);
+            q.qbq32 = (int)((c->cnt_qbq32 - q32_0) / nk);
+        }
+    }
+    return FAME_OK;
+}
+
+/* E_i = invB .* Qr(Sigma .* ev_i), times ||E_i|| (reference scaling), Gamma rule
+ * (FAME_Fast_Algorithms_Isotropic.cu:61-125).  ev: padded reduced vectors with
+ * per-problem stride ev_pstride and per-band stride 2N.                       */
+fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev, size_t ev_pstride, void* E_dev) {
+    const size_t N = c->N, n2 = 2 * N, N3 = 3 * N;
+    const int nk = c->nk;
+    double2* y = (double2*)c->ref64[0];
+    std::vector<double> nr(nk);
+    for (int i = 0; i < nwant; ++i) {
+        HIPCHK(fame_convert(y, 8, ev + (size_t)i * n2, 8, n2, n2, ev_pstride, nk, c->stream));
+        HIPCHK(fame_scale_real_mod(y, y, c->sig, n2, n2, N, N, nk, 8, c->stream));
+        CHK(c->qr_field(y, 0, nk, 0, nullptr));
+        HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->invB64, N3, N3, 0, N3, nk, 8, c->stream));
+        HIPCHK(fame_norm2_partials(c->F64, 8, c->partials, NBLK_VEC, N3, N3, nk, c->stream));
+        HIPCHK(fame_reduce_partials(c->partials, NBLK_VEC, 1, c->scal, nk, c->stream));
+        HIPCHK(hipMemcpyAsync(nr.data(), c->scal, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int p = 0; p < nk; ++p) nr[p] = std::sqrt(nr[p]);   /* Zdscal by the norm itself (:119-121) */
+        HIPCHK(hipMemcpyAsync(c->scal, nr.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->scal, N3, N3, 1, 1, nk, 8, c->stream));
+        for (int p = 0; p < nk; ++p)
+            CHK(c->field_to_ref(c->F64 + (size_t)p * N3, (double2*)E_dev + ((size_t)p * nwant + i) * N3, 1, N3));
+    }
+    for (int p = 0; p < nk; ++p) {
+        if (!c->prob_h[p].gamma) continue;
+        double2* E = (double2*)E_dev + (size_t)p * nwant * N3;
+        for (int i = nwant - 1; i >= 2; --i)
+            HIPCHK(hipMemcpyAsync(E + (size_t)i * N3, E + (size_t)(i - 2) * N3, 16 * N3, hipMemcpyDeviceToDevice, c->stream));
+        std::vector<double2> cst(N3, make_double2(1.0 / std::sqrt((double)N3), 0.0));
+        for (int i = 0; i < std::min(2, nwant); ++i)
+            HIPCHK(hipMemcpyAsync(E + (size_t)i * N3, cst.data(), 16 * N3, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return FAME_OK;
+}
+

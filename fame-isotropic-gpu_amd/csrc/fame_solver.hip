@@ -1,0 +1,574 @@
+/*
+ * fame_solver.hip -- host side of the gfx950 band solver behind include/fame_hip.h.
+ *
+ * Owns the FFT plans and twiddle tables, the per-wave-vector phases, the
+ * permittivity in the kernels' layout and all work buffers, and drives:
+ *   QBQ         five axis passes (fame_kernels.hip)
+ *   CG          batched, device-resident scalars, per-problem masks (CG.cu:5-87)
+ *   invAr       Sigma^-1 CG(QBQ, Sigma^-1 x) in FP64, or FP32 CG inside FP64
+ *               iterative refinement (..._Isotropic_invAr.cu:8-48)
+ *   Lanczos     e_n start, three-term recurrence, CGS2 full reorthogonalisation,
+ *               exact-shift lock/purge restart applied as one U*Q product,
+ *               Ritz vectors by U*z (Lanczos_decomp_gpu.cu, Lanczos_LockPurge_gpu.cu,
+ *               Lanczos_Isotropic.cu)
+ *   restoration E = invB .* Qr(Sigma y) scaled by ||E|| and the Gamma rule
+ *               (FAME_Fast_Algorithms_Isotropic.cu:57-125)
+ */
+#include "fame_hip.h"
+#include "fame_kernels.h"
+#include "fame_host.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace fame;
+
+static thread_local std::string g_err;
+extern "C" const char* fame_last_error(void) { return g_err.c_str(); }
+std::string& fame_err_ref() { return g_err; }
+
+#define FAIL(code, msg) do { g_err = (msg); return (code); } while (0)
+#define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    g_err = std::string("HIP error: ") + hipGetErrorString(e_) + " at " + __FILE__ + ":" + std::to_string(__LINE__); \
+    return FAME_ERR_HIP; } } while (0)
+#define CHK(x) do { fame_status s_ = (x); if (s_ != FAME_OK) return s_; } while (0)
+
+/* ======================================================================== */
+/*  FFT plans                                                                */
+/* ======================================================================== */
+static bool factor_gpu(int n, std::vector<int>* rad) {
+    std::vector<int> r;
+    int m = n;
+    while (m % 4 == 0) { r.push_back(4); m /= 4; }
+    while (m % 2 == 0) { r.push_back(2); m /= 2; }
+    const int ps[] = {3, 5, 7, 11, 13};
+    for (int p : ps) while (m % p == 0) { r.push_back(p); m /= p; }
+    if (m != 1) return false;
+    if ((int)r.size() > FAME_MAX_PASSES) return false;
+    if (rad) *rad = r;
+    return true;
+}
+static double pass_cost(int r) {
+    switch (r) { case 2: return 1.5; case 4: return 2.5; case 3: return 4; case 5: return 6; case 7: return 8;
+                 case 11: return 12; case 13: return 14; default: return 1e9; }
+}
+static double plan_cost(const std::vector<int>& rad) { double c = 0; for (int r : rad) c += pass_cost(r); return c; }
+
+typedef std::complex<long double> cld;
+
+AxisPlan::~AxisPlan() {
+    for (void* p : bufs) if (p) (void)hipFree(p);
+}
+
+fame_status AxisPlan::build(int n_) {
+    n = n_;
+    std::vector<int> rad;
+    double direct = 1e30;
+    if (factor_gpu(n, &rad)) direct = n * plan_cost(rad);
+    int bestM = 0; std::vector<int> bestR; double bestC = 1e30;
+    for (int m = 2 * n - 1; m <= 4 * n + 8; ++m) {
+        std::vector<int> r2;
+        bool ok = factor_gpu(m, &r2);
+        if (!ok) continue;
+        bool small = true;
+        for (int q : r2) if (q > 7) small = false;
+        if (!small) continue;
+        double c = 2.0 * m * plan_cost(r2) + 3.0 * m;
+        if (c < bestC) { bestC = c; bestM = m; bestR = r2; }
+    }
+    if (direct <= bestC && direct < 1e29) { blue = 0; M = n; radices = rad; }
+    else if (bestM) { blue = 1; M = bestM; radices = bestR; }
+    else FAIL(FAME_ERR_UNSUPPORTED, "no FFT plan for length " + std::to_string(n));
+    /* tables in long double, rounded once to each precision */
+    const long double PI_L = 3.14159265358979323846264338327950288L;
+    std::vector<cld> W(M), ch, bm, bp;
+    for (int m = 0; m < M; ++m) W[m] = cld(cosl(2 * PI_L * m / M), -sinl(2 * PI_L * m / M));
+    if (blue) {
+        ch.resize(n);
+        for (int m = 0; m < n; ++m) {
+            long long q = ((long long)m * m) % (2LL * n);
+            ch[m] = cld(cosl(PI_L * q / n), -sinl(PI_L * q / n));     /* exp(-i pi m^2/n) */
+        }
+        /* kernels b = conj(chirp) wrapped, for each sign; bhat = DFT_M(b) (exp(-)) */
+        std::vector<cld> b1(M, 0), b2(M, 0);
+        for (int m = 0; m < n; ++m) {
+            b1[m] = std::conj(ch[m]);            /* sign -1: chirp = exp(-i..) -> b = exp(+i..) */
+            b2[m] = ch[m];                       /* sign +1: chirp = exp(+i..) -> b = exp(-i..) */
+            if (m) { b1[M - m] = b1[m]; b2[M - m] = b2[m]; }
+        }
+        bm.assign(M, 0); bp.assign(M, 0);
+        for (int k = 0; k < M; ++k) {
+            cld s1 = 0, s2 = 0;
+            for (int j = 0; j < M; ++j) {
+                cld w = W[(long long)j * k % M];
+                s1 += b1[j] * w; s2 += b2[j] * w;
+            }
+            bm[k] = s1; bp[k] = s2;
+        }
+    }
+    auto upload = [&](const std::vector<cld>& v, int prec, const void** dst) -> fame_status {
+        if (v.empty()) { *dst = nullptr; return FAME_OK; }
+        void* d = nullptr;
+        size_t bytes = v.size() * (prec == 8 ? 16 : 8);
+        HIPCHK(hipMalloc(&d, bytes));
+        bufs.push_back(d);
+        if (prec == 8) {
+            std::vector<double> h(2 * v.size());
+            for (size_t i = 0; i < v.size(); ++i) { h[2 * i] = (double)v[i].real(); h[2 * i + 1] = (double)v[i].imag(); }
+            HIPCHK(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> h(2 * v.size());
+            for (size_t i = 0; i < v.size(); ++i) { h[2 * i] = (float)v[i].real(); h[2 * i + 1] = (float)v[i].imag(); }
+            HIPCHK(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice));
+        }
+        *dst = d;
+        return FAME_OK;
+    };
+    for (int pi = 0; pi < 2; ++pi) {
+        int prec = pi == 0 ? 8 : 4;
+        FftPlanDev& d = dev[pi];
+        std::memset(&d, 0, sizeof(d));
+        d.n = n; d.M = M; d.blue = blue; d.nrad = (int)radices.size();
+        for (int i = 0; i < d.nrad; ++i) d.rad[i] = radices[i];
+        CHK(upload(W, prec, &d.W));
+        CHK(upload(ch, prec, &d.chirp));
+        CHK(upload(bm, prec, &d.bhat_m));
+        CHK(upload(bp, prec, &d.bhat_p));
+    }
+    return FAME_OK;
+}
+
+/* lines per block so that the kernel's LDS stays within 64 KiB */
+static void pick_tile(const AxisPlan& p, int ncomp, int prec, int maxL, int align, int* L, int* pitch) {
+    const int es = prec == 8 ? 16 : 8;
+    const int budget = 57344 / es;   /* 56 KiB dynamic + static reduction scratch <= 64 KiB */
+    int pt = p.M;
+    int l = (budget - p.M) / (2 * ncomp * pt);
+    if (l > maxL) l = maxL;
+    if (align > 1 && l >= align) l = (l / align) * align;
+    if (l < 1) l = 0;
+    *L = l; *pitch = pt;
+}
+
+/* ======================================================================== */
+/*  Context                                                                  */
+/* ======================================================================== */
+static fame_status dalloc(fame_ctx* c, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        FAIL(FAME_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    c->allocs.push_back(*p);
+    return FAME_OK;
+}
+
+fame_ctx::~fame_ctx() {
+    for (void* p : allocs) if (p) (void)hipFree(p);
+    for (void* p : big) if (p) (void)hipFree(p);
+}
+
+extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int device, fame_ctx** out) {
+    if (!lat || !out || max_batch < 1) FAIL(FAME_ERR_ARG, "fame_create: null argument or max_batch < 1");
+    for (int i = 0; i < 3; ++i) if (lat->grid_nums[i] < 1 || lat->grid_nums[i] > 1024) FAIL(FAME_ERR_ARG, "grid_nums out of range");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) FAIL(FAME_ERR_ARG, "fame_create: no such device");
+    HIPCHK(hipSetDevice(device));
+    fame_ctx* c = new fame_ctx();
+    c->device = device;
+    c->lat = *lat;
+    c->maxb = max_batch;
+    const int Nx = lat->grid_nums[0], Ny = lat->grid_nums[1], Nz = lat->grid_nums[2];
+    GridDev& g = c->g;
+    g.general = lat->general ? 1 : 0;
+    if (g.general) { g.na = Nz; g.nb = Ny; g.nc = Nx; g.ha = lat->mesh_len[2]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[0]; }
+    else { g.na = Nx; g.nb = Ny; g.nc = Nz; g.ha = lat->mesh_len[0]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[2]; }
+    g.N = Nx * Ny * Nz;
+    c->N = g.N;
+    fame_status st;
+    if ((st = c->pa.build(g.na)) != FAME_OK || (st = c->pb.build(g.nb)) != FAME_OK || (st = c->pc.build(g.nc)) != FAME_OK) {
+        delete c; return st;
+    }
+    for (int pi = 0; pi < 2; ++pi) {
+        int prec = pi == 0 ? 8 : 4;
+        pick_tile(c->pa, 3, prec, 16, 1, &c->Lin[pi], &c->pin[pi]);
+        pick_tile(c->pb, 1, prec, 32, 4, &c->Lmid[pi], &c->pmid[pi]);
+        pick_tile(c->pc, 1, prec, 32, 4, &c->Lout[pi], &c->pout[pi]);
+        if (c->Lin[pi] < 1 || c->Lmid[pi] < 1 || c->Lout[pi] < 1) {
+            delete c;
+            FAIL(FAME_ERR_UNSUPPORTED, "FFT length too large for the LDS-resident line transforms");
+        }
+    }
+    const size_t N = g.N, B = max_batch;
+    c->nblk_in[0] = (g.nb * g.nc + c->Lin[0] - 1) / c->Lin[0];
+    c->nblk_in[1] = (g.nb * g.nc + c->Lin[1] - 1) / c->Lin[1];
+#define A(ptr, bytes) do { fame_status s_ = dalloc(c, (void**)&(ptr), (bytes)); if (s_ != FAME_OK) { delete c; return s_; } } while (0)
+    A(c->prob_d, sizeof(ProbDev) * B);
+    A(c->sig, sizeof(double) * N * B);
+    A(c->sig_inv, sizeof(double) * N * B);
+    A(c->invB64, sizeof(double) * 3 * N);
+    A(c->invB32, sizeof(float) * 3 * N);
+    A(c->eps_int, sizeof(double) * 3 * N);
+    A(c->eps_ref, sizeof(double) * 3 * N);
+    A(c->F64, 16 * 3 * N * B);
+    A(c->F32, 8 * 3 * N * B);
+    for (int i = 0; i < 4; ++i) A(c->cg64[i], 16 * 2 * N * B);
+    for (int i = 0; i < 5; ++i) A(c->cg32[i], 8 * 2 * N * B);
+    for (int i = 0; i < 6; ++i) A(c->ref64[i], 16 * 2 * N * B);
+    int nbmax = std::max(std::max(c->nblk_in[0], c->nblk_in[1]), NBLK_VEC);
+    A(c->partials, sizeof(double) * 4 * nbmax * B);
+    A(c->partials2, sizeof(double) * 2 * nbmax * B);
+    A(c->cgsc, sizeof(CgScal) * B);
+    A(c->active, sizeof(int) * B);
+    A(c->mask, sizeof(int) * B);
+    A(c->scal, sizeof(double) * 64 * B);
+#undef A
+    HIPCHK(hipHostMalloc((void**)&c->active_h, sizeof(int) * B, 0));
+    c->big.clear();
+    *out = c;
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_destroy(fame_ctx* c) {
+    if (!c) return FAME_OK;
+    if (c->active_h) (void)hipHostFree(c->active_h);
+    delete c;
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_set_stream(fame_ctx* c, void* stream) {
+    if (!c) FAIL(FAME_ERR_ARG, "null context");
+    c->stream = (hipStream_t)stream;
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_query(const fame_ctx* c, int* N, int* nk, int* len3) {
+    if (!c) FAIL(FAME_ERR_ARG, "null context");
+    if (N) *N = c->N;
+    if (nk) *nk = c->nk;
+    if (len3) { len3[0] = c->pa.M; len3[1] = c->pb.M; len3[2] = c->pc.M; }
+    return FAME_OK;
+}
+
+/* reference layout (component-stacked, x fastest) -> kernel layout [c][b][a] */
+static size_t int_index(const GridDev& g, int x, int y, int z) {
+    if (g.general) return ((size_t)x * g.nb + y) * g.na + z;    /* [x][y][z] */
+    return x + (size_t)g.na * (y + (size_t)g.nb * z);            /* [z][y][x] */
+}
+
+extern "C" fame_status fame_set_eps(fame_ctx* c, const double* eps) {
+    if (!c || !eps) FAIL(FAME_ERR_ARG, "fame_set_eps: null argument");
+    const int Nx = c->lat.grid_nums[0], Ny = c->lat.grid_nums[1], Nz = c->lat.grid_nums[2];
+    const size_t N = c->N;
+    std::vector<double> e(3 * N), ib(3 * N);
+    std::vector<float> ib32(3 * N);
+    for (int comp = 0; comp < 3; ++comp)
+        for (int z = 0; z < Nz; ++z)
+            for (int y = 0; y < Ny; ++y)
+                for (int x = 0; x < Nx; ++x) {
+                    size_t r = x + (size_t)Nx * (y + (size_t)Ny * z);
+                    double v = eps[comp * N + r];
+                    if (!(v > 0)) FAIL(FAME_ERR_ARG, "fame_set_eps: permittivity must be positive");
+                    size_t q = comp * N + int_index(c->g, x, y, z);
+                    e[q] = v; ib[q] = 1.0 / v; ib32[q] = (float)(1.0 / v);
+                }
+    HIPCHK(hipMemcpy(c->eps_int, e.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->eps_ref, eps, sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->invB64, ib.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->invB32, ib32.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+    c->have_eps = true;
+    return FAME_OK;
+}
+
+/* per-k phases, FAME_Matrix_Lambdas.cu:40-76 (theta's) and :155-157 / :402-406 (Gamma) */
+void fame_phases(const fame_lattice& L, const double* k, ProbDev* p) {
+    const double* a = L.lattice_vec_a;
+    auto dot3 = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    const int Nx = L.grid_nums[0], Ny = L.grid_nums[1], Nz = L.grid_nums[2];
+    const double kn = std::sqrt(dot3(k, k));
+    std::memset(p, 0, sizeof(*p));
+    if (!L.general) {
+        p->t0 = dot3(k, a); p->t1 = 0; p->t2 = 0;
+        p->s0 = dot3(k, a + 3); p->s1 = 0;
+        p->r0 = dot3(k, a + 6);
+        p->gamma = kn < 1e-4;
+    } else {
+        double c1 = a[3] < 0 ? 1. - (double)L.m1 / (double)Nx : -(double)L.m1 / (double)Nx;
+        double c2 = (double)L.m3 / (double)Ny, c3 = (double)L.m2 / (double)Nx;
+        double a2h[3], a3h[3];
+        for (int i = 0; i < 3; ++i) a2h[i] = c1 * a[i] + a[3 + i];
+        for (int i = 0; i < 3; ++i) a3h[i] = a[6 + i] - c2 * a2h[i] - c3 * a[i] + L.t1[i];
+        p->t0 = dot3(k, a3h);
+        p->t1 = -((double)L.m3 / (double)Ny);
+        p->t2 = ((double)L.m1 * (double)L.m3) / ((double)Ny * Nx) - ((double)L.m2 / (double)Nx);
+        p->s0 = dot3(k, a2h);
+        p->s1 = -(double)L.m1 / (double)Nx;
+        p->r0 = dot3(k, a);
+        p->gamma = kn < 1e-3;
+        (void)Nz;
+    }
+}
+
+extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* k, int* nd_out) {
+    if (!c || !k || nk < 1) FAIL(FAME_ERR_ARG, "fame_set_wave_vectors: bad arguments");
+    if (nk > c->maxb) FAIL(FAME_ERR_ARG, "fame_set_wave_vectors: more wave vectors than max_batch");
+    HIPCHK(hipSetDevice(c->device));
+    c->nk = nk;
+    c->prob_h.resize(nk);
+    c->nd.resize(nk);
+    for (int i = 0; i < nk; ++i) {
+        fame_phases(c->lat, k + 3 * i, &c->prob_h[i]);
+        c->nd[i] = c->prob_h[i].gamma ? c->N - 1 : c->N;
+        if (nd_out) nd_out[i] = c->nd[i];
+    }
+    HIPCHK(hipMemcpy(c->prob_d, c->prob_h.data(), sizeof(ProbDev) * nk, hipMemcpyHostToDevice));
+    HIPCHK(fame_launch_sigma(c->g, c->prob_d, c->sig, c->sig_inv, nk, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FAME_OK;
+}
+
+/* ======================================================================== */
+/*  Operators                                                                */
+/* ======================================================================== */
+static inline int pix(int prec) { return prec == 8 ? 0 : 1; }
+
+KInner fame_ctx::kin(int prec, int p0, const int* act) const {
+    KInner a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = g; a.prob = prob_d + p0; a.plan = pa.dev[pix(prec)]; a.active = act;
+    a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
+    a.xstride = 2 * (size_t)N; a.fstride = 3 * (size_t)N;
+    a.L = Lin[pix(prec)]; a.pitch = pin[pix(prec)];
+    return a;
+}
+KAxis fame_ctx::kmid(int prec, int p0, const int* act) const {
+    KAxis a;
+    std::memset(&a, 0, sizeof(a));
+    a.g = g; a.prob = prob_d + p0; a.plan = pb.dev[pix(prec)]; a.active = act;
+    a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
+    a.fstride = 3 * (size_t)N; a.L = Lmid[pix(prec)]; a.pitch = pmid[pix(prec)];
+    return a;
+}
+KAxis fame_ctx::kout(int prec, int p0, const int* act) const {
+    KAxis a = kmid(prec, p0, act);
+    a.plan = pc.dev[pix(prec)];
+    a.invB = prec == 8 ? (const void*)invB64 : (const void*)invB32;
+    a.L = Lout[pix(prec)]; a.pitch = pout[pix(prec)];
+    return a;
+}
+
+/* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
+ * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
+fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec) {
+    KInner ki = kin(prec, p0, act);
+    ki.x = x; ki.pikind = 0;
+    HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
+    KAxis km = kmid(prec, p0, act);
+    HIPCHK(fame_launch_mid(km, +1, prec, nprob, stream));
+    KAxis ko = kout(prec, p0, act);
+    HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nprob, stream));
+    HIPCHK(fame_launch_mid(km, -1, prec, nprob, stream));
+    ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
+    HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
+    if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
+    return FAME_OK;
+}
+
+/* field (kernel layout, in F64) = Qr/Pr x  for problems p0.. */
+fame_status fame_ctx::qr_field(const void* x, int p0, int nprob, int pikind, const int* act) {
+    KInner ki = kin(8, p0, act);
+    ki.x = x; ki.pikind = pikind;
+    HIPCHK(fame_launch_inner(ki, 1, 8, nprob, stream));
+    KAxis km = kmid(8, p0, act);
+    HIPCHK(fame_launch_mid(km, +1, 8, nprob, stream));
+    KAxis ko = kout(8, p0, act);
+    HIPCHK(fame_launch_outer(ko, OUTER_QR, 8, nprob, stream));
+    return FAME_OK;
+}
+/* y (padded reduced) = Qrs/Prs of the field in F64 (kernel layout) */
+fame_status fame_ctx::qrs_field(void* y, int p0, int nprob, int pikind, const int* act) {
+    KAxis ko = kout(8, p0, act);
+    HIPCHK(fame_launch_outer(ko, OUTER_QRS, 8, nprob, stream));
+    KAxis km = kmid(8, p0, act);
+    HIPCHK(fame_launch_mid(km, -1, 8, nprob, stream));
+    KInner ki = kin(8, p0, act);
+    ki.y = y; ki.pikind = pikind;
+    HIPCHK(fame_launch_inner(ki, 0, 8, nprob, stream));
+    return FAME_OK;
+}
+
+/* copy a field between the kernel layout (F64 problem slots) and the reference layout */
+fame_status fame_ctx::field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride) {
+    if (g.general) {
+        HIPCHK(fame_launch_xz_transpose(fint, fref, g.nc, g.nb, g.na, +1, 3 * (size_t)N, 3, nprob, 8, stream));
+        (void)dst_stride;
+    } else {
+        HIPCHK(hipMemcpyAsync(fref, fint, 16 * 3 * (size_t)N * nprob, hipMemcpyDeviceToDevice, stream));
+    }
+    return FAME_OK;
+}
+fame_status fame_ctx::field_from_ref(const void* fref, void* fint, int nprob) {
+    if (g.general) {
+        HIPCHK(fame_launch_xz_transpose(fref, fint, g.nc, g.nb, g.na, -1, 3 * (size_t)N, 3, nprob, 8, stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(fint, fref, 16 * 3 * (size_t)N * nprob, hipMemcpyDeviceToDevice, stream));
+    }
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_apply(fame_ctx* c, int op, int ik, const void* x, void* y) {
+    if (!c || !x || !y) FAIL(FAME_ERR_ARG, "fame_apply: null argument");
+    if (ik < 0 || ik >= c->nk) FAIL(FAME_ERR_ARG, "fame_apply: wave vector index out of range");
+    if ((op == FAME_OP_QBQ || op == FAME_OP_INVAR) && !c->have_eps) FAIL(FAME_ERR_STATE, "fame_apply: permittivity not set");
+    HIPCHK(hipSetDevice(c->device));
+    const int N = c->N, Nd = c->nd[ik];
+    double2* xin = (double2*)c->ref64[0];
+    double2* yout = (double2*)c->ref64[1];
+    switch (op) {
+        case FAME_OP_QR:
+        case FAME_OP_PR:
+            HIPCHK(fame_launch_pack(x, xin, N, Nd, +1, 8, c->stream));
+            CHK(c->qr_field(xin, ik, 1, op == FAME_OP_PR ? 1 : 0, nullptr));
+            CHK(c->field_to_ref(c->F64 + (size_t)ik * 3 * N, y, 1, 3 * (size_t)N));
+            break;
+        case FAME_OP_QRS:
+        case FAME_OP_PRS:
+            CHK(c->field_from_ref(x, c->F64 + (size_t)ik * 3 * N, 1));
+            CHK(c->qrs_field(yout, ik, 1, op == FAME_OP_PRS ? 1 : 0, nullptr));
+            HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
+            break;
+        case FAME_OP_QBQ:
+            HIPCHK(fame_launch_pack(x, xin, N, Nd, +1, 8, c->stream));
+            /* the field slot used by qbq is that of problem ik */
+            CHK(c->qbq(8, xin, yout, ik, 1, nullptr, nullptr));
+            HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
+            break;
+        case FAME_OP_INVAR: {
+            HIPCHK(fame_launch_pack(x, xin, N, Nd, +1, 8, c->stream));
+            int it = 0;
+            fame_ls ls = c->ls_last.maxit > 0 ? c->ls_last : fame_ls{1000, 1e-12};
+            CHK(c->invar(FAME_PREC_F64, xin, yout, ik, 1, nullptr, ls, &it));
+            HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
+            break;
+        }
+        default: FAIL(FAME_ERR_ARG, "fame_apply: unknown operator");
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FAME_OK;
+}
+
+/* ======================================================================== */
+/*  CG (batched)                                                             */
+/* ======================================================================== */
+/* Solve QBQ x = b for problems p0..p0+nprob-1 (padded layout, precision prec).
+ * Stops per problem when <r,r> <= max(tol^2, reltol^2 <b,b>) or after maxit
+ * (reference counter semantics, CG.cu:36-78).  iters[] (host) receives k.    */
+fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
+                         const int* mask_dev, int* iters) {
+    void** V = prec == 8 ? cg64 : cg32;
+    const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
+    const size_t es = prec == 8 ? 16 : 8;
+    char* r = (char*)V[0] + es * st * p0;
+    char* p = (char*)V[1] + es * st * p0;
+    char* Ap = (char*)V[2] + es * st * p0;
+    const int nbi = nblk_in[pix(prec)];
+    HIPCHK(fame_cg_begin(x, r, p, b, partials, NBLK_VEC, n, st, nprob, prec, mask_dev, stream));
+    HIPCHK(fame_cg_init_scal2(cgsc, partials, NBLK_VEC, tol, reltol, maxit, nprob, active, mask_dev, stream));
+    int done_it = 0;
+    int chunk = std::max(1, std::min(maxit, cg_predict[pix(prec)] - 1));
+    while (true) {
+        for (int it = 0; it < chunk; ++it) {
+            CHK(qbq(prec, p, Ap, p0, nprob, active, p));
+            HIPCHK(fame_cg_alpha(cgsc, partials2, nbi, nprob, active, stream));
+            HIPCHK(fame_cg_update_xr(x, r, p, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, stream));
+            HIPCHK(fame_cg_beta(cgsc, partials, NBLK_VEC, nprob, active, stream));
+            HIPCHK(fame_cg_update_p(p, r, cgsc, n, st, nprob, prec, active, stream));
+        }
+        done_it += chunk;
+        HIPCHK(hipMemcpyAsync(active_h, active, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        int any = 0;
+        for (int i = 0; i < nprob; ++i) any |= active_h[i];
+        if (!any || done_it >= maxit) break;
+        chunk = 2;
+    }
+    std::vector<CgScal> sc(nprob);
+    HIPCHK(hipMemcpy(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost));
+    int mx = 0;
+    for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc[i].iters; mx = std::max(mx, sc[i].iters); }
+    cg_predict[pix(prec)] = std::max(2, mx - 1);
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_cg(fame_ctx* c, int ik, const void* rhs, void* x, int maxit, double tol, int* iters) {
+    if (!c || !rhs || !x) FAIL(FAME_ERR_ARG, "fame_cg: null argument");
+    if (ik < 0 || ik >= c->nk) FAIL(FAME_ERR_ARG, "fame_cg: wave vector index out of range");
+    if (!c->have_eps) FAIL(FAME_ERR_STATE, "fame_cg: permittivity not set");
+    HIPCHK(hipSetDevice(c->device));
+    const int N = c->N, Nd = c->nd[ik];
+    double2* b = (double2*)c->ref64[0] + (size_t)ik * 2 * N;
+    double2* y = (double2*)c->ref64[1] + (size_t)ik * 2 * N;
+    HIPCHK(fame_launch_pack(rhs, b, N, Nd, +1, 8, c->stream));
+    int it = 0;
+    CHK(c->cg(8, b, y, ik, 1, tol, 0.0, maxit, nullptr, &it));
+    HIPCHK(fame_launch_pack(y, x, N, Nd, -1, 8, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (iters) *iters = it;
+    return FAME_OK;
+}
+
+/* ======================================================================== */
+/*  invAr (batched): y = Sigma^-1 QBQ^-1 Sigma^-1 x                          */
+/* ======================================================================== */
+fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls,
+                            int* iters) {
+    const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
+    double2* b = (double2*)ref64[2] + st * p0;
+    HIPCHK(fame_scale_real_mod(b, x, sig_inv + (size_t)p0 * N, n, st, N, N, nprob, 8, stream));
+    std::vector<int> it(nprob, 0);
+    if (mode == FAME_PREC_F64) {
+        CHK(cg(8, b, y, p0, nprob, ls.tol, 0.0, ls.maxit, mask_dev, it.data()));
+    } else {
+        /* FP64 iterative refinement around FP32 CG solves */
+        double2* xr = (double2*)y;
+        double2* rr = (double2*)ref64[3] + st * p0;
+        char* r32 = (char*)cg32[3] + 8 * st * p0;
+        float2* d32 = (float2*)cg32[4] + st * p0;
+        HIPCHK(hipMemsetAsync(xr, 0, 16 * st * nprob, stream));
+        HIPCHK(hipMemcpyAsync(rr, b, 16 * st * nprob, hipMemcpyDeviceToDevice, stream));
+        std::vector<int> msk(nprob, 1);
+        if (mask_dev) HIPCHK(hipMemcpy(msk.data(), mask_dev, sizeof(int) * nprob, hipMemcpyDeviceToHost));
+        double tol2 = ls.tol * ls.tol;
+        for (int sweep = 0; sweep < refine_max_sweeps; ++sweep) {
+            HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));
+            HIPCHK(fame_reduce_partials(partials, NBLK_VEC, 1, scal, nprob, stream));
+            std::vector<double> rn(nprob);
+            HIPCHK(hipMemcpyAsync(rn.data(), scal, sizeof(double) * nprob, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            int any = 0;
+            for (int i = 0; i < nprob; ++i) { msk[i] = msk[i] && (rn[i] > tol2); any |= msk[i]; }
+            if (!any) break;
+            HIPCHK(hipMemcpyAsync(mask, msk.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, stream));
+            HIPCHK(fame_convert(r32, 4, rr, 8, n, st, st, nprob, stream));
+            std::vector<int> it32(nprob, 0);
+            CHK(cg(4, r32, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data()));
+            for (int i = 0; i < nprob; ++i) if (msk[i]) it[i] += it32[i] - 1;
+            HIPCHK(fame_axpy_mixed(xr, d32, 4, 1.0, n, st, st, nprob, mask, stream));
+            /* rr = b - QBQ64 xr */
+            double2* t = (double2*)ref64[4] + st * p0;
+            CHK(qbq(8, xr, t, p0, nprob, mask, nullptr));
+            HIPCHK(fame_sub(rr, b, t, n, st, nprob, mask, stream));
+        }
+        for (int i = 0; i < nprob; ++i) it[i] += 1;
+    }
+    HIPCHK(fame_scale_real_mod(y, y, sig_inv + (size_t)p0 * N, n, st, N, N, nprob, 8, stream));
+    if (iters) for (int i = 0; i < nprob; ++i) iters[i] = it[i];
+    return FAME_OK;
+}

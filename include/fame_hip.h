@@ -1,0 +1,147 @@
+/*
+ * fame_hip.h -- C-ABI of the MI355X (gfx950) null-space-free Maxwell band solver.
+ *
+ * Plain C types only (pointers, sizes, status codes); no torch or HIP types in
+ * the signatures.  Vector arguments named *_dev are device pointers; layouts are
+ * the reference's:
+ *   reduced vectors  [x1; x2], 2*Nd complex doubles (interleaved re,im),
+ *                    mode i <-> Fourier index N-Nd+i
+ *   fields           [Ex; Ey; Ez], 3*N complex doubles, each x-fastest
+ *   permittivity     3*N doubles on the Yee edges, same layout (B_eps)
+ *
+ * Every entry point below replaces one reference function of the
+ * FAME_Main_Code -> Lanczos_Isotropic -> invAr -> CG -> QBQ -> Qr/Qrs path; the
+ * reference interface is cited next to it (file:line under the reference tree).
+ * No entry point aborts: errors are returned as fame_status, and
+ * fame_last_error() gives the message.
+ */
+#ifndef FAME_HIP_H
+#define FAME_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    FAME_OK = 0,
+    FAME_ERR_ARG = 1,          /* invalid argument or size                       */
+    FAME_ERR_HIP = 2,          /* HIP runtime error (message via fame_last_error) */
+    FAME_ERR_NOMEM = 3,        /* device allocation failed                       */
+    FAME_ERR_STATE = 4,        /* call order (e.g. eigs before set_eps)          */
+    FAME_ERR_UNSUPPORTED = 5   /* e.g. FFT length beyond the supported maximum   */
+} fame_status;
+
+/* Discretisation of one structure.  Fields mirror PAR.mesh / PAR.lattice of
+ * include/FAME_Internal_Common.h:62-127 (grid_nums, mesh_len, lattice_vec_a,
+ * lattice_constant.m1..m4, t1..t4); `general` is the Simple/General dispatch of
+ * FAME_Main/FAME_Main_Code.cu:156-171.                                          */
+typedef struct {
+    int    grid_nums[3];
+    int    general;
+    double mesh_len[3];
+    double lattice_vec_a[9];
+    int    m1, m2, m3, m4;
+    double t1[3], t2[3], t3[3], t4[3];
+} fame_lattice;
+
+/* ES / LS of include/FAME_Internal_Common.h:160-170 */
+typedef struct { int nwant, nstep, maxit; double tol; } fame_es;
+typedef struct { int maxit; double tol; } fame_ls;
+
+/* PROFILE of include/FAME_Internal_Common.h:205-211, per wave vector, plus the
+ * counters of the mixed-precision solver                                        */
+typedef struct {
+    int    ls_iter;        /* CG iterations summed over all invAr (reference counting) */
+    int    es_iter;        /* invAr applies                                          */
+    int    restarts;       /* Lanczos_Isotropic return value                         */
+    int    nd;             /* Nd (N-1 at Gamma)                                      */
+    int    converged;      /* 1 if the Lanczos convergence test passed               */
+    int    qbq64;          /* FP64 QBQ applies                                       */
+    int    qbq32;          /* FP32 QBQ applies (refinement inner solves)             */
+    int    pad;
+    double es_time;        /* seconds, eigensolve of the whole batch                 */
+} fame_profile;
+
+enum { FAME_PREC_F64 = 0, FAME_PREC_F32_REFINE = 1 };
+
+enum {
+    FAME_OP_QR = 0,    /* FAME_Main/FAME_Matrix_Vector_Production_Qr.cu:8,43     2Nd -> 3N */
+    FAME_OP_QRS = 1,   /* FAME_Main/FAME_Matrix_Vector_Production_Qrs.cu:7,31    3N -> 2Nd */
+    FAME_OP_PR = 2,    /* FAME_Main/FAME_Matrix_Vector_Production_Pr.cu:11,45    2Nd -> 3N */
+    FAME_OP_PRS = 3,   /* FAME_Main/FAME_Matrix_Vector_Production_Prs.cu:9,33    3N -> 2Nd */
+    FAME_OP_QBQ = 4,   /* FAME_Main/FAME_Matrix_Vector_Production_Isotropic_QBQ.cu:8,38  2Nd -> 2Nd */
+    FAME_OP_INVAR = 5  /* FAME_Main/FAME_Matrix_Vector_Production_Isotropic_invAr.cu:8   2Nd -> 2Nd */
+};
+
+typedef struct fame_ctx fame_ctx;
+
+/* Create a solver context for one structure and up to max_batch wave vectors.
+ * Replaces FAME_Create_cublas + FAME_Create_Buffer (FAME_Main/FAME_Create_cublas.cu:4,
+ * FAME_Main/FAME_Create_Buffer.cu:4): FFT plans, twiddle tables, work buffers. */
+fame_status fame_create(const fame_lattice* lat, int max_batch, int device, fame_ctx** ctx);
+/* Replaces FAME_Destroy_Main (FAME_Main/FAME_Destroy_Main.cu:4) */
+fame_status fame_destroy(fame_ctx* ctx);
+
+/* Permittivity on the Yee edges, host array of 3N doubles (B_eps layout).
+ * Replaces FAME_Matrix_B_Isotropic (FAME_Main/FAME_Matrix_B_Isotropic.cu:4).     */
+fame_status fame_set_eps(fame_ctx* ctx, const double* eps3N);
+
+/* Wave vectors (host, 3*nk doubles, units of 2*pi as PAR.recip_lattice.WaveVector).
+ * nd_out (host, nk ints, may be NULL) receives Nd per wave vector.
+ * Replaces FAME_Matrix_Lambdas (FAME_Main/FAME_Matrix_Lambdas.cu:40): the Lambda,
+ * Pi and D tables are evaluated on the fly by the kernels from these phases.   */
+fame_status fame_set_wave_vectors(fame_ctx* ctx, int nk, const double* k, int* nd_out);
+
+/* Apply one operator for wave vector ik on device vectors (reference layouts). */
+fame_status fame_apply(fame_ctx* ctx, int op, int ik, const void* x_dev, void* y_dev);
+
+/* CG on QBQ y = rhs for wave vector ik (FAME_Main/CG.cu:5,90): x0 = 0, absolute
+ * tolerance, returns the reference's iteration counter k in *iters.           */
+fame_status fame_cg(fame_ctx* ctx, int ik, const void* rhs_dev, void* x_dev, int maxit, double tol, int* iters);
+
+/* Restarted Lanczos on A_r^{-1} for all wave vectors of the context in one
+ * batch, then eigenvector restoration and the Gamma rule.
+ * Replaces FAME_Fast_Algorithms_Isotropic + Lanczos_Isotropic
+ * (FAME_Main/FAME_Fast_Algorithms_Isotropic.cu:20, FAME_Main/Lanczos_Isotropic.cu:7).
+ *   precision : FAME_PREC_F64 (reference arithmetic) or FAME_PREC_F32_REFINE
+ *               (complex FP32 FFT/CG inside FP64 iterative refinement)
+ *   freq      : host, nk*nwant doubles, omega = sqrt(1/theta) (0 for Gamma zero modes)
+ *   evr_dev   : optional device buffer of nk*nwant*2N complex doubles receiving the
+ *               Ritz vectors of A_r (band b of wave vector k at offset (k*nwant+b)*2N,
+ *               2Nd values in the reference layout, Lanczos order); may be NULL
+ *   E_dev     : optional device buffer of nk*nwant*3N complex doubles receiving the
+ *               restored E fields (reference scaling and Gamma rule); may be NULL
+ *   prof      : optional host array of nk profiles                              */
+fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fame_ls* ls, double* freq,
+                      void* evr_dev, void* E_dev, fame_profile* prof);
+
+/* Relative residuals ||C^H C E - omega^2 B E|| / ||E|| of fields produced by
+ * fame_eigs (device, nk*nwant*3N) -> res (host, nk*nwant).  GPU counterpart of
+ * Check_Residual (FAME_Main/FAME_Main_Code.cu:383), reported relative.       */
+fame_status fame_residuals(fame_ctx* ctx, int nwant, const double* freq, const void* ev_dev, double* res);
+
+/* Vector-Jacobian product of the eigenvalues lambda_kj = omega_kj^2 with respect
+ * to the permittivity of every Yee edge, by Hellmann-Feynman:
+ *   grad_e = sum_kj w_kj d lambda_kj / d eps_e,
+ *   d lambda / d eps_e = -lambda |E_e|^2 / (E^H B E)
+ * for fields E_dev as produced by fame_eigs (nk*nwant*3N).  Inside a degenerate
+ * cluster (relative gap <= 1e-8) the weights are replaced by their mean, i.e. the
+ * derivative of the cluster trace.  weights: host nk*nwant; grad_dev: device 3N
+ * doubles (reference layout), overwritten.  No reference counterpart: this is
+ * the adjoint the band solver adds for inverse design.                        */
+fame_status fame_eig_grad(fame_ctx* ctx, int nwant, const double* freq, const void* E_dev, const double* weights,
+                          void* grad_dev);
+
+/* Stream used by all launches (a hipStream_t passed as void*; NULL = default). */
+fame_status fame_set_stream(fame_ctx* ctx, void* stream);
+
+/* Problem sizes: N and the Stockham/Bluestein lengths of the three axes. */
+fame_status fame_query(const fame_ctx* ctx, int* N, int* nk, int* fft_len3);
+
+/* Message of the last error on this thread. */
+const char* fame_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
