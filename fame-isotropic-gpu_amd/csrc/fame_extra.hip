@@ -39,13 +39,11 @@ __global__ void __launch_bounds__(256) k_curlcurl(KInner args) {
     const ProbDev pr = args.prob[p];
     const FftPlanDev& pl = args.plan;
     const int na = g.na, L = args.L, pitch = args.pitch;
-    T2* Wl = reinterpret_cast<T2*>(smem_raw);
-    T2* A = Wl + pl.M;
+    T2* tbase = reinterpret_cast<T2*>(smem_raw);
+    PlanLds<T2> tb = load_plan_tables<T2>(pl, tbase);
+    T2* A = tbase + plan_table_len(pl);
     T2* B = A + 3 * L * pitch;
-    {
-        const T2* s = reinterpret_cast<const T2*>(pl.W);
-        for (int i = threadIdx.x; i < pl.M; i += blockDim.x) Wl[i] = s[i];
-    }
+    __syncthreads();
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
     const size_t N = (size_t)g.N;
@@ -61,7 +59,7 @@ __global__ void __launch_bounds__(256) k_curlcurl(KInner args) {
         for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
     }
     __syncthreads();
-    T2* R = fft_lines<T2, double>(A, B, 3 * L, pitch, pl, Wl, -1);
+    T2* R = fft_lines<T2, double>(A, B, 3 * L, pitch, pl, tb, -1);
     for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
         const int line = idx / na, a = idx - line * na, gl = l0 + line;
         if (gl >= nlines) continue;
@@ -83,7 +81,7 @@ __global__ void __launch_bounds__(256) k_curlcurl(KInner args) {
     }
     __syncthreads();
     T2* S = (R == A) ? B : A;
-    R = fft_lines<T2, double>(R, S, 3 * L, pitch, pl, Wl, +1);
+    R = fft_lines<T2, double>(R, S, 3 * L, pitch, pl, tb, +1);
     for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
         const int line = idx / na, a = idx - line * na, gl = l0 + line;
         if (gl >= nlines) continue;
@@ -172,7 +170,7 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
             fame_launch_outer(ko, OUTER_QRS, 8, nk, c->stream) != hipSuccess ||
             fame_launch_mid(km, -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
         dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, nk);
-        size_t lds = 16 * (ki.plan.M + 2 * 3 * ki.L * ki.pitch);
+        size_t lds = 16 * ((size_t)plan_table_len(ki.plan) + 2 * 3 * (size_t)ki.L * ki.pitch);
         hipLaunchKernelGGL(k_curlcurl, grid, dim3(256), lds, c->stream, ki);
         if (hipGetLastError() != hipSuccess || fame_launch_mid(km, +1, 8, nk, c->stream) != hipSuccess ||
             fame_launch_outer(ko, OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }

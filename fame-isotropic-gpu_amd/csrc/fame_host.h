@@ -57,6 +57,7 @@ struct fame_ctx {
     double refine_reltol = 1e-6;
     fame_ls ls_last = {0, 0.0};
     long long cnt_qbq64 = 0, cnt_qbq32 = 0;
+    int last_iters = 0;
 
     KInner kin(int prec, int p0, const int* act) const;
     KAxis kmid(int prec, int p0, const int* act) const;

@@ -29,17 +29,13 @@ template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return
 __device__ __forceinline__ double2 to_d(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
 
-template <typename T2>
-__device__ __forceinline__ void load_table(T2* dst, const void* src, int n) {
-    const T2* s = reinterpret_cast<const T2*>(src);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = s[i];
-}
+#define MAX_LINES 32
 
 /* ------------------------------------------------------------------------- */
 /*  P1 / P5: inner axis                                                       */
 /* ------------------------------------------------------------------------- */
-/* expand = 1 : Y(field) = twid .* IFFT_a( Pi x )          (Qr / Pr first stage)
- * expand = 0 : y(reduced) = Pi^H FFT_a( conj(twid) .* F ) (Qrs / Prs last stage)
+/* EXPAND = 1 : F = twid .* IFFT_a( Pi x )          (Qr / Pr first stage)
+ * EXPAND = 0 : y = Pi^H FFT_a( conj(twid) .* F )   (Qrs / Prs last stage)
  * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
  *              (FAME_Matrix_Lambdas.cu:217-243)                                 */
 template <typename T, int EXPAND>
@@ -48,33 +44,41 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
     const int p = blockIdx.y;
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ LineSym lsym[MAX_LINES];
     const GridDev g = args.g;
     const ProbDev pr = args.prob[p];
     const FftPlanDev& pl = args.plan;
     const int na = g.na, L = args.L, pitch = args.pitch;
-    T2* Wl = reinterpret_cast<T2*>(smem_raw);
-    T2* A = Wl + pl.M;
+    T2* base = reinterpret_cast<T2*>(smem_raw);
+    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
+    T2* A = base + plan_table_len(pl);
     T2* B = A + 3 * L * pitch;
-    load_table<T2>(Wl, pl.W, pl.M);
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
+    if ((int)threadIdx.x < L) {
+        const int gl = l0 + threadIdx.x;
+        if (gl < nlines) {
+            const int cf = fdiv(gl, g.m_nb), bf = gl - cf * g.nb;
+            lsym[threadIdx.x] = line_sym(g, pr, bf, cf);
+        }
+    }
+    __syncthreads();
     const size_t N = (size_t)g.N;
     const T2* X = reinterpret_cast<const T2*>(args.x) + (size_t)p * args.xstride;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
+    const double inv_na = 1.0 / (double)na;
 
     if (EXPAND) {
         for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = idx / na, a = idx - line * na;
+            const int line = fdiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
             if (gl < nlines) {
                 const size_t f = (size_t)gl * na + a;
                 if (!(pr.gamma && f == 0)) {
-                    const int bf = gl % g.nb, cf = gl / g.nb;
                     double2 lx, ly, lz, p1[3], p2[3];
-                    double lq;
-                    lambdas(g, pr, a, bf, cf, lx, ly, lz);
-                    pi_basis(g, lx, ly, lz, p1, p2, lq);
+                    lambdas_line(g, lsym[line], a, lx, ly, lz);
+                    pi_basis(g, lx, ly, lz, p1, p2);
                     const double2 x1 = to_d(X[f]), x2 = to_d(X[N + f]);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
@@ -87,27 +91,23 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
             for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = from_d<T2>(u[c]);
         }
         __syncthreads();
-        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, Wl, +1);
+        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, tb, +1);
         for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = idx / na, a = idx - line * na;
+            const int line = fdiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
-            const int bf = gl % g.nb, cf = gl / g.nb;
-            const double ph = (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a;
-            const T2 w = from_d<T2>(expipi(2.0 * ph / (double)na));
+            const T2 w = from_d<T2>(expipi(2.0 * lsym[line].phi * (double)a * inv_na));
             const size_t f = (size_t)gl * na + a;
 #pragma unroll
             for (int c = 0; c < 3; ++c) F[c * N + f] = cmul(R[(c * L + line) * pitch + a], w);
         }
     } else {
         for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = idx / na, a = idx - line * na;
+            const int line = fdiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             T2 v[3];
             if (gl < nlines) {
-                const int bf = gl % g.nb, cf = gl / g.nb;
-                const double ph = (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a;
-                const T2 w = from_d<T2>(expipi(2.0 * ph / (double)na));
+                const T2 w = from_d<T2>(expipi(2.0 * lsym[line].phi * (double)a * inv_na));
                 const size_t f = (size_t)gl * na + a;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + f], w);
@@ -119,22 +119,20 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
             for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
         }
         __syncthreads();
-        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, Wl, -1);
+        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, tb, -1);
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0;
         for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = idx / na, a = idx - line * na;
+            const int line = fdiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
             const size_t f = (size_t)gl * na + a;
             double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
             if (!(pr.gamma && f == 0)) {
-                const int bf = gl % g.nb, cf = gl / g.nb;
                 double2 lx, ly, lz, p1[3], p2[3];
-                double lq;
-                lambdas(g, pr, a, bf, cf, lx, ly, lz);
-                pi_basis(g, lx, ly, lz, p1, p2, lq);
+                lambdas_line(g, lsym[line], a, lx, ly, lz);
+                pi_basis(g, lx, ly, lz, p1, p2);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double2 w = to_d(R[(c * L + line) * pitch + a]);
@@ -181,41 +179,41 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
 /* ------------------------------------------------------------------------- */
 /*  P2 / P4: middle axis, per component                                      */
 /* ------------------------------------------------------------------------- */
-/* dir = +1: F = twid_b .* IFFT_b(F) ; dir = -1: F = FFT_b(conj(twid_b) .* F)   */
+/* DIR = +1: F = twid_b .* IFFT_b(F) ; DIR = -1: F = FFT_b(conj(twid_b) .* F)   */
 template <typename T, int DIR>
 __global__ void __launch_bounds__(256) k_mid(KAxis args) {
     typedef typename Vec<T>::T2 T2;
-    const int p = blockIdx.y / 3, comp = blockIdx.y % 3;
+    const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const GridDev g = args.g;
     const ProbDev pr = args.prob[p];
     const FftPlanDev& pl = args.plan;
     const int na = g.na, nb = g.nb, TA = args.L, pitch = args.pitch;
-    T2* Wl = reinterpret_cast<T2*>(smem_raw);
-    T2* A = Wl + pl.M;
+    T2* base = reinterpret_cast<T2*>(smem_raw);
+    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
+    T2* A = base + plan_table_len(pl);
     T2* B = A + TA * pitch;
-    load_table<T2>(Wl, pl.W, pl.M);
     const int ntile = (na + TA - 1) / TA;
-    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x % ntile) * TA;
+    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x - cf * ntile) * TA;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
-    const double phc = pr.s0 + pr.s1 * (double)cf;
+    const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)nb;
     for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
-        const int ta = idx % TA, b = idx / TA, a = a0 + ta;
+        const int b = idx / TA, ta = idx - b * TA, a = a0 + ta;
         T2 v;
         if (a < na) {
             v = F[(size_t)b * na + a];
-            if (DIR < 0) v = cmulc(v, from_d<T2>(expipi(2.0 * phc * (double)b / (double)nb)));
+            if (DIR < 0) v = cmulc(v, from_d<T2>(expipi(phc * (double)b)));
         } else { v.x = 0; v.y = 0; }
         A[ta * pitch + b] = v;
     }
     __syncthreads();
-    T2* R = fft_lines<T2, T>(A, B, TA, pitch, pl, Wl, DIR > 0 ? +1 : -1);
+    T2* R = fft_lines<T2, T>(A, B, TA, pitch, pl, tb, DIR > 0 ? +1 : -1);
     for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
-        const int ta = idx % TA, b = idx / TA, a = a0 + ta;
+        const int b = idx / TA, ta = idx - b * TA, a = a0 + ta;
         if (a >= na) continue;
         T2 v = R[ta * pitch + b];
-        if (DIR > 0) v = cmul(v, from_d<T2>(expipi(2.0 * phc * (double)b / (double)nb)));
+        if (DIR > 0) v = cmul(v, from_d<T2>(expipi(phc * (double)b)));
         F[(size_t)b * na + a] = v;
     }
 }
@@ -229,7 +227,7 @@ __global__ void __launch_bounds__(256) k_mid(KAxis args) {
 template <typename T, int MODE>
 __global__ void __launch_bounds__(256) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
-    const int p = blockIdx.y / 3, comp = blockIdx.y % 3;
+    const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const GridDev g = args.g;
@@ -237,14 +235,14 @@ __global__ void __launch_bounds__(256) k_outer(KAxis args) {
     const FftPlanDev& pl = args.plan;
     const int nc = g.nc, TQ = args.L, pitch = args.pitch;
     const int plane = g.na * g.nb;
-    T2* Wl = reinterpret_cast<T2*>(smem_raw);
-    T2* A = Wl + pl.M;
+    T2* base = reinterpret_cast<T2*>(smem_raw);
+    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
+    T2* A = base + plan_table_len(pl);
     T2* B = A + TQ * pitch;
-    load_table<T2>(Wl, pl.W, pl.M);
     const int q0 = blockIdx.x * TQ;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
     for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-        const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+        const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
         T2 v;
         if (q < plane) {
             v = F[(size_t)c * plane + q];
@@ -254,34 +252,34 @@ __global__ void __launch_bounds__(256) k_outer(KAxis args) {
     }
     __syncthreads();
     if (MODE == OUTER_EPS) {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, +1);
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, +1);
         T2* S = (R == A) ? B : A;
         const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
         const T invN = (T)(1.0 / (double)g.N);
         for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
             T s = (q < plane) ? invB[(size_t)c * plane + q] * invN : (T)0;
             R[tq * pitch + c] = cscale(R[tq * pitch + c], s);
         }
         __syncthreads();
-        R = fft_lines<T2, T>(R, S, TQ, pitch, pl, Wl, -1);
+        R = fft_lines<T2, T>(R, S, TQ, pitch, pl, tb, -1);
         for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
             if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
         }
     } else if (MODE == OUTER_QR) {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, +1);
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, +1);
         const double invN = 1.0 / (double)g.N;
         for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
             if (q >= plane) continue;
             double2 w = expipi(2.0 * pr.r0 * (double)c / (double)nc);
             F[(size_t)c * plane + q] = cmul(R[tq * pitch + c], from_d<T2>(cscale(w, invN)));
         }
     } else {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, Wl, -1);
+        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, -1);
         for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int tq = idx % TQ, c = idx / TQ, q = q0 + tq;
+            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
             if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
         }
     }
@@ -303,7 +301,6 @@ __global__ void k_xz_transpose(const T2* __restrict__ in, T2* __restrict__ out, 
     const size_t N = (size_t)nx * ny * nz;
     const T2* I = in + p * stride + comp * N;
     T2* O = out + p * stride + comp * N;
-    /* threads: 32 x 8 */
     const int lx = threadIdx.x, ly = threadIdx.y;
     if (dir > 0) {
         for (int r = ly; r < 32; r += 8) {
@@ -344,6 +341,27 @@ __global__ void k_pack(const T2* __restrict__ in, T2* __restrict__ out, int N, i
     }
 }
 
+/* Sigma_r = |Lambda| per Fourier mode and its inverse (0 for the removed Gamma
+ * mode), FAME_Matrix_Lambdas.cu:193-205 / :442-454, duplicated over [x1;x2] by
+ * the callers through a modulo-N index.                                       */
+__global__ void k_sigma(GridDev g, const ProbDev* prob, double* sig, double* sig_inv) {
+    const int p = blockIdx.y;
+    const ProbDev pr = prob[p];
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < g.N; f += gridDim.x * blockDim.x) {
+        const int a = f % g.na, bc = f / g.na, b = bc % g.nb, c = bc / g.nb;
+        double s = 0.0, si = 0.0;
+        if (!(pr.gamma && f == 0)) {
+            double2 lx, ly, lz;
+            lambdas(g, pr, a, b, c, lx, ly, lz);
+            double Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
+            s = sqrt(Lq);
+            si = 1.0 / s;
+        }
+        sig[(size_t)p * g.N + f] = s;
+        sig_inv[(size_t)p * g.N + f] = si;
+    }
+}
+
 }  // namespace fame
 
 /* ========================================================================= */
@@ -352,10 +370,15 @@ __global__ void k_pack(const T2* __restrict__ in, T2* __restrict__ out, int N, i
 using namespace fame;
 
 template <typename T>
+static size_t lds_bytes(const FftPlanDev& pl, int nbuf_lines, int pitch) {
+    return sizeof(typename Vec<T>::T2) * ((size_t)plan_table_len(pl) + 2 * (size_t)nbuf_lines * pitch);
+}
+
+template <typename T>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
     dim3 grid((nlines + a.L - 1) / a.L, nprob);
-    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * 3 * a.L * a.pitch);
+    size_t lds = lds_bytes<T>(a.plan, 3 * a.L, a.pitch);
     if (expand) hipLaunchKernelGGL((k_inner<T, 1>), grid, dim3(256), lds, s, a);
     else hipLaunchKernelGGL((k_inner<T, 0>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
@@ -365,7 +388,7 @@ template <typename T>
 static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) {
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(ntile * a.g.nc, 3 * nprob);
-    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * a.L * a.pitch);
+    size_t lds = lds_bytes<T>(a.plan, a.L, a.pitch);
     if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1>), grid, dim3(256), lds, s, a);
     else hipLaunchKernelGGL((k_mid<T, -1>), grid, dim3(256), lds, s, a);
     return hipGetLastError();
@@ -375,7 +398,7 @@ template <typename T>
 static hipError_t launch_outer(const KAxis& a, int mode, int nprob, hipStream_t s) {
     const int plane = a.g.na * a.g.nb;
     dim3 grid((plane + a.L - 1) / a.L, 3 * nprob);
-    size_t lds = sizeof(typename Vec<T>::T2) * (a.plan.M + 2 * a.L * a.pitch);
+    size_t lds = lds_bytes<T>(a.plan, a.L, a.pitch);
     switch (mode) {
         case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS>), grid, dim3(256), lds, s, a); break;
         case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR>), grid, dim3(256), lds, s, a); break;
@@ -410,27 +433,6 @@ hipError_t fame_launch_pack(const void* in, void* out, int N, int Nd, int dir, i
     if (prec == 8) hipLaunchKernelGGL(k_pack<double2>, grid, dim3(256), 0, s, (const double2*)in, (double2*)out, N, Nd, dir);
     else hipLaunchKernelGGL(k_pack<float2>, grid, dim3(256), 0, s, (const float2*)in, (float2*)out, N, Nd, dir);
     return hipGetLastError();
-}
-
-/* Sigma_r = |Lambda| per Fourier mode and its inverse (0 for the removed Gamma
- * mode), FAME_Matrix_Lambdas.cu:193-205 / :442-454, duplicated over [x1;x2] by
- * the callers through a modulo-N index.                                       */
-__global__ void k_sigma(GridDev g, const ProbDev* prob, double* sig, double* sig_inv) {
-    const int p = blockIdx.y;
-    const ProbDev pr = prob[p];
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < g.N; f += gridDim.x * blockDim.x) {
-        const int a = f % g.na, bc = f / g.na, b = bc % g.nb, c = bc / g.nb;
-        double s = 0.0, si = 0.0;
-        if (!(pr.gamma && f == 0)) {
-            double2 lx, ly, lz;
-            lambdas(g, pr, a, b, c, lx, ly, lz);
-            double Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
-            s = sqrt(Lq);
-            si = 1.0 / s;
-        }
-        sig[(size_t)p * g.N + f] = s;
-        sig_inv[(size_t)p * g.N + f] = si;
-    }
 }
 
 hipError_t fame_launch_sigma(const GridDev& g, const ProbDev* prob, double* sig, double* sig_inv, int nprob, hipStream_t s) {

@@ -130,12 +130,20 @@ fame_status AxisPlan::build(int n_) {
         *dst = d;
         return FAME_OK;
     };
+    auto magic = [](int d) -> unsigned long long { return ((1ULL << 32) + (unsigned long long)d - 1) / (unsigned long long)d; };
     for (int pi = 0; pi < 2; ++pi) {
         int prec = pi == 0 ? 8 : 4;
         FftPlanDev& d = dev[pi];
         std::memset(&d, 0, sizeof(d));
         d.n = n; d.M = M; d.blue = blue; d.nrad = (int)radices.size();
-        for (int i = 0; i < d.nrad; ++i) d.rad[i] = radices[i];
+        d.mM = magic(M); d.mN = magic(n);
+        int Ns = 1;
+        for (int i = 0; i < d.nrad; ++i) {
+            PassDev& ps = d.pass[i];
+            ps.R = radices[i]; ps.Ns = Ns; ps.nb = M / ps.R; ps.tstep = M / (Ns * ps.R);
+            ps.mNs = magic(Ns); ps.mNb = magic(ps.nb);
+            Ns *= ps.R;
+        }
         CHK(upload(W, prec, &d.W));
         CHK(upload(ch, prec, &d.chirp));
         CHK(upload(bm, prec, &d.bhat_m));
@@ -144,12 +152,14 @@ fame_status AxisPlan::build(int n_) {
     return FAME_OK;
 }
 
-/* lines per block so that the kernel's LDS stays within 64 KiB */
+/* lines per block so that the kernel's LDS (plan tables + two line buffers)
+ * stays within 56 KiB, leaving room for the static reduction scratch */
 static void pick_tile(const AxisPlan& p, int ncomp, int prec, int maxL, int align, int* L, int* pitch) {
     const int es = prec == 8 ? 16 : 8;
-    const int budget = 57344 / es;   /* 56 KiB dynamic + static reduction scratch <= 64 KiB */
+    const int budget = 57344 / es;
+    const int tables = plan_table_len(p.dev[prec == 8 ? 0 : 1]);
     int pt = p.M;
-    int l = (budget - p.M) / (2 * ncomp * pt);
+    int l = (budget - tables) / (2 * ncomp * pt);
     if (l > maxL) l = maxL;
     if (align > 1 && l >= align) l = (l / align) * align;
     if (l < 1) l = 0;
@@ -191,6 +201,9 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     if (g.general) { g.na = Nz; g.nb = Ny; g.nc = Nx; g.ha = lat->mesh_len[2]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[0]; }
     else { g.na = Nx; g.nb = Ny; g.nc = Nz; g.ha = lat->mesh_len[0]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[2]; }
     g.N = Nx * Ny * Nz;
+    g.iha = 1.0 / g.ha; g.ihb = 1.0 / g.hb; g.ihc = 1.0 / g.hc;
+    g.m_na = ((1ULL << 32) + g.na - 1) / (unsigned long long)g.na;
+    g.m_nb = ((1ULL << 32) + g.nb - 1) / (unsigned long long)g.nb;
     c->N = g.N;
     fame_status st;
     if ((st = c->pa.build(g.na)) != FAME_OK || (st = c->pb.build(g.nb)) != FAME_OK || (st = c->pc.build(g.nc)) != FAME_OK) {
@@ -426,7 +439,8 @@ fame_status fame_ctx::field_from_ref(const void* fref, void* fint, int nprob) {
 extern "C" fame_status fame_apply(fame_ctx* c, int op, int ik, const void* x, void* y) {
     if (!c || !x || !y) FAIL(FAME_ERR_ARG, "fame_apply: null argument");
     if (ik < 0 || ik >= c->nk) FAIL(FAME_ERR_ARG, "fame_apply: wave vector index out of range");
-    if ((op == FAME_OP_QBQ || op == FAME_OP_INVAR) && !c->have_eps) FAIL(FAME_ERR_STATE, "fame_apply: permittivity not set");
+    if ((op == FAME_OP_QBQ || op == FAME_OP_INVAR || op == FAME_OP_INVAR_MIXED || op == FAME_OP_QBQ32) && !c->have_eps)
+        FAIL(FAME_ERR_STATE, "fame_apply: permittivity not set");
     HIPCHK(hipSetDevice(c->device));
     const int N = c->N, Nd = c->nd[ik];
     double2* xin = (double2*)c->ref64[0];
@@ -450,12 +464,25 @@ extern "C" fame_status fame_apply(fame_ctx* c, int op, int ik, const void* x, vo
             CHK(c->qbq(8, xin, yout, ik, 1, nullptr, nullptr));
             HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
             break;
-        case FAME_OP_INVAR: {
+        case FAME_OP_QBQ32: {
+            /* FP32 QBQ (the inner operator of the mixed-precision solver), FP64 in/out */
+            float2* x32 = (float2*)c->cg32[3] + (size_t)ik * 2 * N;
+            float2* y32 = (float2*)c->cg32[4] + (size_t)ik * 2 * N;
+            HIPCHK(fame_launch_pack(x, xin, N, Nd, +1, 8, c->stream));
+            HIPCHK(fame_convert(x32, 4, xin, 8, 2 * (size_t)N, 0, 0, 1, c->stream));
+            CHK(c->qbq(4, x32, y32, ik, 1, nullptr, nullptr));
+            HIPCHK(fame_convert(yout, 8, y32, 4, 2 * (size_t)N, 0, 0, 1, c->stream));
+            HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
+            break;
+        }
+        case FAME_OP_INVAR:
+        case FAME_OP_INVAR_MIXED: {
             HIPCHK(fame_launch_pack(x, xin, N, Nd, +1, 8, c->stream));
             int it = 0;
             fame_ls ls = c->ls_last.maxit > 0 ? c->ls_last : fame_ls{1000, 1e-12};
-            CHK(c->invar(FAME_PREC_F64, xin, yout, ik, 1, nullptr, ls, &it));
+            CHK(c->invar(op == FAME_OP_INVAR ? FAME_PREC_F64 : FAME_PREC_F32_REFINE, xin, yout, ik, 1, nullptr, ls, &it));
             HIPCHK(fame_launch_pack(yout, y, N, Nd, -1, 8, c->stream));
+            c->last_iters = it;
             break;
         }
         default: FAIL(FAME_ERR_ARG, "fame_apply: unknown operator");

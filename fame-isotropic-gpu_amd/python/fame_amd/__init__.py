@@ -356,3 +356,5 @@ class Solver:
         _check(lib().fame_eig_grad(self._ctx, nwant, _ptr(f), C.c_void_p(E.data_ptr()), _ptr(w),
                                    C.c_void_p(g.data_ptr())), "fame_eig_grad")
         return g
+
+OP_INVAR_MIXED, OP_QBQ32 = 6, 7

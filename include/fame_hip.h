@@ -70,7 +70,9 @@ enum {
     FAME_OP_PR = 2,    /* FAME_Main/FAME_Matrix_Vector_Production_Pr.cu:11,45    2Nd -> 3N */
     FAME_OP_PRS = 3,   /* FAME_Main/FAME_Matrix_Vector_Production_Prs.cu:9,33    3N -> 2Nd */
     FAME_OP_QBQ = 4,   /* FAME_Main/FAME_Matrix_Vector_Production_Isotropic_QBQ.cu:8,38  2Nd -> 2Nd */
-    FAME_OP_INVAR = 5  /* FAME_Main/FAME_Matrix_Vector_Production_Isotropic_invAr.cu:8   2Nd -> 2Nd */
+    FAME_OP_INVAR = 5, /* FAME_Main/FAME_Matrix_Vector_Production_Isotropic_invAr.cu:8   2Nd -> 2Nd */
+    FAME_OP_INVAR_MIXED = 6, /* invAr with FP32 CG inside FP64 iterative refinement (no reference counterpart) */
+    FAME_OP_QBQ32 = 7  /* QBQ evaluated in complex FP32 (inner operator of the mixed solver), FP64 in/out */
 };
 
 typedef struct fame_ctx fame_ctx;
