@@ -189,6 +189,47 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
     return st;
 }
 
+/* Average duration (microseconds, hipEvent timing on the context stream) of each
+ * of the five QBQ passes and of a whole QBQ, for all nk problems of the context
+ * in one launch, in precision prec (8 or 4).  Used by the benchmark to price the
+ * dominant kernel against the HBM roofline.  us6 = {P1, P2, P3, P4, P5, QBQ}.  */
+extern "C" fame_status fame_time_qbq_passes(fame_ctx* c, int prec, int reps, double* us6) {
+    if (!c || !us6 || reps < 1 || (prec != 8 && prec != 4)) FAIL(FAME_ERR_ARG, "fame_time_qbq_passes: bad arguments");
+    if (!c->have_eps || c->nk < 1) FAIL(FAME_ERR_STATE, "fame_time_qbq_passes: permittivity and wave vectors must be set");
+    HIPCHK(hipSetDevice(c->device));
+    const int nk = c->nk;
+    void* x = prec == 8 ? c->cg64[1] : c->cg32[1];
+    void* y = prec == 8 ? c->cg64[2] : c->cg32[2];
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    KInner ki = c->kin(prec, 0, nullptr);
+    KAxis km = c->kmid(prec, 0, nullptr);
+    KAxis ko = c->kout(prec, 0, nullptr);
+    ki.x = x; ki.y = y; ki.pikind = 0;
+    for (int which = 0; which < 6; ++which) {
+        for (int rep = -1; rep < reps; ++rep) {
+            if (rep == 0) HIPCHK(hipEventRecord(e0, c->stream));
+            switch (which) {
+                case 0: HIPCHK(fame_launch_inner(ki, 1, prec, nk, c->stream)); break;
+                case 1: HIPCHK(fame_launch_mid(km, +1, prec, nk, c->stream)); break;
+                case 2: HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nk, c->stream)); break;
+                case 3: HIPCHK(fame_launch_mid(km, -1, prec, nk, c->stream)); break;
+                case 4: HIPCHK(fame_launch_inner(ki, 0, prec, nk, c->stream)); break;
+                default: CHK(c->qbq(prec, x, y, 0, nk, nullptr, nullptr)); break;
+            }
+        }
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        us6[which] = 1000.0 * ms / reps;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return FAME_OK;
+}
+
 extern "C" fame_status fame_eig_grad(fame_ctx* c, int nwant, const double* freq, const void* E_dev, const double* weights,
                                      void* grad_dev) {
     if (!c || !freq || !E_dev || !weights || !grad_dev || nwant < 1) FAIL(FAME_ERR_ARG, "fame_eig_grad: bad arguments");

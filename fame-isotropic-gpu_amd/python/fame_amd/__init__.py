@@ -106,6 +106,8 @@ def lib():
     L.fame_eigs.argtypes = [vp, i, C.POINTER(fame_es), C.POINTER(fame_ls), vp, vp, vp, vp]
     L.fame_residuals.argtypes = [vp, i, vp, vp, vp]
     L.fame_eig_grad.argtypes = [vp, i, vp, vp, vp, vp]
+    L.fame_time_qbq_passes.argtypes = [vp, i, i, vp]
+    L.fame_time_qbq_passes.restype = C.c_int
     L.fame_set_stream.argtypes = [vp, vp]
     L.fame_query.argtypes = [vp, vp, vp, vp]
     L.fame_last_error.restype = C.c_char_p
@@ -345,6 +347,12 @@ class Solver:
         f = np.ascontiguousarray(freq, np.float64)
         _check(lib().fame_residuals(self._ctx, nwant, _ptr(f), C.c_void_p(E.data_ptr()), _ptr(res)), "fame_residuals")
         return res
+
+    def time_qbq_passes(self, prec_bytes: int = 8, reps: int = 20):
+        """hipEvent-timed average microseconds of the 5 QBQ passes and one QBQ (all wave vectors)."""
+        us = np.zeros(6)
+        _check(lib().fame_time_qbq_passes(self._ctx, prec_bytes, reps, _ptr(us)), "fame_time_qbq_passes")
+        return us
 
     def eig_vjp(self, freq, E, weights):
         """sum_{k,j} w_kj d lambda_kj / d eps  (3N, reference layout) by Hellmann-Feynman."""

@@ -134,6 +134,13 @@ fame_status fame_residuals(fame_ctx* ctx, int nwant, const double* freq, const v
 fame_status fame_eig_grad(fame_ctx* ctx, int nwant, const double* freq, const void* E_dev, const double* weights,
                           void* grad_dev);
 
+/* Average durations in microseconds (hipEvent timing on the context stream) of
+ * the five QBQ passes and of a whole QBQ over all wave vectors of the context,
+ * us6 = {inner expand, middle inverse, outer eps, middle forward, inner contract,
+ * whole QBQ}; prec = 8 (FP64) or 4 (FP32).  Measurement helper, no reference
+ * counterpart.                                                                 */
+fame_status fame_time_qbq_passes(fame_ctx* ctx, int prec, int reps, double* us6);
+
 /* Stream used by all launches (a hipStream_t passed as void*; NULL = default). */
 fame_status fame_set_stream(fame_ctx* ctx, void* stream);
 
