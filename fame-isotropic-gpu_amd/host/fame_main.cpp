@@ -1,0 +1,164 @@
+/*
+ * fame_main.cpp -- the reference's FAME_Main_Code / Lanczos_Isotropic call
+ * surface on top of the C-ABI (include/fame_hip.h).
+ *
+ * FAME_Main_Code follows FAME_Main/FAME_Main_Code.cu:37-140: permittivity from
+ * the Yee-edge material mask (FAME_Matrix_B_Isotropic.cu:4-74), the wave-vector
+ * loop -- here batched so that many wave vectors share each launch --, the
+ * restored fields, the residual check with the reference's absolute 1e-10
+ * threshold and sign flag, the PROFILE counters and Data_Omega_array.txt
+ * (FAME_Tools/FAME_Create_Frequency_txt.cpp:3-21).
+ */
+#include "fame_main.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+bool simple_type(const char* t) {
+    return !std::strcmp(t, "simple_cubic") || !std::strcmp(t, "primitive_orthorhombic") ||
+           !std::strcmp(t, "primitive_tetragonal");
+}
+
+fame_lattice lattice_of(const PAR& P) {
+    fame_lattice L;
+    std::memset(&L, 0, sizeof(L));
+    for (int i = 0; i < 3; ++i) { L.grid_nums[i] = P.mesh.grid_nums[i]; L.mesh_len[i] = P.mesh.mesh_len[i]; }
+    for (int i = 0; i < 9; ++i) L.lattice_vec_a[i] = P.lattice.lattice_vec_a[i];
+    L.general = simple_type(P.lattice.lattice_type) ? 0 : 1;
+    const LATTICE_CONSTANT& c = P.lattice.lattice_constant;
+    L.m1 = c.m1; L.m2 = c.m2; L.m3 = c.m3; L.m4 = c.m4;
+    for (int i = 0; i < 3; ++i) {
+        L.t1[i] = c.t1 ? c.t1[i] : 0.0; L.t2[i] = c.t2 ? c.t2[i] : 0.0;
+        L.t3[i] = c.t3 ? c.t3[i] : 0.0; L.t4[i] = c.t4 ? c.t4[i] : 0.0;
+    }
+    return L;
+}
+
+}  // namespace
+
+extern "C" int Lanczos_Isotropic(double* Freq_array, void* ev_dev, fame_ctx* ctx, ES es, LS ls, int Nx, int Ny,
+                                 int Nz, const char* flag_CompType, PROFILE* Profile) {
+    int N = 0, nk = 0, len3[3];
+    fame_status st = fame_query(ctx, &N, &nk, len3);
+    if (st != FAME_OK) return -(int)st;
+    if (N != Nx * Ny * Nz || !flag_CompType) return -(int)FAME_ERR_ARG;
+    fame_es e = {es.nwant, es.nstep, es.maxit, es.tol};
+    fame_ls l = {ls.maxit, ls.tol};
+    std::vector<fame_profile> prof(nk);
+    const char* mode = std::getenv("FAME_PRECISION");
+    int prec = (mode && !std::strcmp(mode, "mixed")) ? FAME_PREC_F32_REFINE : FAME_PREC_F64;
+    st = fame_eigs(ctx, prec, &e, &l, Freq_array, ev_dev, nullptr, prof.data());
+    if (st != FAME_OK) return -(int)st;
+    int mx = 0;
+    for (int i = 0; i < nk; ++i) {
+        mx = std::max(mx, prof[i].restarts);
+        if (Profile && Profile->ls_iter) {
+            Profile->ls_iter[Profile->idx + i] += prof[i].ls_iter;
+            Profile->es_iter[Profile->idx + i] += prof[i].es_iter;
+        }
+    }
+    return mx;
+}
+
+extern "C" int FAME_Main_Code(PAR Par, PROFILE* Profile) {
+    const int Nx = Par.mesh.grid_nums[0], Ny = Par.mesh.grid_nums[1], Nz = Par.mesh.grid_nums[2];
+    const int N = Nx * Ny * Nz;
+    const size_t N3 = 3 * (size_t)N;
+    const int Nwant = Par.es.nwant, Nstep = Par.es.nstep, Nk = Par.recip_lattice.Wave_vec_num;
+    if (N <= 0 || Nwant <= 0 || Nk <= 0) return FAME_ERR_ARG;
+    if (std::strcmp(Par.material.material_type, "isotropic") != 0) return FAME_ERR_UNSUPPORTED;
+
+    /* B_eps (FAME_Matrix_B_Isotropic.cu:31-62); too few eps values is an error here
+     * (the reference falls back to an interactive prompt, :13-24) */
+    const int nm = Par.material.material_num;
+    if (nm > Par.material.num_ele_permitt_in) {
+        std::fprintf(stderr, "FAME_Main_Code: %d materials but %d permittivity values\n", nm,
+                     Par.material.num_ele_permitt_in);
+        return FAME_ERR_ARG;
+    }
+    std::vector<double> eps(N3);
+    const size_t temp = (size_t)N * nm;
+    for (int i = 0; i < N; ++i) {
+        for (int c = 0; c < 3; ++c) eps[c * (size_t)N + i] = Par.material.ele_permitt_out;
+        for (int j = 0; j < nm; ++j)
+            for (int c = 0; c < 3; ++c)
+                if (Par.material.Binout && Par.material.Binout[i + (size_t)j * N + c * temp] == 1)
+                    eps[c * (size_t)N + i] = Par.material.ele_permitt_in[j];
+    }
+
+    fame_lattice L = lattice_of(Par);
+    /* batch size from the free device memory: Lanczos basis, fields and CG work */
+    if (hipSetDevice(Par.flag.device) != hipSuccess) return FAME_ERR_HIP;
+    size_t freeb = 0, totb = 0;
+    if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return FAME_ERR_HIP;
+    const double per_k = 16.0 * 2 * N * (Nstep + Nwant + 16) + 24.0 * N3 * (Nwant + 2);
+    int batch = (int)std::max(1.0, std::min((double)Nk, 0.7 * (double)freeb / per_k));
+
+    fame_ctx* ctx = nullptr;
+    fame_status st = fame_create(&L, batch, Par.flag.device, &ctx);
+    if (st != FAME_OK) { std::fprintf(stderr, "FAME_Main_Code: %s\n", fame_last_error()); return st; }
+    st = fame_set_eps(ctx, eps.data());
+    if (st != FAME_OK) { fame_destroy(ctx); return st; }
+
+    std::vector<double> freq_all((size_t)Nk * Nwant);
+    void* E_dev = nullptr;
+    if (hipMalloc(&E_dev, 16 * N3 * Nwant * batch) != hipSuccess) { fame_destroy(ctx); return FAME_ERR_NOMEM; }
+    std::vector<double> Eh(2 * N3);
+    const char* mode = std::getenv("FAME_PRECISION");
+    const int prec = (mode && !std::strcmp(mode, "mixed")) ? FAME_PREC_F32_REFINE : FAME_PREC_F64;
+    fame_es es = {Par.es.nwant, Par.es.nstep, Par.es.maxit, Par.es.tol};
+    fame_ls ls = {Par.ls.maxit, Par.ls.tol};
+    for (int k0 = 0; k0 < Nk && st == FAME_OK; k0 += batch) {
+        const int nb = std::min(batch, Nk - k0);
+        if ((st = fame_set_wave_vectors(ctx, nb, Par.recip_lattice.WaveVector + 3 * (size_t)k0, nullptr)) != FAME_OK) break;
+        std::vector<fame_profile> prof(nb);
+        auto t0 = std::chrono::steady_clock::now();
+        double* freq = &freq_all[(size_t)k0 * Nwant];
+        if ((st = fame_eigs(ctx, prec, &es, &ls, freq, nullptr, E_dev, prof.data())) != FAME_OK) break;
+        const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::vector<double> rel((size_t)nb * Nwant);
+        if ((st = fame_residuals(ctx, Nwant, freq, E_dev, rel.data())) != FAME_OK) break;
+        for (int p = 0; p < nb; ++p) {
+            for (int i = 0; i < Nwant; ++i) {
+                /* absolute residual of the reference-scaled field: relative residual x ||E|| */
+                if (hipMemcpy(Eh.data(), (double*)E_dev + 2 * N3 * ((size_t)p * Nwant + i), 16 * N3,
+                              hipMemcpyDeviceToHost) != hipSuccess) { st = FAME_ERR_HIP; break; }
+                double nrm = 0;
+                for (size_t q = 0; q < 2 * N3; ++q) nrm += Eh[q] * Eh[q];
+                const double res = rel[(size_t)p * Nwant + i] * std::sqrt(nrm);
+                std::printf("                 Freq(%2d) = %10.8f, residual = %e.\n", i, freq[(size_t)p * Nwant + i], res);
+                if (res > 1e-10) freq[(size_t)p * Nwant + i] = -freq[(size_t)p * Nwant + i];
+            }
+            if (Profile && Profile->ls_iter) {
+                Profile->idx = k0 + p;
+                Profile->ls_iter[k0 + p] = prof[p].ls_iter;
+                Profile->es_iter[k0 + p] = prof[p].es_iter;
+                Profile->es_time[k0 + p] = secs / nb;
+                Profile->ls_time[k0 + p] = secs / nb;
+            }
+        }
+    }
+    (void)hipFree(E_dev);
+    fame_destroy(ctx);
+    if (st != FAME_OK) { std::fprintf(stderr, "FAME_Main_Code: %s\n", fame_last_error()); return st; }
+    FILE* fp = std::fopen("Data_Omega_array.txt", "w");
+    if (fp) {
+        std::fprintf(fp, "Freq_array = []\n");
+        for (int i = 0; i < Nk; i++) {
+            std::fprintf(fp, "Freq_array(:, %d) = [\n", i + 1);
+            for (int j = 0; j < Nwant; j++) std::fprintf(fp, "% 10.12lf\n", freq_all[j + (size_t)i * Nwant]);
+            std::fprintf(fp, "];\n");
+        }
+        std::fclose(fp);
+    }
+    return 0;
+}

@@ -243,18 +243,25 @@ def bz_path(lo: fame_lattice_out, path: Optional[str] = None, part_num: int = 10
 
 
 def path_by_arclength(lo: fame_lattice_out, vertices: str, npts: int) -> np.ndarray:
-    """npts wave vectors spaced evenly in arc length along the vertex path (ends
-    included), in the computational frame.  Used for explicit k lists whose length
-    the reference's part_num rule (4(p-1)+1 points for 5 vertices) cannot give."""
+    """npts wave vectors on the piecewise-linear vertex path, every vertex included
+    and the remaining points shared among the segments in proportion to their
+    length (largest remainder), evenly spaced inside each segment; computational
+    frame.  Used for explicit k lists whose length the reference's part_num rule
+    (4(p-1)+1 points for 5 vertices) cannot give."""
     V = np.array([bz_vertex(lo, ch) for ch in vertices])
+    nseg = len(V) - 1
+    if npts < nseg + 1:
+        raise FameError("need at least one point per vertex")
     seg = np.linalg.norm(np.diff(V, axis=0), axis=1)
-    s = np.concatenate([[0.0], np.cumsum(seg)])
-    t = np.linspace(0.0, s[-1], npts)
-    pts = []
-    for tt in t:
-        j = min(np.searchsorted(s, tt, side="right") - 1, len(seg) - 1)
-        w = (tt - s[j]) / seg[j] if seg[j] > 0 else 0.0
-        pts.append(V[j] + w * (V[j + 1] - V[j]))
+    inner = npts - (nseg + 1)
+    share = inner * seg / seg.sum()
+    cnt = np.floor(share).astype(int)
+    for j in np.argsort(-(share - cnt))[: inner - cnt.sum()]:
+        cnt[j] += 1
+    pts = [V[0]]
+    for j in range(nseg):
+        for q in range(1, cnt[j] + 2):
+            pts.append(V[j] + (q / (cnt[j] + 1)) * (V[j + 1] - V[j]))
     return np.array([rotate_wave_vector(lo, p) for p in pts])
 
 
