@@ -23,25 +23,48 @@ template <> struct V2<float> { typedef float2 t; };
 __device__ __forceinline__ double2 d2(double2 v) { return v; }
 __device__ __forceinline__ double2 d2(float2 v) { return make_double2(v.x, v.y); }
 
+/* sum of `NC` doubles over the block (blockDim.x a multiple of 64, at most RB):
+ * wave-level butterfly shuffles, then the per-wave sums in a fixed order, so
+ * every thread receives the same, run-to-run deterministic total            */
+template <int NC>
+__device__ __forceinline__ void block_sum_all(double (&v)[NC]) {
+    __shared__ double wred[NC][RB / 64];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) wred[c][w] = v[c];
+    }
+    __syncthreads();
+    const int nw = blockDim.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        double s = 0.0;
+        for (int i = 0; i < nw; ++i) s += wred[c][i];
+        v[c] = s;
+    }
+}
+
 /* block sum of `ncomp` doubles per thread -> partials[p][blockIdx.x][c] */
 template <int NC>
 __device__ __forceinline__ void block_partials(double (&v)[NC], double* partials, int p) {
-    __shared__ double red[NC][RB];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) red[c][threadIdx.x] = v[c];
-    __syncthreads();
-    for (int s = RB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-#pragma unroll
-            for (int c = 0; c < NC; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + s];
-        }
-        __syncthreads();
-    }
+    block_sum_all<NC>(v);
     if (threadIdx.x == 0) {
         double* o = partials + ((size_t)p * gridDim.x + blockIdx.x) * NC;
 #pragma unroll
-        for (int c = 0; c < NC; ++c) o[c] = red[c][0];
+        for (int c = 0; c < NC; ++c) o[c] = v[c];
     }
+}
+
+/* sum over the nblk partials of problem p, component c of ncomp, by the whole block */
+__device__ __forceinline__ double sum_partials(const double* partials, int p, int nblk, int ncomp, int c) {
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) v[0] += partials[((size_t)p * nblk + b) * ncomp + c];
+    block_sum_all<1>(v);
+    return v[0];
 }
 
 /* ---------------------------------------------------------------- CG ---- */
@@ -68,9 +91,8 @@ __global__ void __launch_bounds__(RB) k_cg_begin(typename V2<T>::t* x, typename 
 __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int* active,
                                const int* mask) {
     const int p = blockIdx.x;
+    const double s = sum_partials(partials, p, nblk, 1, 0);
     if (threadIdx.x != 0) return;
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
     CgScal c;
     c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
     c.tol2 = tol * tol; c.iters = 1; c.maxit = maxit;
@@ -82,9 +104,15 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
 
 __global__ void k_cg_alpha(CgScal* sc, const double* partials, int nblk, const int* active) {
     const int p = blockIdx.x;
-    if (threadIdx.x != 0 || !active[p]) return;
-    double re = 0, im = 0;
-    for (int b = 0; b < nblk; ++b) { re += partials[((size_t)p * nblk + b) * 2]; im += partials[((size_t)p * nblk + b) * 2 + 1]; }
+    if (!active[p]) return;
+    double v[2] = {0.0, 0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+        v[0] += partials[((size_t)p * nblk + b) * 2];
+        v[1] += partials[((size_t)p * nblk + b) * 2 + 1];
+    }
+    block_sum_all<2>(v);
+    if (threadIdx.x != 0) return;
+    const double re = v[0], im = v[1];
     CgScal& c = sc[p];
     c.pap_re = re; c.pap_im = im;
     /* a = r1 / dot  (CG.cu:60-63) */
@@ -118,9 +146,9 @@ __global__ void __launch_bounds__(RB) k_cg_update_xr(typename V2<T>::t* x, typen
 
 __global__ void k_cg_beta(CgScal* sc, const double* partials, int nblk, int* active) {
     const int p = blockIdx.x;
-    if (threadIdx.x != 0 || !active[p]) return;
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
+    if (!active[p]) return;
+    const double s = sum_partials(partials, p, nblk, 1, 0);
+    if (threadIdx.x != 0) return;
     CgScal& c = sc[p];
     c.beta = s / c.rr;
     c.rr = s;
@@ -201,43 +229,48 @@ __global__ void __launch_bounds__(RB) k_norm2(const typename V2<T>::t* x, double
     block_partials<1>(acc, partials, p);
 }
 
-__global__ void k_reduce_partials(const double* partials, int nblk, int ncomp, double* out) {
-    const int p = blockIdx.x;
-    for (int c = threadIdx.x; c < ncomp; c += blockDim.x) {
-        double s = 0;
-        for (int b = 0; b < nblk; ++b) s += partials[((size_t)p * nblk + b) * ncomp + c];
-        out[(size_t)p * ncomp + c] = s;
-    }
+/* grid (nprob, ncomp): one block per problem and component */
+__global__ void __launch_bounds__(RB) k_reduce_partials(const double* partials, int nblk, int ncomp, double* out) {
+    const int p = blockIdx.x, c = blockIdx.y;
+    const double s = sum_partials(partials, p, nblk, ncomp, c);
+    if (threadIdx.x == 0) out[(size_t)p * ncomp + c] = s;
 }
 
 /* ------------------------------------------------------------ Lanczos ---- */
-/* h[j] = sum_i conj(V[j][i]) w[i], j < m  (partials of 2m doubles per block) */
+/* h[j] = sum_i conj(V[j][i]) w[i], j < m  (partials of 2m doubles per block).
+ * Columns go four at a time so that w is read once per four columns.        */
 __global__ void __launch_bounds__(RB) k_gemv_h(const double2* V, const double2* w, double* partials, int m, size_t n,
                                                size_t cstride, size_t pstride, size_t wstride, const int* mask) {
     const int p = blockIdx.y;
-    __shared__ double red[RB];
-    const bool on = !mask || mask[p];
+    double* out = partials + ((size_t)p * gridDim.x + blockIdx.x) * 2 * m;
+    if (mask && !mask[p]) {
+        for (int t = threadIdx.x; t < 2 * m; t += blockDim.x) out[t] = 0.0;
+        return;
+    }
     const double2* Vp = V + (size_t)p * pstride;
     const double2* wp = w + (size_t)p * wstride;
-    for (int j = 0; j < m; ++j) {
-        double re = 0, im = 0;
-        if (on) {
-            for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-                double2 v = Vp[(size_t)j * cstride + i], x = wp[i];
-                re += v.x * x.x + v.y * x.y;
-                im += v.x * x.y - v.y * x.x;
+    for (int j0 = 0; j0 < m; j0 += 4) {
+        const int nj = min(4, m - j0);
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+            const double2 x = wp[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q < nj) {
+                    const double2 v = Vp[(size_t)(j0 + q) * cstride + i];
+                    acc[2 * q] += v.x * x.x + v.y * x.y;
+                    acc[2 * q + 1] += v.x * x.y - v.y * x.x;
+                }
             }
         }
-        for (int c = 0; c < 2; ++c) {
-            red[threadIdx.x] = c == 0 ? re : im;
-            __syncthreads();
-            for (int s = RB / 2; s > 0; s >>= 1) {
-                if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-                __syncthreads();
-            }
-            if (threadIdx.x == 0) partials[((size_t)p * gridDim.x + blockIdx.x) * 2 * m + 2 * j + c] = red[0];
-            __syncthreads();
+        block_sum_all<8>(acc);
+        if ((int)threadIdx.x < 2 * nj) {
+            double r = acc[0];
+#pragma unroll
+            for (int t = 1; t < 8; ++t) if ((int)threadIdx.x == t) r = acc[t];
+            out[2 * j0 + threadIdx.x] = r;
         }
+        __syncthreads();
     }
 }
 
@@ -368,11 +401,11 @@ hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* parti
 }
 hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob, int* active,
                              hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
+    hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
     return hipGetLastError();
 }
 hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob, const int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, active);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, const CgScal* sc, double* partials, int nblk,
@@ -386,7 +419,7 @@ hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, co
     return hipGetLastError();
 }
 hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, active);
+    hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
@@ -451,7 +484,7 @@ hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nb
     return hipGetLastError();
 }
 hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, double* out, int nprob, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(nprob), dim3(256), 0, s, partials, nblk, ncomp, out);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(nprob, ncomp), dim3(RB), 0, s, partials, nblk, ncomp, out);
     return hipGetLastError();
 }
 hipError_t fame_gemv_h(const void* V, const void* w, double* partials, int nblk, int m, size_t n, size_t cstride,
@@ -507,9 +540,8 @@ __global__ void k_lz_beta(const double* nrm2, double* T1d, int ld, int j, double
 __global__ void k_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
                                 int* active, const int* mask) {
     const int p = blockIdx.x;
+    const double s = sum_partials(partials, p, nblk, 1, 0);
     if (threadIdx.x != 0) return;
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partials[(size_t)p * nblk + b];
     CgScal c;
     c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
     c.tol2 = fmax(tol * tol, reltol * reltol * s); c.iters = 1; c.maxit = maxit;
@@ -531,6 +563,6 @@ hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* 
 }
 hipError_t fame_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
                               int nprob, int* active, const int* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_init_scal2, dim3(nprob), dim3(64), 0, s, sc, partials, nblk, tol, reltol, maxit, active, mask);
+    hipLaunchKernelGGL(k_cg_init_scal2, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, tol, reltol, maxit, active, mask);
     return hipGetLastError();
 }

@@ -1,8 +1,8 @@
 /*
  * fame_device.hpp -- device-side building blocks for gfx950:
- * complex arithmetic, the batched-line FFT engine (LDS Stockham autosort with
- * radices 2,3,4,5,7,11,13 and Bluestein chirp-z for lengths with larger prime
- * factors) and the on-the-fly Fourier symbols / Pi bases of the null-space-free
+ * complex arithmetic, the batched-line FFT engine (in-place LDS Stockham
+ * autosort with radices 2,3,4,5,7,8,11,13 and Bluestein chirp-z for lengths
+ * with larger prime factors) and the on-the-fly Fourier symbols / Pi bases of the null-space-free
  * Maxwell operator.
  *
  * Axes.  Every problem is stored with three axes (a = inner, contiguous;
@@ -41,193 +41,281 @@ __device__ __forceinline__ double2 lam_of(double x, double inv_h) {
     return make_double2(-2.0 * s * s * inv_h, 2.0 * s * c * inv_h);
 }
 
-/* exact quotient n / d for n*d < 2^32 from m = ceil(2^32 / d) */
-__device__ __forceinline__ int fdiv(int n, unsigned long long m) { return (int)(((unsigned long long)(unsigned)n * m) >> 32); }
+
+/* quotient n / d from m = ceil(2^32 / d) (m = 0 encodes d = 1); exact for n*d < 2^32 */
+__device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulhi((unsigned)n, m) : n; }
 
 /* ------------------------------------------------------------------------- */
-/*  FFT plan as seen by a kernel                                              */
+/*  Line transforms                                                           */
 /* ------------------------------------------------------------------------- */
-#define FAME_MAX_PASSES 10
-struct PassDev {
-    int R, Ns, nb, tstep;           /* radix, span so far, butterflies per line, twiddle step */
-    unsigned long long mNs, mNb;    /* magic multipliers for / Ns and / nb              */
-};
-struct FftPlanDev {
-    int n;          /* transform length                                        */
-    int M;          /* Stockham length: n (direct) or the Bluestein length     */
-    int blue;       /* 1 when Bluestein chirp-z is used                        */
-    int nrad;
-    PassDev pass[FAME_MAX_PASSES];
-    unsigned long long mM, mN;      /* magic for / M and / n                   */
-    const void* W;      /* M entries exp(-2 pi i m / M), precision of the kernel */
-    const void* chirp;  /* n entries exp(-i pi m^2 / n)                          */
-    const void* bhat_m; /* M entries: FFT_M of the kernel for the forward sign   */
-    const void* bhat_p; /* M entries: same for the inverse sign                  */
+/* A kernel transforms `nl` lines of length M held in one LDS buffer (line pitch
+ * P, odd, so that column accesses of consecutive lines hit different banks).
+ * The work is a list of stages run by the whole block:
+ *   R > 0 : one in-place Stockham autosort pass of radix R.  Every thread loads
+ *           its butterflies' inputs into registers (applying the optional input
+ *           multiplier and the twiddles), computes them, and after a barrier
+ *           writes the outputs back to the same buffer.
+ *   R = 0 : an elementwise step supplied by the kernel (1/eps, curl-curl, ...).
+ * Bluestein chirp-z (lengths with a prime factor above 13) appears as two
+ * length-M transforms in the list: the pre-chirp is the input multiplier of the
+ * first pass, the kernel spectrum bhat that of the first inverse pass, and the
+ * post-chirp (with 1/M) is applied by the kernel's store ("post").          */
+#define FAME_MAX_STAGES 24
+#define FFT_THREADS 256
+
+enum { PRE_NONE = 0, PRE_CHIRP = 1, PRE_CHIRP_CONJ = 2, PRE_BM = 3, PRE_BP = 4 };
+enum { POST_NONE = 0, POST_CHIRP = 1, POST_CHIRP_CONJ = 2 };
+
+struct StageDev {
+    int R;              /* radix; 0 = kernel-supplied elementwise step          */
+    int Ns;             /* product of the radices of the earlier passes          */
+    int nb;             /* butterflies per line, M / R                          */
+    int tstep;          /* twiddle table step, M / (Ns R)                       */
+    unsigned mNs, mNb;  /* udiv magics for / Ns and / nb                        */
+    int sign;           /* -1 forward (exp(-i..)), +1 inverse                  */
+    int pre;            /* PRE_*: multiplier of the pass inputs (first passes) */
 };
 
-/* LDS-resident tables of a plan: W (M), and for Bluestein chirp (n), bhat_m, bhat_p (M each) */
-template <typename T2>
-struct PlanLds {
-    T2* W; T2* chirp; T2* bm; T2* bp;
+struct LinePlanDev {
+    int n;              /* transform length                                    */
+    int M;              /* Stockham length (n, or the Bluestein length)        */
+    int blue;           /* 1 for Bluestein                                     */
+    int nst;            /* stages in st[]                                      */
+    int post;           /* POST_*: multiplier the store applies                */
+    int ntab;           /* LDS table entries: M (W) or 4M (W, chirp, bm, bp)   */
+    unsigned mM, mN;    /* udiv magics for / M and / n                         */
+    const void* W;      /* M entries exp(-2 pi i m / M) in the kernel precision */
+    const void* chirp;  /* M entries exp(-i pi m^2 / n) for m < n, 0 beyond     */
+    const void* bm;     /* M entries: DFT_M of the chirp kernel, forward sign   */
+    const void* bp;     /* M entries: same for the inverse sign                 */
+    StageDev st[FAME_MAX_STAGES];
 };
-__host__ __device__ inline int plan_table_len(const FftPlanDev& p) { return p.blue ? (3 * p.M + p.n) : p.M; }
 
 template <typename T2>
-__device__ __forceinline__ PlanLds<T2> load_plan_tables(const FftPlanDev& pl, T2* base) {
-    PlanLds<T2> t;
-    t.W = base;
-    const T2* W = reinterpret_cast<const T2*>(pl.W);
-    for (int i = threadIdx.x; i < pl.M; i += blockDim.x) t.W[i] = W[i];
-    if (pl.blue) {
-        t.chirp = base + pl.M;
-        t.bm = t.chirp + pl.n;
-        t.bp = t.bm + pl.M;
-        const T2* c = reinterpret_cast<const T2*>(pl.chirp);
-        const T2* bm = reinterpret_cast<const T2*>(pl.bhat_m);
-        const T2* bp = reinterpret_cast<const T2*>(pl.bhat_p);
-        for (int i = threadIdx.x; i < pl.n; i += blockDim.x) t.chirp[i] = c[i];
-        for (int i = threadIdx.x; i < pl.M; i += blockDim.x) { t.bm[i] = bm[i]; t.bp[i] = bp[i]; }
-    } else {
-        t.chirp = t.bm = t.bp = nullptr;
+struct Tabs { const T2* W; const T2* chirp; const T2* bm; const T2* bp; };
+
+/* copy a plan's tables into LDS at `base` (caller synchronises) */
+template <typename T2>
+__device__ __forceinline__ Tabs<T2> load_tabs(const LinePlanDev& pl, T2* base) {
+    const int M = pl.M;
+    const T2* src[4] = {reinterpret_cast<const T2*>(pl.W), reinterpret_cast<const T2*>(pl.chirp),
+                        reinterpret_cast<const T2*>(pl.bm), reinterpret_cast<const T2*>(pl.bp)};
+    for (int i = threadIdx.x; i < pl.ntab; i += blockDim.x) {
+        const int t = i >= M ? (i >= 2 * M ? (i >= 3 * M ? 3 : 2) : 1) : 0;
+        base[i] = src[t][i - t * M];
     }
-    return t;
+    Tabs<T2> tb;
+    tb.W = base;
+    tb.chirp = base + M; tb.bm = base + 2 * M; tb.bp = base + 3 * M;
+    return tb;
 }
 
-/* small DFT of radix R in registers: o[q] = sum_t v[t] wr[(q t) mod R] */
+/* store-side multiplier of output j of a line */
+template <typename T2, typename T>
+__device__ __forceinline__ T2 post_mul(const LinePlanDev& pl, const Tabs<T2>& tb, int j, T2 v) {
+    if (pl.post == POST_NONE) return v;
+    T2 w = tb.chirp[j];
+    if (pl.post == POST_CHIRP_CONJ) w.y = -w.y;
+    const T invM = (T)1 / (T)pl.M;
+    return cscale(cmul(v, w), invM);
+}
+
+/* cos / sin (2 pi k / R), k = 1..(R-1)/2, for the symmetric odd-radix DFTs */
+template <int R> struct OddTrig;
+template <> struct OddTrig<3> {
+    static constexpr double c[1] = {-0.5};
+    static constexpr double s[1] = {0.8660254037844386};
+};
+template <> struct OddTrig<5> {
+    static constexpr double c[2] = {0.30901699437494745, -0.8090169943749475};
+    static constexpr double s[2] = {0.9510565162951535, 0.5877852522924731};
+};
+template <> struct OddTrig<7> {
+    static constexpr double c[3] = {0.6234898018587335, -0.2225209339563144, -0.9009688679024191};
+    static constexpr double s[3] = {0.7818314824680298, 0.9749279121818236, 0.4338837391175581};
+};
+template <> struct OddTrig<11> {
+    static constexpr double c[5] = {0.8412535328311812, 0.41541501300188644, -0.14231483827328514,
+                                    -0.6548607339452851, -0.9594929736144974};
+    static constexpr double s[5] = {0.5406408174555976, 0.9096319953545183, 0.9898214418809327,
+                                    0.7557495743542583, 0.28173255684142967};
+};
+template <> struct OddTrig<13> {
+    static constexpr double c[6] = {0.8854560256532099, 0.5680647467311558, 0.12053668025532305,
+                                    -0.3546048870425356, -0.7485107481711011, -0.970941817426052};
+    static constexpr double s[6] = {0.46472317204376856, 0.8229838658936564, 0.992708874098054,
+                                    0.9350162426854148, 0.6631226582407952, 0.23931566428755777};
+};
+
+/* in-place DFT of odd length R = 2m+1 with the pair sums / differences
+ * s_t = v_t + v_{R-t}, d_t = v_t - v_{R-t}:
+ *   X_q = v_0 + sum_t cos(2 pi q t/R) s_t  -/+  i sum_t sin(2 pi q t/R) d_t
+ * (the sign of the i-term is the transform sign), X_{R-q} with the opposite.  */
 template <typename T2, int R>
-__device__ __forceinline__ void small_dft(const T2 (&v)[R], T2 (&o)[R], const T2 (&wr)[R], int sign) {
-    if (R == 2) {
-        o[0] = cadd(v[0], v[1]); o[1] = csub(v[0], v[1]);
-    } else if (R == 4) {
-        T2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]), a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
-        T2 j3;
-        if (sign < 0) { j3.x = a3.y; j3.y = -a3.x; } else { j3.x = -a3.y; j3.y = a3.x; }
-        o[0] = cadd(a0, a2); o[2] = csub(a0, a2); o[1] = cadd(a1, j3); o[3] = csub(a1, j3);
-    } else if (R == 3) {
-        typedef decltype(v[0].x) T;
-        T2 s = cadd(v[1], v[2]), d = csub(v[1], v[2]);
-        T2 m; m.x = v[0].x - (T)0.5 * s.x; m.y = v[0].y - (T)0.5 * s.y;
-        /* i*sign*(sqrt(3)/2)*d */
-        const T h = sign < 0 ? (T)-0.86602540378443864676 : (T)0.86602540378443864676;
-        T2 r; r.x = -h * d.y; r.y = h * d.x;
-        o[0] = cadd(v[0], s); o[1] = cadd(m, r); o[2] = csub(m, r);
-    } else {
+__device__ __forceinline__ void dft_odd(T2 (&v)[R], bool inv) {
+    typedef decltype(v[0].x) T;
+    constexpr int m = (R - 1) / 2;
+    T2 sp[m], sm[m];
+    const T2 v0 = v[0];
+    T2 o0 = v0;
 #pragma unroll
-        for (int q = 0; q < R; ++q) {
-            T2 acc = v[0];
+    for (int t = 1; t <= m; ++t) {
+        sp[t - 1] = cadd(v[t], v[R - t]);
+        sm[t - 1] = csub(v[t], v[R - t]);
+        o0 = cadd(o0, sp[t - 1]);
+    }
 #pragma unroll
-            for (int t = 1; t < R; ++t) acc = cadd(acc, cmul(v[t], wr[(q * t) % R]));
-            o[q] = acc;
+    for (int q = 1; q <= m; ++q) {
+        T2 A = v0, B;
+        B.x = 0; B.y = 0;
+#pragma unroll
+        for (int t = 1; t <= m; ++t) {
+            const int r = (q * t) % R;
+            const int k = r <= m ? r : R - r;
+            const T c = (T)OddTrig<R>::c[k - 1];
+            const T sn = r <= m ? (T)OddTrig<R>::s[k - 1] : -(T)OddTrig<R>::s[k - 1];
+            A.x += c * sp[t - 1].x; A.y += c * sp[t - 1].y;
+            B.x += sn * sm[t - 1].x; B.y += sn * sm[t - 1].y;
         }
+        /* forward: X_q = A - iB, X_{R-q} = A + iB */
+        T2 lo, hi;
+        if (!inv) { lo.x = A.x + B.y; lo.y = A.y - B.x; hi.x = A.x - B.y; hi.y = A.y + B.x; }
+        else      { lo.x = A.x - B.y; lo.y = A.y + B.x; hi.x = A.x + B.y; hi.y = A.y - B.x; }
+        v[q] = lo;
+        v[R - q] = hi;
     }
+    v[0] = o0;
 }
 
-/* One Stockham radix-R pass over L lines held in LDS (src -> dst).
- * Butterfly j of a line: inputs src[j + t*nb], twiddle W_{Ns R}^{t k},
- * outputs dst[(j/Ns)*Ns*R + k + t*Ns], k = j mod Ns. sign < 0: forward. */
+template <typename T2>
+__device__ __forceinline__ void dft4(T2& v0, T2& v1, T2& v2, T2& v3, bool inv) {
+    const T2 a0 = cadd(v0, v2), a1 = csub(v0, v2), a2 = cadd(v1, v3), a3 = csub(v1, v3);
+    T2 j3;   /* -i a3 forward, +i a3 inverse */
+    if (!inv) { j3.x = a3.y; j3.y = -a3.x; } else { j3.x = -a3.y; j3.y = a3.x; }
+    v0 = cadd(a0, a2); v2 = csub(a0, a2); v1 = cadd(a1, j3); v3 = csub(a1, j3);
+}
+
 template <typename T2, int R>
-__device__ __forceinline__ void stockham_pass(const T2* __restrict__ src, T2* __restrict__ dst, int L, int pitch,
-                                              const PassDev& ps, const T2* __restrict__ W, int sign) {
-    const int nb = ps.nb, Ns = ps.Ns, tstep = ps.tstep;
-    T2 wr[R];
+__device__ __forceinline__ void dft(T2 (&v)[R], bool inv) {
+    typedef decltype(v[0].x) T;
+    if constexpr (R == 2) {
+        const T2 a = v[0];
+        v[0] = cadd(a, v[1]); v[1] = csub(a, v[1]);
+    } else if constexpr (R == 4) {
+        dft4(v[0], v[1], v[2], v[3], inv);
+    } else if constexpr (R == 8) {
+        /* split into even / odd outputs: a_k = v_k + v_{k+4} -> X_{2q},
+         * b_k = (v_k - v_{k+4}) W8^k -> X_{2q+1} */
+        const T h = (T)0.7071067811865476;
+        T2 a[4], b[4];
 #pragma unroll
-    for (int m = 0; m < R; ++m) {
-        T2 w = W[m * nb];
-        if (sign > 0) w = cconj(w);
-        wr[m] = w;
-    }
-    const int total = L * nb;
-    for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-        const int line = fdiv(idx, ps.mNb);
-        const int j = idx - line * nb;
-        const int qd = fdiv(j, ps.mNs);
-        const int k = j - qd * Ns;
-        const T2* a = src + line * pitch;
-        T2* b = dst + line * pitch;
-        T2 v[R];
-        v[0] = a[j];
-        if (k > 0) {
-#pragma unroll
-            for (int t = 1; t < R; ++t) {
-                T2 w = W[t * k * tstep];
-                if (sign > 0) w = cconj(w);
-                v[t] = cmul(a[j + t * nb], w);
-            }
+        for (int k = 0; k < 4; ++k) { a[k] = cadd(v[k], v[k + 4]); b[k] = csub(v[k], v[k + 4]); }
+        T2 t;
+        if (!inv) {
+            t = b[1]; b[1].x = h * (t.x + t.y); b[1].y = h * (t.y - t.x);
+            t = b[2]; b[2].x = t.y; b[2].y = -t.x;
+            t = b[3]; b[3].x = h * (t.y - t.x); b[3].y = -h * (t.x + t.y);
         } else {
-#pragma unroll
-            for (int t = 1; t < R; ++t) v[t] = a[j + t * nb];
+            t = b[1]; b[1].x = h * (t.x - t.y); b[1].y = h * (t.x + t.y);
+            t = b[2]; b[2].x = -t.y; b[2].y = t.x;
+            t = b[3]; b[3].x = -h * (t.x + t.y); b[3].y = h * (t.x - t.y);
         }
-        T2 o[R];
-        small_dft<T2, R>(v, o, wr, sign);
-        const int base = qd * Ns * R + k;
+        dft4(a[0], a[1], a[2], a[3], inv);
+        dft4(b[0], b[1], b[2], b[3], inv);
 #pragma unroll
-        for (int t = 0; t < R; ++t) b[base + t * Ns] = o[t];
+        for (int q = 0; q < 4; ++q) { v[2 * q] = a[q]; v[2 * q + 1] = b[q]; }
+    } else {
+        dft_odd<T2, R>(v, inv);
     }
 }
 
-/* Full Stockham transform of L lines of length pl.M held in LDS buffer A (B is
- * scratch of the same size).  Returns the buffer holding the result.          */
-template <typename T2>
-__device__ T2* stockham_lines(T2* A, T2* B, int L, int pitch, const FftPlanDev& pl, const T2* Wl, int sign) {
-    for (int p = 0; p < pl.nrad; ++p) {
-        const PassDev& ps = pl.pass[p];
-        switch (ps.R) {
-            case 2: stockham_pass<T2, 2>(A, B, L, pitch, ps, Wl, sign); break;
-            case 3: stockham_pass<T2, 3>(A, B, L, pitch, ps, Wl, sign); break;
-            case 4: stockham_pass<T2, 4>(A, B, L, pitch, ps, Wl, sign); break;
-            case 5: stockham_pass<T2, 5>(A, B, L, pitch, ps, Wl, sign); break;
-            case 7: stockham_pass<T2, 7>(A, B, L, pitch, ps, Wl, sign); break;
-            case 11: stockham_pass<T2, 11>(A, B, L, pitch, ps, Wl, sign); break;
-            case 13: stockham_pass<T2, 13>(A, B, L, pitch, ps, Wl, sign); break;
+/* butterflies per thread and pass: at most MAXE registers-resident elements */
+__host__ __device__ constexpr int nbt_of(int R, int MAXE) { return MAXE / R >= 1 ? MAXE / R : 1; }
+
+/* one in-place radix-R Stockham pass over nl lines.
+ * Butterfly j of a line (k = j mod Ns): inputs x[j + t nb] (times the input
+ * multiplier, then W_{Ns R}^{t k}), outputs y[(j / Ns) Ns R + k + t Ns].       */
+template <typename T2, int R, int MAXE>
+__device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb) {
+    constexpr int NBT = nbt_of(R, MAXE);
+    const int nb = st.nb, Ns = st.Ns, tstep = st.tstep;
+    const int total = nl * nb;
+    const bool inv = st.sign > 0;
+    const T2* pre = st.pre == PRE_NONE ? nullptr
+                  : (st.pre <= PRE_CHIRP_CONJ ? tb.chirp : (st.pre == PRE_BM ? tb.bm : tb.bp));
+    const bool preconj = st.pre == PRE_CHIRP_CONJ;
+    T2 v[NBT][R];
+    int ob[NBT];
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        const int idx = threadIdx.x + i * FFT_THREADS;
+        ob[i] = -1;
+        if (idx < total) {
+            const int line = udiv(idx, st.mNb);
+            const int j = idx - line * nb;
+            const int qd = udiv(j, st.mNs);
+            const int k = j - qd * Ns;
+            const T2* a = S + line * P;
+#pragma unroll
+            for (int t = 0; t < R; ++t) v[i][t] = a[j + t * nb];
+            if (pre) {
+#pragma unroll
+                for (int t = 0; t < R; ++t) {
+                    T2 w = pre[j + t * nb];
+                    if (preconj) w.y = -w.y;
+                    v[i][t] = cmul(v[i][t], w);
+                }
+            }
+            if (k) {
+#pragma unroll
+                for (int t = 1; t < R; ++t) {
+                    T2 w = tb.W[t * k * tstep];
+                    if (inv) w.y = -w.y;
+                    v[i][t] = cmul(v[i][t], w);
+                }
+            }
+            dft<T2, R>(v[i], inv);
+            ob[i] = line * P + qd * Ns * R + k;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+        if (ob[i] >= 0) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) S[ob[i] + t * Ns] = v[i][t];
+        }
+    }
+    __syncthreads();
+}
+
+struct NoMid { template <typename T2> __device__ void operator()(T2*) const {} };
+
+/* run the whole stage list; `mid` handles R = 0 stages and must end with a barrier */
+template <typename T2, int MAXE, typename Mid>
+__device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanDev& pl, const Tabs<T2>& tb,
+                                           const Mid& mid) {
+    for (int s = 0; s < pl.nst; ++s) {
+        const StageDev& st = pl.st[s];
+        switch (st.R) {
+            case 0: mid(S); break;
+            case 2: fft_pass<T2, 2, MAXE>(S, nl, P, st, tb); break;
+            case 3: fft_pass<T2, 3, MAXE>(S, nl, P, st, tb); break;
+            case 4: fft_pass<T2, 4, MAXE>(S, nl, P, st, tb); break;
+            case 5: fft_pass<T2, 5, MAXE>(S, nl, P, st, tb); break;
+            case 7: fft_pass<T2, 7, MAXE>(S, nl, P, st, tb); break;
+            case 8: fft_pass<T2, 8, MAXE>(S, nl, P, st, tb); break;
+            case 11: fft_pass<T2, 11, MAXE>(S, nl, P, st, tb); break;
+            case 13: fft_pass<T2, 13, MAXE>(S, nl, P, st, tb); break;
             default: break;
         }
-        __syncthreads();
-        T2* t = A; A = B; B = t;
     }
-    return A;
 }
 
-/* Transform L lines of length pl.n that sit (unpadded) at the front of each
- * LDS line of `pitch` >= pl.M elements.  Handles both direct and Bluestein
- * plans.  Returns the buffer (A or B) holding the n outputs of each line.
- * Caller must have synchronised after filling A.                              */
-template <typename T2, typename T>
-__device__ T2* fft_lines(T2* A, T2* B, int L, int pitch, const FftPlanDev& pl, const PlanLds<T2>& tb, int sign) {
-    if (!pl.blue) return stockham_lines<T2>(A, B, L, pitch, pl, tb.W, sign);
-    const T2* bhat = sign < 0 ? tb.bm : tb.bp;
-    const int n = pl.n, M = pl.M;
-    /* pre-chirp and zero padding: a[j] = x[j] w[j], w = exp(sign i pi j^2/n) */
-    for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
-        const int line = fdiv(idx, pl.mM), j = idx - line * M;
-        T2* a = A + line * pitch;
-        if (j < n) {
-            T2 w = tb.chirp[j];
-            if (sign > 0) w = cconj(w);
-            a[j] = cmul(a[j], w);
-        } else {
-            T2 z; z.x = 0; z.y = 0;
-            a[j] = z;
-        }
-    }
-    __syncthreads();
-    T2* R = stockham_lines<T2>(A, B, L, pitch, pl, tb.W, -1);
-    T2* S = (R == A) ? B : A;
-    for (int idx = threadIdx.x; idx < L * M; idx += blockDim.x) {
-        const int line = fdiv(idx, pl.mM), j = idx - line * M;
-        R[line * pitch + j] = cmul(R[line * pitch + j], bhat[j]);
-    }
-    __syncthreads();
-    R = stockham_lines<T2>(R, S, L, pitch, pl, tb.W, +1);
-    const T invM = (T)1 / (T)M;
-    for (int idx = threadIdx.x; idx < L * n; idx += blockDim.x) {
-        const int line = fdiv(idx, pl.mN), j = idx - line * n;
-        T2 w = tb.chirp[j];
-        if (sign > 0) w = cconj(w);
-        R[line * pitch + j] = cscale(cmul(R[line * pitch + j], w), invM);
-    }
-    __syncthreads();
-    return R;
-}
+/* register budget (complex elements per thread and pass) of each precision */
+template <typename T> struct MaxE;
+template <> struct MaxE<double> { static constexpr int v = 8; };
+template <> struct MaxE<float> { static constexpr int v = 16; };
 
 /* ------------------------------------------------------------------------- */
 /*  Per-problem constants of the operator                                      */
@@ -246,7 +334,7 @@ struct GridDev {
     int general;
     double ha, hb, hc;
     double iha, ihb, ihc;     /* 1/h */
-    unsigned long long m_na, m_nb;   /* magic for / na, / nb */
+    unsigned m_na, m_nb;      /* udiv magics for / na, / nb */
 };
 
 /* Line constants of the inner axis for line gl = c*nb + b: the symbols of the

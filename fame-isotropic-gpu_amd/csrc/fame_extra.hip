@@ -30,64 +30,71 @@ std::string& fame_err_ref();
 #define CHK(x) do { fame_status s_ = (x); if (s_ != FAME_OK) return s_; } while (0)
 
 /* inner pass of the curl-curl operator: F (after T* stages c and b) ->
- * conj(twid).F -> FFT_a -> -conj(L) x (L x .) -> IFFT_a -> twid . -> F        */
-__global__ void __launch_bounds__(256) k_curlcurl(KInner args) {
+ * conj(twid).F -> FFT_a -> -conj(L) x (L x .) -> IFFT_a -> twid . -> F
+ * (one stage list: forward transform, the curl-curl step, inverse transform;
+ * with Bluestein the forward post-chirp and inverse pre-chirp cancel and only
+ * 1/M remains in the step) */
+__global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
     typedef double2 T2;
     const int p = blockIdx.y;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const GridDev g = args.g;
+    const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
-    const FftPlanDev& pl = args.plan;
-    const int na = g.na, L = args.L, pitch = args.pitch;
-    T2* tbase = reinterpret_cast<T2*>(smem_raw);
-    PlanLds<T2> tb = load_plan_tables<T2>(pl, tbase);
-    T2* A = tbase + plan_table_len(pl);
-    T2* B = A + 3 * L * pitch;
-    __syncthreads();
+    const LinePlanDev& pl = args.plan;
+    const int na = g.na, L = args.L, P = args.P;
+    T2* base = reinterpret_cast<T2*>(smem_raw);
+    const Tabs<T2> tb = load_tabs<T2>(pl, base);
+    T2* S = base + pl.ntab;
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
     const size_t N = (size_t)g.N;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
-    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-        const int line = idx / na, a = idx - line * na, gl = l0 + line;
+    const int Wd = pl.blue ? pl.M : na;
+    const unsigned mW = pl.blue ? pl.mM : g.m_na;
+    for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+        const int line = udiv(idx, mW), a = idx - line * Wd, gl = l0 + line;
         T2 v[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-        if (gl < nlines) {
+        if (gl < nlines && a < na) {
             const int bf = gl % g.nb, cf = gl / g.nb;
             const T2 w = expipi(2.0 * (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a / (double)na);
             for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + (size_t)gl * na + a], w);
         }
-        for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
+        for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = v[c];
     }
     __syncthreads();
-    T2* R = fft_lines<T2, double>(A, B, 3 * L, pitch, pl, tb, -1);
-    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-        const int line = idx / na, a = idx - line * na, gl = l0 + line;
-        if (gl >= nlines) continue;
-        const int bf = gl % g.nb, cf = gl / g.nb;
-        double2 l[3];
-        lambdas(g, pr, a, bf, cf, l[0], l[1], l[2]);
-        double2 e[3];
-        for (int c = 0; c < 3; ++c) e[c] = R[(c * L + line) * pitch + a];
-        double2 u[3];   /* u = L x e */
-        u[0] = csub(cmul(l[1], e[2]), cmul(l[2], e[1]));
-        u[1] = csub(cmul(l[2], e[0]), cmul(l[0], e[2]));
-        u[2] = csub(cmul(l[0], e[1]), cmul(l[1], e[0]));
-        double2 m[3] = {make_double2(-l[0].x, l[0].y), make_double2(-l[1].x, l[1].y), make_double2(-l[2].x, l[2].y)};
-        double2 r[3];   /* r = (-conj L) x u */
-        r[0] = csub(cmul(m[1], u[2]), cmul(m[2], u[1]));
-        r[1] = csub(cmul(m[2], u[0]), cmul(m[0], u[2]));
-        r[2] = csub(cmul(m[0], u[1]), cmul(m[1], u[0]));
-        for (int c = 0; c < 3; ++c) R[(c * L + line) * pitch + a] = r[c];
-    }
-    __syncthreads();
-    T2* S = (R == A) ? B : A;
-    R = fft_lines<T2, double>(R, S, 3 * L, pitch, pl, tb, +1);
-    for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-        const int line = idx / na, a = idx - line * na, gl = l0 + line;
+    const double sc = pl.blue ? 1.0 / (double)pl.M : 1.0;
+    auto curlcurl = [&](T2* Sx) {
+        for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+            const int line = udiv(idx, mW), a = idx - line * Wd, gl = l0 + line;
+            double2 r[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+            if (gl < nlines && a < na) {
+                const int bf = gl % g.nb, cf = gl / g.nb;
+                double2 l[3];
+                lambdas(g, pr, a, bf, cf, l[0], l[1], l[2]);
+                double2 e[3];
+                for (int c = 0; c < 3; ++c) e[c] = cscale(Sx[(c * L + line) * P + a], sc);
+                double2 u[3];   /* u = L x e */
+                u[0] = csub(cmul(l[1], e[2]), cmul(l[2], e[1]));
+                u[1] = csub(cmul(l[2], e[0]), cmul(l[0], e[2]));
+                u[2] = csub(cmul(l[0], e[1]), cmul(l[1], e[0]));
+                double2 m[3] = {make_double2(-l[0].x, l[0].y), make_double2(-l[1].x, l[1].y), make_double2(-l[2].x, l[2].y)};
+                /* r = (-conj L) x u */
+                r[0] = csub(cmul(m[1], u[2]), cmul(m[2], u[1]));
+                r[1] = csub(cmul(m[2], u[0]), cmul(m[0], u[2]));
+                r[2] = csub(cmul(m[0], u[1]), cmul(m[1], u[0]));
+            }
+            for (int c = 0; c < 3; ++c) Sx[(c * L + line) * P + a] = r[c];
+        }
+        __syncthreads();
+    };
+    run_stages<T2, MaxE<double>::v>(S, 3 * L, P, pl, tb, curlcurl);
+    for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
+        const int line = udiv(idx, g.m_na), a = idx - line * na, gl = l0 + line;
         if (gl >= nlines) continue;
         const int bf = gl % g.nb, cf = gl / g.nb;
         const T2 w = expipi(2.0 * (pr.t0 + pr.t1 * (double)bf + pr.t2 * (double)cf) * (double)a / (double)na);
-        for (int c = 0; c < 3; ++c) F[c * N + (size_t)gl * na + a] = cmul(R[(c * L + line) * pitch + a], w);
+        for (int c = 0; c < 3; ++c)
+            F[c * N + (size_t)gl * na + a] = cmul(post_mul<T2, double>(pl, tb, a, S[(c * L + line) * P + a]), w);
     }
 }
 
@@ -163,17 +170,16 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
             w2[p] = freq[(size_t)p * nwant + i] * freq[(size_t)p * nwant + i];
         }
         if (st != FAME_OK) break;
-        KAxis ko = c->kout(8, 0, nullptr);
-        KAxis km = c->kmid(8, 0, nullptr);
-        KInner ki = c->kin(8, 0, nullptr);
+        KInner ki = c->kin(8, 0, nullptr, LIST_FI);
         if (hipMemcpyAsync(w2d, w2.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-            fame_launch_outer(ko, OUTER_QRS, 8, nk, c->stream) != hipSuccess ||
-            fame_launch_mid(km, -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+            fame_launch_outer(c->kout(8, 0, nullptr, LIST_FWD), OUTER_QRS, 8, nk, c->stream) != hipSuccess ||
+            fame_launch_mid(c->kmid(8, 0, nullptr, LIST_FWD), -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
         dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, nk);
-        size_t lds = 16 * ((size_t)plan_table_len(ki.plan) + 2 * 3 * (size_t)ki.L * ki.pitch);
-        hipLaunchKernelGGL(k_curlcurl, grid, dim3(256), lds, c->stream, ki);
-        if (hipGetLastError() != hipSuccess || fame_launch_mid(km, +1, 8, nk, c->stream) != hipSuccess ||
-            fame_launch_outer(ko, OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+        const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
+        hipLaunchKernelGGL(k_curlcurl, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        if (hipGetLastError() != hipSuccess ||
+            fame_launch_mid(c->kmid(8, 0, nullptr, LIST_INV), +1, 8, nk, c->stream) != hipSuccess ||
+            fame_launch_outer(c->kout(8, 0, nullptr, LIST_INV), OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
         hipLaunchKernelGGL(k_resid, dim3(NB, nk), dim3(256), 0, c->stream, c->F64, Ecopy, c->eps_int, w2d, c->partials,
                            N3, N3);
         if (hipMemcpyAsync(hp.data(), c->partials, sizeof(double) * nk * NB * 2, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -203,19 +209,20 @@ extern "C" fame_status fame_time_qbq_passes(fame_ctx* c, int prec, int reps, dou
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    KInner ki = c->kin(prec, 0, nullptr);
-    KAxis km = c->kmid(prec, 0, nullptr);
-    KAxis ko = c->kout(prec, 0, nullptr);
-    ki.x = x; ki.y = y; ki.pikind = 0;
+    KInner ki = c->kin(prec, 0, nullptr, LIST_INV), kf = c->kin(prec, 0, nullptr, LIST_FWD);
+    const KAxis kmi = c->kmid(prec, 0, nullptr, LIST_INV), kmf = c->kmid(prec, 0, nullptr, LIST_FWD);
+    const KAxis ko = c->kout(prec, 0, nullptr, LIST_EPS);
+    ki.x = x; ki.pikind = 0;
+    kf.y = y; kf.pikind = 0;
     for (int which = 0; which < 6; ++which) {
         for (int rep = -1; rep < reps; ++rep) {
             if (rep == 0) HIPCHK(hipEventRecord(e0, c->stream));
             switch (which) {
                 case 0: HIPCHK(fame_launch_inner(ki, 1, prec, nk, c->stream)); break;
-                case 1: HIPCHK(fame_launch_mid(km, +1, prec, nk, c->stream)); break;
+                case 1: HIPCHK(fame_launch_mid(kmi, +1, prec, nk, c->stream)); break;
                 case 2: HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nk, c->stream)); break;
-                case 3: HIPCHK(fame_launch_mid(km, -1, prec, nk, c->stream)); break;
-                case 4: HIPCHK(fame_launch_inner(ki, 0, prec, nk, c->stream)); break;
+                case 3: HIPCHK(fame_launch_mid(kmf, -1, prec, nk, c->stream)); break;
+                case 4: HIPCHK(fame_launch_inner(kf, 0, prec, nk, c->stream)); break;
                 default: CHK(c->qbq(prec, x, y, 0, nk, nullptr, nullptr)); break;
             }
         }

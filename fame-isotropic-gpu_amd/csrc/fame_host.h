@@ -12,7 +12,8 @@
 struct AxisPlan {
     int n = 0, M = 0, blue = 0;
     std::vector<int> radices;
-    fame::FftPlanDev dev[2];      /* [0] FP64 tables, [1] FP32 tables */
+    fame::LinePlanDev lists[2][NLIST];   /* [0] FP64, [1] FP32; LIST_* stage lists */
+    int cap[2] = {0, 0};                 /* elements a 256-thread block transforms at once */
     std::vector<void*> bufs;
     fame_status build(int n);
     ~AxisPlan();
@@ -25,7 +26,7 @@ struct fame_ctx {
     fame::GridDev g;
     int N = 0, maxb = 0;
     AxisPlan pa, pb, pc;
-    int Lin[2] = {0, 0}, pin[2] = {0, 0}, Lmid[2] = {0, 0}, pmid[2] = {0, 0}, Lout[2] = {0, 0}, pout[2] = {0, 0};
+    int Lin[2] = {0, 0}, Pin[2] = {0, 0}, Lmid[2] = {0, 0}, Pmid[2] = {0, 0}, Lout[2] = {0, 0}, Pout[2] = {0, 0};
     int nblk_in[2] = {0, 0};
     /* permittivity, kernel layout */
     double* invB64 = nullptr; float* invB32 = nullptr; double* eps_int = nullptr; double* eps_ref = nullptr;
@@ -59,9 +60,9 @@ struct fame_ctx {
     long long cnt_qbq64 = 0, cnt_qbq32 = 0;
     int last_iters = 0;
 
-    KInner kin(int prec, int p0, const int* act) const;
-    KAxis kmid(int prec, int p0, const int* act) const;
-    KAxis kout(int prec, int p0, const int* act) const;
+    KInner kin(int prec, int p0, const int* act, int list) const;
+    KAxis kmid(int prec, int p0, const int* act, int list) const;
+    KAxis kout(int prec, int p0, const int* act, int list) const;
     fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec);
     fame_status qr_field(const void* x, int p0, int nprob, int pikind, const int* act);
     fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
@@ -74,3 +75,4 @@ struct fame_ctx {
 };
 
 void fame_phases(const fame_lattice& L, const double* k, fame::ProbDev* p);
+unsigned fame_magic32(int d);

@@ -9,10 +9,18 @@
 
 enum { OUTER_EPS = 0, OUTER_QR = 1, OUTER_QRS = 2 };
 
+#define MAX_LINES_INNER 32   /* lines (b, c) per block of the inner-axis kernels */
+
+/* stage lists an axis plan provides (per precision):
+ * FWD  forward transform, INV inverse transform (both unnormalised),
+ * EPS  INV, kernel elementwise step, FWD (outer pass of QBQ),
+ * FI   FWD, kernel elementwise step, INV (curl-curl inner pass).             */
+enum { LIST_FWD = 0, LIST_INV = 1, LIST_EPS = 2, LIST_FI = 3, NLIST = 4 };
+
 struct KInner {
     fame::GridDev g;
     const fame::ProbDev* prob;   /* device, one per problem                    */
-    fame::FftPlanDev plan;       /* transform along the inner axis              */
+    fame::LinePlanDev plan;      /* stage list along the inner axis            */
     const int* active;           /* device flags or nullptr                    */
     const void* x;               /* EXPAND: reduced input (padded, 2N/problem) */
     void* y;                     /* !EXPAND: reduced output                    */
@@ -20,20 +28,24 @@ struct KInner {
     const void* dotvec;          /* !EXPAND: optional d for partial conj(d).y  */
     double* partials;            /* [problem][block][2]                        */
     size_t xstride, fstride;
-    int L, pitch;                /* lines per block, LDS line pitch            */
+    int L, P;                    /* lines per block, LDS line pitch            */
     int pikind;                  /* 0 Pi_Qr, 1 Pi_Pr                           */
 };
 
 struct KAxis {
     fame::GridDev g;
     const fame::ProbDev* prob;
-    fame::FftPlanDev plan;
+    fame::LinePlanDev plan;
     const int* active;
     void* f;
     const void* invB;            /* outer EPS mode: 3N reals, internal layout  */
     size_t fstride;
-    int L, pitch;
+    int L, P;                    /* lines per block, LDS line pitch            */
+    unsigned mL;                 /* udiv magic for / L                         */
 };
+
+/* LDS bytes of a line-transform launch: tables + nl lines + extra entries */
+size_t fame_line_lds(const fame::LinePlanDev& pl, int prec, int nl, int P, int extra);
 
 hipError_t fame_launch_inner(const KInner& a, int expand, int prec, int nprob, hipStream_t s);
 hipError_t fame_launch_mid(const KAxis& a, int dir, int prec, int nprob, hipStream_t s);

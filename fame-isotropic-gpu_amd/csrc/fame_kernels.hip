@@ -29,7 +29,16 @@ template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return
 __device__ __forceinline__ double2 to_d(double2 v) { return v; }
 __device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
 
-#define MAX_LINES 32
+/* exp(i pi x) in the kernel precision.  FP32: the argument is reduced to
+ * [-1, 1] in FP64 first, so that the FP32 evaluation loses nothing for large x. */
+template <typename T2> __device__ __forceinline__ T2 tw_expipi(double x);
+template <> __device__ __forceinline__ double2 tw_expipi<double2>(double x) { return expipi(x); }
+template <> __device__ __forceinline__ float2 tw_expipi<float2>(double x) {
+    const float r = (float)(x - 2.0 * rint(0.5 * x));
+    float s, c;
+    sincospif(r, &s, &c);
+    return make_float2(c, s);
+}
 
 /* ------------------------------------------------------------------------- */
 /*  P1 / P5: inner axis                                                       */
@@ -37,28 +46,28 @@ __device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y
 /* EXPAND = 1 : F = twid .* IFFT_a( Pi x )          (Qr / Pr first stage)
  * EXPAND = 0 : y = Pi^H FFT_a( conj(twid) .* F )   (Qrs / Prs last stage)
  * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
- *              (FAME_Matrix_Lambdas.cu:217-243)                                 */
+ *              (FAME_Matrix_Lambdas.cu:217-243)
+ * A block owns L lines (b, c) and their three components: 3L LDS lines.      */
 template <typename T, int EXPAND>
-__global__ void __launch_bounds__(256) k_inner(KInner args) {
+__global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y;
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    __shared__ LineSym lsym[MAX_LINES];
-    const GridDev g = args.g;
+    __shared__ LineSym lsym[MAX_LINES_INNER];
+    const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
-    const FftPlanDev& pl = args.plan;
-    const int na = g.na, L = args.L, pitch = args.pitch;
+    const LinePlanDev& pl = args.plan;
+    const int na = g.na, L = args.L, P = args.P;
     T2* base = reinterpret_cast<T2*>(smem_raw);
-    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
-    T2* A = base + plan_table_len(pl);
-    T2* B = A + 3 * L * pitch;
+    const Tabs<T2> tb = load_tabs<T2>(pl, base);
+    T2* S = base + pl.ntab;
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
     if ((int)threadIdx.x < L) {
         const int gl = l0 + threadIdx.x;
         if (gl < nlines) {
-            const int cf = fdiv(gl, g.m_nb), bf = gl - cf * g.nb;
+            const int cf = udiv(gl, g.m_nb), bf = gl - cf * g.nb;
             lsym[threadIdx.x] = line_sym(g, pr, bf, cf);
         }
     }
@@ -67,13 +76,16 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
     const T2* X = reinterpret_cast<const T2*>(args.x) + (size_t)p * args.xstride;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
     const double inv_na = 1.0 / (double)na;
+    /* width written per LDS line: na, or the zero-padded Bluestein length */
+    const int Wd = pl.blue ? pl.M : na;
+    const unsigned mW = pl.blue ? pl.mM : g.m_na;
 
     if (EXPAND) {
-        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = fdiv(idx, g.m_na), a = idx - line * na;
+        for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+            const int line = udiv(idx, mW), a = idx - line * Wd;
             const int gl = l0 + line;
             double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-            if (gl < nlines) {
+            if (gl < nlines && a < na) {
                 const size_t f = (size_t)gl * na + a;
                 if (!(pr.gamma && f == 0)) {
                     double2 lx, ly, lz, p1[3], p2[3];
@@ -88,43 +100,43 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
                 }
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = from_d<T2>(u[c]);
+            for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
         }
         __syncthreads();
-        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, tb, +1);
-        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = fdiv(idx, g.m_na), a = idx - line * na;
+        run_stages<T2, MaxE<T>::v>(S, 3 * L, P, pl, tb, NoMid());
+        for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
+            const int line = udiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
-            const T2 w = from_d<T2>(expipi(2.0 * lsym[line].phi * (double)a * inv_na));
+            const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
             const size_t f = (size_t)gl * na + a;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) F[c * N + f] = cmul(R[(c * L + line) * pitch + a], w);
+            for (int c = 0; c < 3; ++c)
+                F[c * N + f] = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w);
         }
     } else {
-        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = fdiv(idx, g.m_na), a = idx - line * na;
+        for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+            const int line = udiv(idx, mW), a = idx - line * Wd;
             const int gl = l0 + line;
             T2 v[3];
-            if (gl < nlines) {
-                const T2 w = from_d<T2>(expipi(2.0 * lsym[line].phi * (double)a * inv_na));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) { v[c].x = 0; v[c].y = 0; }
+            if (gl < nlines && a < na) {
+                const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
                 const size_t f = (size_t)gl * na + a;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + f], w);
-            } else {
-#pragma unroll
-                for (int c = 0; c < 3; ++c) { v[c].x = 0; v[c].y = 0; }
             }
 #pragma unroll
-            for (int c = 0; c < 3; ++c) A[(c * L + line) * pitch + a] = v[c];
+            for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = v[c];
         }
         __syncthreads();
-        T2* R = fft_lines<T2, T>(A, B, 3 * L, pitch, pl, tb, -1);
+        run_stages<T2, MaxE<T>::v>(S, 3 * L, P, pl, tb, NoMid());
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0;
-        for (int idx = threadIdx.x; idx < L * na; idx += blockDim.x) {
-            const int line = fdiv(idx, g.m_na), a = idx - line * na;
+        for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
+            const int line = udiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
             const size_t f = (size_t)gl * na + a;
@@ -135,7 +147,7 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
                 pi_basis(g, lx, ly, lz, p1, p2);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const double2 w = to_d(R[(c * L + line) * pitch + a]);
+                    const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
                     if (args.pikind == 0) {
                         y1 = cadd(y1, cmul(cconj(p1[c]), w));
                         y2 = cadd(y2, cmul(cconj(p2[c]), w));
@@ -156,21 +168,18 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
             }
         }
         if (D) {
-            /* block reduction of conj(d) . y into partials[p][blockIdx.x] */
-            __shared__ double red[2][256];
+            /* block sum of conj(d) . y into partials[p][blockIdx.x]: wave
+             * shuffles, then the four wave sums in a fixed order */
+            __shared__ double red[2][FFT_THREADS / 64];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) { dre += __shfl_xor(dre, o); dim += __shfl_xor(dim, o); }
+            if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = dre; red[1][threadIdx.x >> 6] = dim; }
             __syncthreads();
-            red[0][threadIdx.x] = dre; red[1][threadIdx.x] = dim;
-            __syncthreads();
-            for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-                if ((int)threadIdx.x < s) {
-                    red[0][threadIdx.x] += red[0][threadIdx.x + s];
-                    red[1][threadIdx.x] += red[1][threadIdx.x + s];
-                }
-                __syncthreads();
-            }
             if (threadIdx.x == 0) {
+                double sr = 0, si = 0;
+                for (int w = 0; w < FFT_THREADS / 64; ++w) { sr += red[0][w]; si += red[1][w]; }
                 double* part = args.partials + ((size_t)p * gridDim.x + blockIdx.x) * 2;
-                part[0] = red[0][0]; part[1] = red[1][0];
+                part[0] = sr; part[1] = si;
             }
         }
     }
@@ -179,41 +188,47 @@ __global__ void __launch_bounds__(256) k_inner(KInner args) {
 /* ------------------------------------------------------------------------- */
 /*  P2 / P4: middle axis, per component                                      */
 /* ------------------------------------------------------------------------- */
-/* DIR = +1: F = twid_b .* IFFT_b(F) ; DIR = -1: F = FFT_b(conj(twid_b) .* F)   */
+/* DIR = +1: F = twid_b .* IFFT_b(F) ; DIR = -1: F = FFT_b(conj(twid_b) .* F).
+ * A block owns L consecutive a of one (problem, component, c) plane; the
+ * middle twiddle depends on (c, b) only and is tabulated once per block.     */
 template <typename T, int DIR>
-__global__ void __launch_bounds__(256) k_mid(KAxis args) {
+__global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const GridDev g = args.g;
+    const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
-    const FftPlanDev& pl = args.plan;
-    const int na = g.na, nb = g.nb, TA = args.L, pitch = args.pitch;
+    const LinePlanDev& pl = args.plan;
+    const int na = g.na, nb = g.nb, L = args.L, P = args.P;
     T2* base = reinterpret_cast<T2*>(smem_raw);
-    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
-    T2* A = base + plan_table_len(pl);
-    T2* B = A + TA * pitch;
-    const int ntile = (na + TA - 1) / TA;
-    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x - cf * ntile) * TA;
+    const Tabs<T2> tb = load_tabs<T2>(pl, base);
+    T2* twb = base + pl.ntab;
+    T2* S = twb + nb;
+    const int ntile = (na + L - 1) / L;
+    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x - cf * ntile) * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
     const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)nb;
-    for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
-        const int b = idx / TA, ta = idx - b * TA, a = a0 + ta;
+    for (int b = threadIdx.x; b < nb; b += FFT_THREADS) twb[b] = tw_expipi<T2>(phc * (double)b);
+    __syncthreads();
+    const int Wd = pl.blue ? pl.M : nb;
+    for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+        const int b = udiv(idx, args.mL), ta = idx - b * L, a = a0 + ta;
         T2 v;
-        if (a < na) {
+        v.x = 0; v.y = 0;
+        if (a < na && b < nb) {
             v = F[(size_t)b * na + a];
-            if (DIR < 0) v = cmulc(v, from_d<T2>(expipi(phc * (double)b)));
-        } else { v.x = 0; v.y = 0; }
-        A[ta * pitch + b] = v;
+            if (DIR < 0) v = cmulc(v, twb[b]);
+        }
+        S[ta * P + b] = v;
     }
     __syncthreads();
-    T2* R = fft_lines<T2, T>(A, B, TA, pitch, pl, tb, DIR > 0 ? +1 : -1);
-    for (int idx = threadIdx.x; idx < TA * nb; idx += blockDim.x) {
-        const int b = idx / TA, ta = idx - b * TA, a = a0 + ta;
+    run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, NoMid());
+    for (int idx = threadIdx.x; idx < L * nb; idx += FFT_THREADS) {
+        const int b = udiv(idx, args.mL), ta = idx - b * L, a = a0 + ta;
         if (a >= na) continue;
-        T2 v = R[ta * pitch + b];
-        if (DIR > 0) v = cmul(v, from_d<T2>(expipi(phc * (double)b)));
+        T2 v = post_mul<T2, T>(pl, tb, b, S[ta * P + b]);
+        if (DIR > 0) v = cmul(v, twb[b]);
         F[(size_t)b * na + a] = v;
     }
 }
@@ -223,65 +238,62 @@ __global__ void __launch_bounds__(256) k_mid(KAxis args) {
 /* ------------------------------------------------------------------------- */
 /* mode OUTER_EPS   : F = FFT_c( invB/N .* IFFT_c(F) )
  * mode OUTER_QR    : F = D_c/N .* IFFT_c(F)            (end of Qr / Pr)
- * mode OUTER_QRS   : F = FFT_c( conj(D_c) .* F )       (start of Qrs / Prs)   */
+ * mode OUTER_QRS   : F = FFT_c( conj(D_c) .* F )       (start of Qrs / Prs)
+ * A block owns L consecutive (a, b) positions q of one (problem, component).
+ * In EPS mode with Bluestein the inverse post-chirp and the forward pre-chirp
+ * are conjugate unit factors whose product is 1; both are left out and only
+ * the 1/M remains in the elementwise step.                                  */
 template <typename T, int MODE>
-__global__ void __launch_bounds__(256) k_outer(KAxis args) {
+__global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    const GridDev g = args.g;
+    const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
-    const FftPlanDev& pl = args.plan;
-    const int nc = g.nc, TQ = args.L, pitch = args.pitch;
+    const LinePlanDev& pl = args.plan;
+    const int nc = g.nc, L = args.L, P = args.P;
     const int plane = g.na * g.nb;
     T2* base = reinterpret_cast<T2*>(smem_raw);
-    PlanLds<T2> tb = load_plan_tables<T2>(pl, base);
-    T2* A = base + plan_table_len(pl);
-    T2* B = A + TQ * pitch;
-    const int q0 = blockIdx.x * TQ;
+    const Tabs<T2> tb = load_tabs<T2>(pl, base);
+    T2* S = base + pl.ntab;
+    const int q0 = blockIdx.x * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
-    for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-        const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
+    const int Wd = pl.blue ? pl.M : nc;
+    for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+        const int c = udiv(idx, args.mL), tq = idx - c * L, q = q0 + tq;
         T2 v;
-        if (q < plane) {
+        v.x = 0; v.y = 0;
+        if (q < plane && c < nc) {
             v = F[(size_t)c * plane + q];
-            if (MODE == OUTER_QRS) v = cmulc(v, from_d<T2>(expipi(2.0 * pr.r0 * (double)c / (double)nc)));
-        } else { v.x = 0; v.y = 0; }
-        A[tq * pitch + c] = v;
+            if (MODE == OUTER_QRS) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
+        }
+        S[tq * P + c] = v;
     }
     __syncthreads();
     if (MODE == OUTER_EPS) {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, +1);
-        T2* S = (R == A) ? B : A;
         const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
-        const T invN = (T)(1.0 / (double)g.N);
-        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
-            T s = (q < plane) ? invB[(size_t)c * plane + q] * invN : (T)0;
-            R[tq * pitch + c] = cscale(R[tq * pitch + c], s);
-        }
-        __syncthreads();
-        R = fft_lines<T2, T>(R, S, TQ, pitch, pl, tb, -1);
-        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
-            if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
-        }
-    } else if (MODE == OUTER_QR) {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, +1);
-        const double invN = 1.0 / (double)g.N;
-        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
-            if (q >= plane) continue;
-            double2 w = expipi(2.0 * pr.r0 * (double)c / (double)nc);
-            F[(size_t)c * plane + q] = cmul(R[tq * pitch + c], from_d<T2>(cscale(w, invN)));
-        }
+        const T scale = (T)(1.0 / (double)g.N / (pl.blue ? (double)pl.M : 1.0));
+        const unsigned mL = args.mL;
+        auto eps_step = [&](T2* Sx) {
+            for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
+                const int c = udiv(idx, mL), tq = idx - c * L, q = q0 + tq;
+                const T s = (q < plane && c < nc) ? invB[(size_t)c * plane + q] * scale : (T)0;
+                Sx[tq * P + c] = cscale(Sx[tq * P + c], s);
+            }
+            __syncthreads();
+        };
+        run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, eps_step);
     } else {
-        T2* R = fft_lines<T2, T>(A, B, TQ, pitch, pl, tb, -1);
-        for (int idx = threadIdx.x; idx < TQ * nc; idx += blockDim.x) {
-            const int c = idx / TQ, tq = idx - c * TQ, q = q0 + tq;
-            if (q < plane) F[(size_t)c * plane + q] = R[tq * pitch + c];
-        }
+        run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, NoMid());
+    }
+    const double invN = 1.0 / (double)g.N;
+    for (int idx = threadIdx.x; idx < L * nc; idx += FFT_THREADS) {
+        const int c = udiv(idx, args.mL), tq = idx - c * L, q = q0 + tq;
+        if (q >= plane) continue;
+        T2 v = post_mul<T2, T>(pl, tb, c, S[tq * P + c]);
+        if (MODE == OUTER_QR) v = cmul(v, from_d<T2>(cscale(expipi(2.0 * pr.r0 * (double)c / (double)nc), invN)));
+        F[(size_t)c * plane + q] = v;
     }
 }
 
@@ -369,18 +381,17 @@ __global__ void k_sigma(GridDev g, const ProbDev* prob, double* sig, double* sig
 /* ========================================================================= */
 using namespace fame;
 
-template <typename T>
-static size_t lds_bytes(const FftPlanDev& pl, int nbuf_lines, int pitch) {
-    return sizeof(typename Vec<T>::T2) * ((size_t)plan_table_len(pl) + 2 * (size_t)nbuf_lines * pitch);
+size_t fame_line_lds(const LinePlanDev& pl, int prec, int nl, int P, int extra) {
+    return (size_t)(prec == 8 ? 16 : 8) * ((size_t)pl.ntab + (size_t)nl * P + (size_t)extra);
 }
 
 template <typename T>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
     dim3 grid((nlines + a.L - 1) / a.L, nprob);
-    size_t lds = lds_bytes<T>(a.plan, 3 * a.L, a.pitch);
-    if (expand) hipLaunchKernelGGL((k_inner<T, 1>), grid, dim3(256), lds, s, a);
-    else hipLaunchKernelGGL((k_inner<T, 0>), grid, dim3(256), lds, s, a);
+    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, 0);
+    if (expand) hipLaunchKernelGGL((k_inner<T, 1>), grid, dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_inner<T, 0>), grid, dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
@@ -388,9 +399,9 @@ template <typename T>
 static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) {
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(ntile * a.g.nc, 3 * nprob);
-    size_t lds = lds_bytes<T>(a.plan, a.L, a.pitch);
-    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1>), grid, dim3(256), lds, s, a);
-    else hipLaunchKernelGGL((k_mid<T, -1>), grid, dim3(256), lds, s, a);
+    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
+    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1>), grid, dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_mid<T, -1>), grid, dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
@@ -398,11 +409,11 @@ template <typename T>
 static hipError_t launch_outer(const KAxis& a, int mode, int nprob, hipStream_t s) {
     const int plane = a.g.na * a.g.nb;
     dim3 grid((plane + a.L - 1) / a.L, 3 * nprob);
-    size_t lds = lds_bytes<T>(a.plan, a.L, a.pitch);
+    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, 0);
     switch (mode) {
-        case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS>), grid, dim3(256), lds, s, a); break;
-        case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR>), grid, dim3(256), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_outer<T, OUTER_QRS>), grid, dim3(256), lds, s, a); break;
+        case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS>), grid, dim3(FFT_THREADS), lds, s, a); break;
+        case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR>), grid, dim3(FFT_THREADS), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_outer<T, OUTER_QRS>), grid, dim3(FFT_THREADS), lds, s, a); break;
     }
     return hipGetLastError();
 }

@@ -42,28 +42,63 @@ std::string& fame_err_ref() { return g_err; }
 /* ======================================================================== */
 /*  FFT plans                                                                */
 /* ======================================================================== */
+/* radices 8,4,2 then odd primes up to 13, largest first (the first pass of a
+ * Stockham transform needs no twiddles, so it gets the largest butterfly) */
 static bool factor_gpu(int n, std::vector<int>* rad) {
     std::vector<int> r;
     int m = n;
+    while (m % 8 == 0) { r.push_back(8); m /= 8; }
     while (m % 4 == 0) { r.push_back(4); m /= 4; }
     while (m % 2 == 0) { r.push_back(2); m /= 2; }
     const int ps[] = {3, 5, 7, 11, 13};
     for (int p : ps) while (m % p == 0) { r.push_back(p); m /= p; }
     if (m != 1) return false;
-    if ((int)r.size() > FAME_MAX_PASSES) return false;
+    std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
     if (rad) *rad = r;
     return true;
 }
+/* relative cost of one element through one pass of radix r (LDS round trip,
+ * twiddle and butterfly arithmetic) */
 static double pass_cost(int r) {
-    switch (r) { case 2: return 1.5; case 4: return 2.5; case 3: return 4; case 5: return 6; case 7: return 8;
-                 case 11: return 12; case 13: return 14; default: return 1e9; }
+    switch (r) { case 2: return 18; case 3: return 20; case 4: return 20; case 5: return 23; case 7: return 26;
+                 case 8: return 23; case 11: return 30; case 13: return 32; default: return 1e9; }
 }
 static double plan_cost(const std::vector<int>& rad) { double c = 0; for (int r : rad) c += pass_cost(r); return c; }
+
+unsigned fame_magic32(int d) {
+    return d <= 1 ? 0u : (unsigned)(((1ULL << 32) + (unsigned long long)d - 1) / (unsigned long long)d);
+}
 
 typedef std::complex<long double> cld;
 
 AxisPlan::~AxisPlan() {
     for (void* p : bufs) if (p) (void)hipFree(p);
+}
+
+/* append one transform of sign `sign` to a stage list */
+static bool add_transform(LinePlanDev& d, const std::vector<int>& rad, int M, int sign, int blue, bool prechirp) {
+    auto passes = [&](int sgn, int pre0) -> bool {
+        int Ns = 1;
+        for (size_t i = 0; i < rad.size(); ++i) {
+            if (d.nst >= FAME_MAX_STAGES) return false;
+            StageDev& s = d.st[d.nst++];
+            s.R = rad[i]; s.Ns = Ns; s.nb = M / s.R; s.tstep = M / (Ns * s.R);
+            s.mNs = fame_magic32(Ns); s.mNb = fame_magic32(s.nb);
+            s.sign = sgn; s.pre = i == 0 ? pre0 : PRE_NONE;
+            Ns *= s.R;
+        }
+        return true;
+    };
+    if (!blue) return passes(sign, PRE_NONE);
+    /* chirp-z: X = post .* IFFT_M( bhat_sign .* FFT_M( pre .* x ) ) */
+    return passes(-1, prechirp ? (sign < 0 ? PRE_CHIRP : PRE_CHIRP_CONJ) : PRE_NONE) &&
+           passes(+1, sign < 0 ? PRE_BM : PRE_BP);
+}
+static bool add_mid(LinePlanDev& d) {
+    if (d.nst >= FAME_MAX_STAGES) return false;
+    StageDev& s = d.st[d.nst++];
+    std::memset(&s, 0, sizeof(s));
+    return true;
 }
 
 fame_status AxisPlan::build(int n_) {
@@ -74,12 +109,8 @@ fame_status AxisPlan::build(int n_) {
     int bestM = 0; std::vector<int> bestR; double bestC = 1e30;
     for (int m = 2 * n - 1; m <= 4 * n + 8; ++m) {
         std::vector<int> r2;
-        bool ok = factor_gpu(m, &r2);
-        if (!ok) continue;
-        bool small = true;
-        for (int q : r2) if (q > 7) small = false;
-        if (!small) continue;
-        double c = 2.0 * m * plan_cost(r2) + 3.0 * m;
+        if (!factor_gpu(m, &r2)) continue;
+        double c = 2.0 * m * plan_cost(r2) + 12.0 * m;
         if (c < bestC) { bestC = c; bestM = m; bestR = r2; }
     }
     if (direct <= bestC && direct < 1e29) { blue = 0; M = n; radices = rad; }
@@ -90,7 +121,7 @@ fame_status AxisPlan::build(int n_) {
     std::vector<cld> W(M), ch, bm, bp;
     for (int m = 0; m < M; ++m) W[m] = cld(cosl(2 * PI_L * m / M), -sinl(2 * PI_L * m / M));
     if (blue) {
-        ch.resize(n);
+        ch.assign(M, 0);
         for (int m = 0; m < n; ++m) {
             long long q = ((long long)m * m) % (2LL * n);
             ch[m] = cld(cosl(PI_L * q / n), -sinl(PI_L * q / n));     /* exp(-i pi m^2/n) */
@@ -130,40 +161,58 @@ fame_status AxisPlan::build(int n_) {
         *dst = d;
         return FAME_OK;
     };
-    auto magic = [](int d) -> unsigned long long { return ((1ULL << 32) + (unsigned long long)d - 1) / (unsigned long long)d; };
     for (int pi = 0; pi < 2; ++pi) {
-        int prec = pi == 0 ? 8 : 4;
-        FftPlanDev& d = dev[pi];
-        std::memset(&d, 0, sizeof(d));
-        d.n = n; d.M = M; d.blue = blue; d.nrad = (int)radices.size();
-        d.mM = magic(M); d.mN = magic(n);
-        int Ns = 1;
-        for (int i = 0; i < d.nrad; ++i) {
-            PassDev& ps = d.pass[i];
-            ps.R = radices[i]; ps.Ns = Ns; ps.nb = M / ps.R; ps.tstep = M / (Ns * ps.R);
-            ps.mNs = magic(Ns); ps.mNb = magic(ps.nb);
-            Ns *= ps.R;
+        const int prec = pi == 0 ? 8 : 4;
+        LinePlanDev base;
+        std::memset(&base, 0, sizeof(base));
+        base.n = n; base.M = M; base.blue = blue;
+        base.ntab = blue ? 4 * M : M;
+        base.mM = fame_magic32(M); base.mN = fame_magic32(n);
+        CHK(upload(W, prec, &base.W));
+        CHK(upload(ch, prec, &base.chirp));
+        CHK(upload(bm, prec, &base.bm));
+        CHK(upload(bp, prec, &base.bp));
+        bool ok = true;
+        for (int l = 0; l < NLIST; ++l) {
+            LinePlanDev& d = lists[pi][l];
+            d = base;
+            switch (l) {
+                case LIST_FWD: ok = ok && add_transform(d, radices, M, -1, blue, true); d.post = blue ? POST_CHIRP : POST_NONE; break;
+                case LIST_INV: ok = ok && add_transform(d, radices, M, +1, blue, true); d.post = blue ? POST_CHIRP_CONJ : POST_NONE; break;
+                case LIST_EPS:
+                    ok = ok && add_transform(d, radices, M, +1, blue, true) && add_mid(d) && add_transform(d, radices, M, -1, blue, false);
+                    d.post = blue ? POST_CHIRP : POST_NONE;
+                    break;
+                default:
+                    ok = ok && add_transform(d, radices, M, -1, blue, true) && add_mid(d) && add_transform(d, radices, M, +1, blue, false);
+                    d.post = blue ? POST_CHIRP_CONJ : POST_NONE;
+                    break;
+            }
         }
-        CHK(upload(W, prec, &d.W));
-        CHK(upload(ch, prec, &d.chirp));
-        CHK(upload(bm, prec, &d.bhat_m));
-        CHK(upload(bp, prec, &d.bhat_p));
+        if (!ok) FAIL(FAME_ERR_UNSUPPORTED, "too many FFT passes for length " + std::to_string(n));
+        /* register budget per thread: 8 (FP64) / 16 (FP32) complex elements */
+        const int maxe = prec == 8 ? MaxE<double>::v : MaxE<float>::v;
+        int c = 1 << 30;
+        for (int r : radices) c = std::min(c, r * nbt_of(r, maxe));
+        cap[pi] = FFT_THREADS * c;
     }
     return FAME_OK;
 }
 
-/* lines per block so that the kernel's LDS (plan tables + two line buffers)
- * stays within 56 KiB, leaving room for the static reduction scratch */
-static void pick_tile(const AxisPlan& p, int ncomp, int prec, int maxL, int align, int* L, int* pitch) {
-    const int es = prec == 8 ? 16 : 8;
-    const int budget = 57344 / es;
-    const int tables = plan_table_len(p.dev[prec == 8 ? 0 : 1]);
-    int pt = p.M;
-    int l = (budget - tables) / (2 * ncomp * pt);
+/* lines per block: as many as the per-thread register budget allows (cap
+ * elements of ncomp x L lines of length M), at most maxL, rounded down to a
+ * multiple of `align` (a full 128-byte row of the strided axes) when possible;
+ * the LDS line pitch is odd so that strided columns spread over the banks.  */
+static void pick_tile(const AxisPlan& p, int ncomp, int prec, int maxL, int align, int extra, int* L, int* P) {
+    const int pi = prec == 8 ? 0 : 1;
+    int l = p.cap[pi] / (ncomp * p.M);
     if (l > maxL) l = maxL;
     if (align > 1 && l >= align) l = (l / align) * align;
-    if (l < 1) l = 0;
-    *L = l; *pitch = pt;
+    const int pitch = p.M | 1;
+    const size_t lds_max = 60 * 1024;   /* + the kernels' static LDS stays within 64 KiB */
+    while (l >= 1 && fame_line_lds(p.lists[pi][0], prec, ncomp * l, pitch, extra) > lds_max) --l;
+    *L = l < 1 ? 0 : l;
+    *P = pitch;
 }
 
 /* ======================================================================== */
@@ -202,8 +251,8 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     else { g.na = Nx; g.nb = Ny; g.nc = Nz; g.ha = lat->mesh_len[0]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[2]; }
     g.N = Nx * Ny * Nz;
     g.iha = 1.0 / g.ha; g.ihb = 1.0 / g.hb; g.ihc = 1.0 / g.hc;
-    g.m_na = ((1ULL << 32) + g.na - 1) / (unsigned long long)g.na;
-    g.m_nb = ((1ULL << 32) + g.nb - 1) / (unsigned long long)g.nb;
+    g.m_na = fame_magic32(g.na);
+    g.m_nb = fame_magic32(g.nb);
     c->N = g.N;
     fame_status st;
     if ((st = c->pa.build(g.na)) != FAME_OK || (st = c->pb.build(g.nb)) != FAME_OK || (st = c->pc.build(g.nc)) != FAME_OK) {
@@ -211,9 +260,10 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     }
     for (int pi = 0; pi < 2; ++pi) {
         int prec = pi == 0 ? 8 : 4;
-        pick_tile(c->pa, 3, prec, 16, 1, &c->Lin[pi], &c->pin[pi]);
-        pick_tile(c->pb, 1, prec, 32, 4, &c->Lmid[pi], &c->pmid[pi]);
-        pick_tile(c->pc, 1, prec, 32, 4, &c->Lout[pi], &c->pout[pi]);
+        const int align = prec == 8 ? 8 : 16;
+        pick_tile(c->pa, 3, prec, MAX_LINES_INNER, 1, 0, &c->Lin[pi], &c->Pin[pi]);
+        pick_tile(c->pb, 1, prec, 64, align, g.nb, &c->Lmid[pi], &c->Pmid[pi]);
+        pick_tile(c->pc, 1, prec, 64, align, 0, &c->Lout[pi], &c->Pout[pi]);
         if (c->Lin[pi] < 1 || c->Lmid[pi] < 1 || c->Lout[pi] < 1) {
             delete c;
             FAIL(FAME_ERR_UNSUPPORTED, "FFT length too large for the LDS-resident line transforms");
@@ -352,42 +402,43 @@ extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* 
 /* ======================================================================== */
 static inline int pix(int prec) { return prec == 8 ? 0 : 1; }
 
-KInner fame_ctx::kin(int prec, int p0, const int* act) const {
+KInner fame_ctx::kin(int prec, int p0, const int* act, int list) const {
     KInner a;
     std::memset(&a, 0, sizeof(a));
-    a.g = g; a.prob = prob_d + p0; a.plan = pa.dev[pix(prec)]; a.active = act;
+    a.g = g; a.prob = prob_d + p0; a.plan = pa.lists[pix(prec)][list]; a.active = act;
     a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
     a.xstride = 2 * (size_t)N; a.fstride = 3 * (size_t)N;
-    a.L = Lin[pix(prec)]; a.pitch = pin[pix(prec)];
+    a.L = Lin[pix(prec)]; a.P = Pin[pix(prec)];
     return a;
 }
-KAxis fame_ctx::kmid(int prec, int p0, const int* act) const {
+KAxis fame_ctx::kmid(int prec, int p0, const int* act, int list) const {
     KAxis a;
     std::memset(&a, 0, sizeof(a));
-    a.g = g; a.prob = prob_d + p0; a.plan = pb.dev[pix(prec)]; a.active = act;
+    a.g = g; a.prob = prob_d + p0; a.plan = pb.lists[pix(prec)][list]; a.active = act;
     a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
-    a.fstride = 3 * (size_t)N; a.L = Lmid[pix(prec)]; a.pitch = pmid[pix(prec)];
+    a.fstride = 3 * (size_t)N; a.L = Lmid[pix(prec)]; a.P = Pmid[pix(prec)];
+    a.mL = fame_magic32(a.L);
     return a;
 }
-KAxis fame_ctx::kout(int prec, int p0, const int* act) const {
-    KAxis a = kmid(prec, p0, act);
-    a.plan = pc.dev[pix(prec)];
+KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
+    KAxis a = kmid(prec, p0, act, list);
+    a.plan = pc.lists[pix(prec)][list];
     a.invB = prec == 8 ? (const void*)invB64 : (const void*)invB32;
-    a.L = Lout[pix(prec)]; a.pitch = pout[pix(prec)];
+    a.L = Lout[pix(prec)]; a.P = Pout[pix(prec)];
+    a.mL = fame_magic32(a.L);
     return a;
 }
 
 /* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
  * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
 fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec) {
-    KInner ki = kin(prec, p0, act);
+    KInner ki = kin(prec, p0, act, LIST_INV);
     ki.x = x; ki.pikind = 0;
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
-    KAxis km = kmid(prec, p0, act);
-    HIPCHK(fame_launch_mid(km, +1, prec, nprob, stream));
-    KAxis ko = kout(prec, p0, act);
-    HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nprob, stream));
-    HIPCHK(fame_launch_mid(km, -1, prec, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
+    HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
+    ki.plan = pa.lists[pix(prec)][LIST_FWD];
     ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
     HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
     if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
@@ -396,22 +447,18 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
 
 /* field (kernel layout, in F64) = Qr/Pr x  for problems p0.. */
 fame_status fame_ctx::qr_field(const void* x, int p0, int nprob, int pikind, const int* act) {
-    KInner ki = kin(8, p0, act);
+    KInner ki = kin(8, p0, act, LIST_INV);
     ki.x = x; ki.pikind = pikind;
     HIPCHK(fame_launch_inner(ki, 1, 8, nprob, stream));
-    KAxis km = kmid(8, p0, act);
-    HIPCHK(fame_launch_mid(km, +1, 8, nprob, stream));
-    KAxis ko = kout(8, p0, act);
-    HIPCHK(fame_launch_outer(ko, OUTER_QR, 8, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_INV), +1, 8, nprob, stream));
+    HIPCHK(fame_launch_outer(kout(8, p0, act, LIST_INV), OUTER_QR, 8, nprob, stream));
     return FAME_OK;
 }
 /* y (padded reduced) = Qrs/Prs of the field in F64 (kernel layout) */
 fame_status fame_ctx::qrs_field(void* y, int p0, int nprob, int pikind, const int* act) {
-    KAxis ko = kout(8, p0, act);
-    HIPCHK(fame_launch_outer(ko, OUTER_QRS, 8, nprob, stream));
-    KAxis km = kmid(8, p0, act);
-    HIPCHK(fame_launch_mid(km, -1, 8, nprob, stream));
-    KInner ki = kin(8, p0, act);
+    HIPCHK(fame_launch_outer(kout(8, p0, act, LIST_FWD), OUTER_QRS, 8, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_FWD), -1, 8, nprob, stream));
+    KInner ki = kin(8, p0, act, LIST_FWD);
     ki.y = y; ki.pikind = pikind;
     HIPCHK(fame_launch_inner(ki, 0, 8, nprob, stream));
     return FAME_OK;
