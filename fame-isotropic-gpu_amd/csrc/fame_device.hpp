@@ -66,14 +66,19 @@ __device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulh
 enum { PRE_NONE = 0, PRE_CHIRP = 1, PRE_CHIRP_CONJ = 2, PRE_BM = 3, PRE_BP = 4 };
 enum { POST_NONE = 0, POST_CHIRP = 1, POST_CHIRP_CONJ = 2 };
 
+#define DFT_MFMA (-1)    /* StageDev.R of a direct DFT on the matrix cores       */
+#define DFT_KS_MAX 16    /* K steps of 4 -> direct DFT lengths up to 127          */
+
 struct StageDev {
-    int R;              /* radix; 0 = kernel-supplied elementwise step          */
+    int R;              /* radix; 0 = kernel-supplied elementwise step; DFT_MFMA */
     int Ns;             /* product of the radices of the earlier passes          */
     int nb;             /* butterflies per line, M / R                          */
     int tstep;          /* twiddle table step, M / (Ns R)                       */
     unsigned mNs, mNb;  /* udiv magics for / Ns and / nb                        */
     int sign;           /* -1 forward (exp(-i..)), +1 inverse                  */
     int pre;            /* PRE_*: multiplier of the pass inputs (first passes) */
+    int dn, dm, dQ, dK; /* direct DFT: length n, pairs m = (n-1)/2, outputs Q = n/2+1, K = m rounded up to 4 */
+    int dqt, pad;       /* direct DFT: 16-row output tiles ceil(Q/16) (one per wave)                          */
 };
 
 struct LinePlanDev {
@@ -83,6 +88,8 @@ struct LinePlanDev {
     int nst;            /* stages in st[]                                      */
     int post;           /* POST_*: multiplier the store applies                */
     int ntab;           /* LDS table entries: M (W) or 4M (W, chirp, bm, bp)   */
+    int wsz;            /* LDS workspace entries of the direct DFT stages (0: none) */
+    int pad;
     unsigned mM, mN;    /* udiv magics for / M and / n                         */
     const void* W;      /* M entries exp(-2 pi i m / M) in the kernel precision */
     const void* chirp;  /* M entries exp(-i pi m^2 / n) for m < n, 0 beyond     */
@@ -92,7 +99,11 @@ struct LinePlanDev {
 };
 
 template <typename T2>
-struct Tabs { const T2* W; const T2* chirp; const T2* bm; const T2* bp; };
+struct Tabs { const T2* W; const T2* chirp; const T2* bm; const T2* bp; T2* ws; };
+
+/* LDS layout of a line-transform block: tables (ntab), direct-DFT workspace
+ * (wsz), then the kernel's own entries and lines */
+__host__ __device__ inline int lds_head(const LinePlanDev& pl) { return pl.ntab + pl.wsz; }
 
 /* copy a plan's tables into LDS at `base` (caller synchronises) */
 template <typename T2>
@@ -107,6 +118,7 @@ __device__ __forceinline__ Tabs<T2> load_tabs(const LinePlanDev& pl, T2* base) {
     Tabs<T2> tb;
     tb.W = base;
     tb.chirp = base + M; tb.bm = base + 2 * M; tb.bp = base + 3 * M;
+    tb.ws = base + pl.ntab;
     return tb;
 }
 
@@ -289,27 +301,162 @@ __device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& s
     __syncthreads();
 }
 
+/* ------------------------------------------------------------------------- */
+/*  Direct DFT on the matrix cores                                            */
+/* ------------------------------------------------------------------------- */
+/* For a line x of length n (< 128), with s_t = x_t + x_{n-t}, d_t = x_t - x_{n-t}
+ * (t = 1..m, m = (n-1)/2) and h = x_{n/2} for even n,
+ *   X_q     = x_0 + (-1)^q h + A_q + sign i B_q,
+ *   X_{n-q} = x_0 + (-1)^q h + A_q - sign i B_q,    q = 0..n/2,
+ *   A_q = sum_t cos(2 pi q t / n) s_t,   B_q = sum_t sin(2 pi q t / n) d_t.
+ * A and B are two real matrix products (Q x m) x (m x 2 columns per line) run
+ * as 16x16x4 MFMAs: wave w owns output rows q = 16w..16w+15 and keeps its cos /
+ * sin operands in registers; the s / d columns of 16 lines at a time are staged
+ * in the LDS workspace.  About a quarter of the flops of the complex DFT-GEMM. */
+template <typename T> struct Mfma;
+/* Operands of both 16x16x4 variants: lane l holds A[l % 16][l / 16] and
+ * B[l / 16][l % 16].  Result row of accumulator element v in lane l (column
+ * l % 16): 4 (l / 16) + v for f32, (l / 16) + 4 v for f64 (checked on gfx950). */
+template <> struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kq, int v) { return 4 * kq + v; }
+};
+template <> struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ acc_t mac(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int kq, int v) { return kq + 4 * v; }
+};
+
+/* workspace entries (in complex units) of a direct DFT with K rows */
+__host__ __device__ inline int dft_ws_entries(int K) { return 2 * K * 16 + 32; }
+
+template <typename T2>
+__device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb) {
+    typedef decltype(S[0].x) T;
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int n = st.dn, m = st.dm, Q = st.dQ, K = st.dK, nqt = st.dqt;
+    const int nks = K >> 2;
+    const bool inv = st.sign > 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    /* A operands: lane holds row q = 16w + row, column k = 4 ks + kq (t = k + 1);
+     * r = q t mod n advances by 4q mod n per k step and indexes the twiddle table */
+    const int q = w * 16 + row;
+    const bool qon = w < nqt && q < Q;
+    const int dr = (4 * q) % n;
+    const int r0 = (q * (kq + 1)) % n;
+    T* SPr = reinterpret_cast<T*>(tb.ws);
+    T* SPi = SPr + K * 16;
+    T* SMr = SPr + 2 * K * 16;
+    T* SMi = SPr + 3 * K * 16;
+    T2* xh = reinterpret_cast<T2*>(SPr + 4 * K * 16);    /* x_0 [16], h [16] */
+    const bool even = (n & 1) == 0;
+    const int ngr = (nl + 15) >> 4;
+    for (int gi = 0; gi < ngr; ++gi) {
+        for (int idx = threadIdx.x; idx < K * 16; idx += FFT_THREADS) {
+            const int k = idx >> 4, j = idx & 15, line = gi * 16 + j, t = k + 1;
+            T2 sp, sm;
+            sp.x = 0; sp.y = 0; sm.x = 0; sm.y = 0;
+            if (line < nl && t <= m) {
+                const T2 a = S[line * P + t], b = S[line * P + n - t];
+                sp = cadd(a, b); sm = csub(a, b);
+            }
+            SPr[idx] = sp.x; SPi[idx] = sp.y; SMr[idx] = sm.x; SMi[idx] = sm.y;
+        }
+        if (threadIdx.x < 16) {
+            const int line = gi * 16 + threadIdx.x;
+            T2 x0, h;
+            x0.x = 0; x0.y = 0; h.x = 0; h.y = 0;
+            if (line < nl) {
+                x0 = S[line * P];
+                if (even) h = S[line * P + (n >> 1)];
+            }
+            xh[threadIdx.x] = x0; xh[16 + threadIdx.x] = h;
+        }
+        __syncthreads();
+        if (w < nqt) {
+            acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+            int r = r0;
+            for (int ks = 0; ks < nks; ++ks) {
+                const int t = ks * 4 + kq + 1;
+                T a_c = 0, a_s = 0;
+                if (qon && t <= m) {
+                    const T2 wv = tb.W[r];      /* exp(-2 pi i r / n) = cos - i sin */
+                    a_c = wv.x; a_s = -wv.y;
+                }
+                r += dr;
+                if (r >= n) r -= n;
+                /* B operand: lane holds row k = 4 ks + kq, column (line) row */
+                const int o = (ks * 4 + kq) * 16 + row;
+                cr = Mfma<T>::mac(a_c, SPr[o], cr);
+                ci = Mfma<T>::mac(a_c, SPi[o], ci);
+                sr = Mfma<T>::mac(a_s, SMr[o], sr);
+                si = Mfma<T>::mac(a_s, SMi[o], si);
+            }
+            /* result: lane holds rows q = 16w + Mfma::row(kq, v), column (line) row */
+            const int line = gi * 16 + row;
+            if (line < nl) {
+                const T2 x0 = xh[row], h = xh[16 + row];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int qq = w * 16 + Mfma<T>::row(kq, v);
+                    if (qq < Q) {
+                        const T Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
+                        const T Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
+                        const T Bx = sr[v], By = si[v];
+                        T2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
+                        if (!inv) { lo.x = Ax + By; lo.y = Ay - Bx; hi.x = Ax - By; hi.y = Ay + Bx; }
+                        else      { lo.x = Ax - By; lo.y = Ay + Bx; hi.x = Ax + By; hi.y = Ay - Bx; }
+                        S[line * P + qq] = lo;
+                        if (qq != 0 && 2 * qq != n) S[line * P + n - qq] = hi;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 struct NoMid { template <typename T2> __device__ void operator()(T2*) const {} };
 
-/* run the whole stage list; `mid` handles R = 0 stages and must end with a barrier */
-template <typename T2, int MAXE, typename Mid>
+/* run the whole stage list; `mid` handles R = 0 stages and must end with a
+ * barrier.  ENG selects the stage kinds compiled in (a plan's lists are all
+ * direct-DFT or all Stockham), so that each kernel variant is register-
+ * allocated for its own engine only.                                        */
+enum { ENG_STOCKHAM = 0, ENG_MFMA = 1 };
+template <typename T2, int MAXE, int ENG, typename Mid>
 __device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanDev& pl, const Tabs<T2>& tb,
                                            const Mid& mid) {
     for (int s = 0; s < pl.nst; ++s) {
         const StageDev& st = pl.st[s];
-        switch (st.R) {
-            case 0: mid(S); break;
-            case 2: fft_pass<T2, 2, MAXE>(S, nl, P, st, tb); break;
-            case 3: fft_pass<T2, 3, MAXE>(S, nl, P, st, tb); break;
-            case 4: fft_pass<T2, 4, MAXE>(S, nl, P, st, tb); break;
-            case 5: fft_pass<T2, 5, MAXE>(S, nl, P, st, tb); break;
-            case 7: fft_pass<T2, 7, MAXE>(S, nl, P, st, tb); break;
-            case 8: fft_pass<T2, 8, MAXE>(S, nl, P, st, tb); break;
-            case 11: fft_pass<T2, 11, MAXE>(S, nl, P, st, tb); break;
-            case 13: fft_pass<T2, 13, MAXE>(S, nl, P, st, tb); break;
-            default: break;
+        if (st.R == 0) { mid(S); continue; }
+        if constexpr (ENG == ENG_MFMA) {
+            dft_mfma<T2>(S, nl, P, st, tb);
+        } else {
+            switch (st.R) {
+                case 2: fft_pass<T2, 2, MAXE>(S, nl, P, st, tb); break;
+                case 3: fft_pass<T2, 3, MAXE>(S, nl, P, st, tb); break;
+                case 4: fft_pass<T2, 4, MAXE>(S, nl, P, st, tb); break;
+                case 5: fft_pass<T2, 5, MAXE>(S, nl, P, st, tb); break;
+                case 7: fft_pass<T2, 7, MAXE>(S, nl, P, st, tb); break;
+                case 8: fft_pass<T2, 8, MAXE>(S, nl, P, st, tb); break;
+                case 11: fft_pass<T2, 11, MAXE>(S, nl, P, st, tb); break;
+                case 13: fft_pass<T2, 13, MAXE>(S, nl, P, st, tb); break;
+                default: break;
+            }
         }
     }
+}
+
+/* engine of a stage list (host and device) */
+__host__ __device__ inline int plan_engine(const LinePlanDev& pl) {
+    for (int s = 0; s < pl.nst; ++s) if (pl.st[s].R == DFT_MFMA) return ENG_MFMA;
+    return ENG_STOCKHAM;
 }
 
 /* register budget (complex elements per thread and pass) of each precision */

@@ -34,6 +34,7 @@ std::string& fame_err_ref();
  * (one stage list: forward transform, the curl-curl step, inverse transform;
  * with Bluestein the forward post-chirp and inverse pre-chirp cancel and only
  * 1/M remains in the step) */
+template <int ENG>
 __global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
     typedef double2 T2;
     const int p = blockIdx.y;
@@ -44,7 +45,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
     const int na = g.na, L = args.L, P = args.P;
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
-    T2* S = base + pl.ntab;
+    T2* S = base + lds_head(pl);
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
     const size_t N = (size_t)g.N;
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
         }
         __syncthreads();
     };
-    run_stages<T2, MaxE<double>::v>(S, 3 * L, P, pl, tb, curlcurl);
+    run_stages<T2, MaxE<double>::v, ENG>(S, 3 * L, P, pl, tb, curlcurl);
     for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
         const int line = udiv(idx, g.m_na), a = idx - line * na, gl = l0 + line;
         if (gl >= nlines) continue;
@@ -176,7 +177,8 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
             fame_launch_mid(c->kmid(8, 0, nullptr, LIST_FWD), -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
         dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, nk);
         const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
-        hipLaunchKernelGGL(k_curlcurl, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        if (plan_engine(ki.plan) == ENG_MFMA) hipLaunchKernelGGL(k_curlcurl<ENG_MFMA>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        else hipLaunchKernelGGL(k_curlcurl<ENG_STOCKHAM>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
         if (hipGetLastError() != hipSuccess ||
             fame_launch_mid(c->kmid(8, 0, nullptr, LIST_INV), +1, 8, nk, c->stream) != hipSuccess ||
             fame_launch_outer(c->kout(8, 0, nullptr, LIST_INV), OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }

@@ -9,13 +9,15 @@
 
 #define NBLK_VEC 1024   /* blocks per problem of the vector/reduction kernels */
 
+enum { FFT_ENGINE_AUTO = 0, FFT_ENGINE_STOCKHAM = 1, FFT_ENGINE_MFMA = 2 };
+
 struct AxisPlan {
-    int n = 0, M = 0, blue = 0;
+    int n = 0, M = 0, blue = 0, mfma = 0;
     std::vector<int> radices;
     fame::LinePlanDev lists[2][NLIST];   /* [0] FP64, [1] FP32; LIST_* stage lists */
     int cap[2] = {0, 0};                 /* elements a 256-thread block transforms at once */
     std::vector<void*> bufs;
-    fame_status build(int n);
+    fame_status build(int n, int engine);
     ~AxisPlan();
 };
 

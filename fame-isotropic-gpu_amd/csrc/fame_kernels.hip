@@ -48,7 +48,7 @@ template <> __device__ __forceinline__ float2 tw_expipi<float2>(double x) {
  * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
  *              (FAME_Matrix_Lambdas.cu:217-243)
  * A block owns L lines (b, c) and their three components: 3L LDS lines.      */
-template <typename T, int EXPAND>
+template <typename T, int EXPAND, int ENG>
 __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y;
@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     const int na = g.na, L = args.L, P = args.P;
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
-    T2* S = base + pl.ntab;
+    T2* S = base + lds_head(pl);
     const int nlines = g.nb * g.nc;
     const int l0 = blockIdx.x * L;
     if ((int)threadIdx.x < L) {
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
         }
         __syncthreads();
-        run_stages<T2, MaxE<T>::v>(S, 3 * L, P, pl, tb, NoMid());
+        run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
             const int line = udiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = v[c];
         }
         __syncthreads();
-        run_stages<T2, MaxE<T>::v>(S, 3 * L, P, pl, tb, NoMid());
+        run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0;
@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
 /* DIR = +1: F = twid_b .* IFFT_b(F) ; DIR = -1: F = FFT_b(conj(twid_b) .* F).
  * A block owns L consecutive a of one (problem, component, c) plane; the
  * middle twiddle depends on (c, b) only and is tabulated once per block.     */
-template <typename T, int DIR>
+template <typename T, int DIR, int ENG>
 __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     const int na = g.na, nb = g.nb, L = args.L, P = args.P;
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
-    T2* twb = base + pl.ntab;
+    T2* twb = base + lds_head(pl);
     T2* S = twb + nb;
     const int ntile = (na + L - 1) / L;
     const int cf = blockIdx.x / ntile, a0 = (blockIdx.x - cf * ntile) * L;
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
         S[ta * P + b] = v;
     }
     __syncthreads();
-    run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, NoMid());
+    run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
     for (int idx = threadIdx.x; idx < L * nb; idx += FFT_THREADS) {
         const int b = udiv(idx, args.mL), ta = idx - b * L, a = a0 + ta;
         if (a >= na) continue;
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
  * In EPS mode with Bluestein the inverse post-chirp and the forward pre-chirp
  * are conjugate unit factors whose product is 1; both are left out and only
  * the 1/M remains in the elementwise step.                                  */
-template <typename T, int MODE>
+template <typename T, int MODE, int ENG>
 __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     const int plane = g.na * g.nb;
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
-    T2* S = base + pl.ntab;
+    T2* S = base + lds_head(pl);
     const int q0 = blockIdx.x * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
     const int Wd = pl.blue ? pl.M : nc;
@@ -283,9 +283,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
             }
             __syncthreads();
         };
-        run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, eps_step);
+        run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
     } else {
-        run_stages<T2, MaxE<T>::v>(S, L, P, pl, tb, NoMid());
+        run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
     }
     const double invN = 1.0 / (double)g.N;
     for (int idx = threadIdx.x; idx < L * nc; idx += FFT_THREADS) {
@@ -382,50 +382,56 @@ __global__ void k_sigma(GridDev g, const ProbDev* prob, double* sig, double* sig
 using namespace fame;
 
 size_t fame_line_lds(const LinePlanDev& pl, int prec, int nl, int P, int extra) {
-    return (size_t)(prec == 8 ? 16 : 8) * ((size_t)pl.ntab + (size_t)nl * P + (size_t)extra);
+    return (size_t)(prec == 8 ? 16 : 8) * ((size_t)lds_head(pl) + (size_t)nl * P + (size_t)extra);
 }
 
-template <typename T>
+template <typename T, int ENG>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
     dim3 grid((nlines + a.L - 1) / a.L, nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, 0);
-    if (expand) hipLaunchKernelGGL((k_inner<T, 1>), grid, dim3(FFT_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_inner<T, 0>), grid, dim3(FFT_THREADS), lds, s, a);
+    if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_inner<T, 0, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, int ENG>
 static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) {
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(ntile * a.g.nc, 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
-    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1>), grid, dim3(FFT_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_mid<T, -1>), grid, dim3(FFT_THREADS), lds, s, a);
+    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
+    else hipLaunchKernelGGL((k_mid<T, -1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, int ENG>
 static hipError_t launch_outer(const KAxis& a, int mode, int nprob, hipStream_t s) {
     const int plane = a.g.na * a.g.nb;
     dim3 grid((plane + a.L - 1) / a.L, 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, 0);
     switch (mode) {
-        case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS>), grid, dim3(FFT_THREADS), lds, s, a); break;
-        case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR>), grid, dim3(FFT_THREADS), lds, s, a); break;
-        default: hipLaunchKernelGGL((k_outer<T, OUTER_QRS>), grid, dim3(FFT_THREADS), lds, s, a); break;
+        case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS, ENG>), grid, dim3(FFT_THREADS), lds, s, a); break;
+        case OUTER_QR: hipLaunchKernelGGL((k_outer<T, OUTER_QR, ENG>), grid, dim3(FFT_THREADS), lds, s, a); break;
+        default: hipLaunchKernelGGL((k_outer<T, OUTER_QRS, ENG>), grid, dim3(FFT_THREADS), lds, s, a); break;
     }
     return hipGetLastError();
 }
 
 hipError_t fame_launch_inner(const KInner& a, int expand, int prec, int nprob, hipStream_t s) {
-    return prec == 8 ? launch_inner<double>(a, expand, nprob, s) : launch_inner<float>(a, expand, nprob, s);
+    if (plan_engine(a.plan) == ENG_MFMA)
+        return prec == 8 ? launch_inner<double, ENG_MFMA>(a, expand, nprob, s) : launch_inner<float, ENG_MFMA>(a, expand, nprob, s);
+    return prec == 8 ? launch_inner<double, ENG_STOCKHAM>(a, expand, nprob, s) : launch_inner<float, ENG_STOCKHAM>(a, expand, nprob, s);
 }
 hipError_t fame_launch_mid(const KAxis& a, int dir, int prec, int nprob, hipStream_t s) {
-    return prec == 8 ? launch_mid<double>(a, dir, nprob, s) : launch_mid<float>(a, dir, nprob, s);
+    if (plan_engine(a.plan) == ENG_MFMA)
+        return prec == 8 ? launch_mid<double, ENG_MFMA>(a, dir, nprob, s) : launch_mid<float, ENG_MFMA>(a, dir, nprob, s);
+    return prec == 8 ? launch_mid<double, ENG_STOCKHAM>(a, dir, nprob, s) : launch_mid<float, ENG_STOCKHAM>(a, dir, nprob, s);
 }
 hipError_t fame_launch_outer(const KAxis& a, int mode, int prec, int nprob, hipStream_t s) {
-    return prec == 8 ? launch_outer<double>(a, mode, nprob, s) : launch_outer<float>(a, mode, nprob, s);
+    if (plan_engine(a.plan) == ENG_MFMA)
+        return prec == 8 ? launch_outer<double, ENG_MFMA>(a, mode, nprob, s) : launch_outer<float, ENG_MFMA>(a, mode, nprob, s);
+    return prec == 8 ? launch_outer<double, ENG_STOCKHAM>(a, mode, nprob, s) : launch_outer<float, ENG_STOCKHAM>(a, mode, nprob, s);
 }
 
 hipError_t fame_launch_xz_transpose(const void* in, void* out, int nx, int ny, int nz, int dir, size_t stride,

@@ -318,30 +318,35 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
     return FAME_OK;
 }
 
-/* E_i = invB .* Qr(Sigma .* ev_i), times ||E_i|| (reference scaling), Gamma rule
- * (FAME_Fast_Algorithms_Isotropic.cu:61-125).  ev: padded reduced vectors with
- * per-problem stride ev_pstride and per-band stride 2N.                       */
-fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev, size_t ev_pstride, void* E_dev) {
+/* E_i = invB .* Qr(Sigma .* ev_i), times ||E_i|| (reference scaling), for band i
+ * of every problem (FAME_Fast_Algorithms_Isotropic.cu:90-125).  ev: padded
+ * reduced vectors of band i with per-problem stride ev_pstride.              */
+static fame_status restore_band(fame_ctx* c, int i, int nwant, const double2* ev, size_t ev_pstride, void* E_dev) {
     const size_t N = c->N, n2 = 2 * N, N3 = 3 * N;
     const int nk = c->nk;
     double2* y = (double2*)c->ref64[0];
     std::vector<double> nr(nk);
-    for (int i = 0; i < nwant; ++i) {
-        HIPCHK(fame_convert(y, 8, ev + (size_t)i * n2, 8, n2, n2, ev_pstride, nk, c->stream));
-        HIPCHK(fame_scale_real_mod(y, y, c->sig, n2, n2, N, N, nk, 8, c->stream));
-        CHK(c->qr_field(y, 0, nk, 0, nullptr));
-        HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->invB64, N3, N3, 0, N3, nk, 8, c->stream));
-        HIPCHK(fame_norm2_partials(c->F64, 8, c->partials, NBLK_VEC, N3, N3, nk, c->stream));
-        HIPCHK(fame_reduce_partials(c->partials, NBLK_VEC, 1, c->scal, nk, c->stream));
-        HIPCHK(hipMemcpyAsync(nr.data(), c->scal, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        for (int p = 0; p < nk; ++p) nr[p] = std::sqrt(nr[p]);   /* Zdscal by the norm itself (:119-121) */
-        HIPCHK(hipMemcpyAsync(c->scal, nr.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->scal, N3, N3, 1, 1, nk, 8, c->stream));
-        for (int p = 0; p < nk; ++p)
-            CHK(c->field_to_ref(c->F64 + (size_t)p * N3, (double2*)E_dev + ((size_t)p * nwant + i) * N3, 1, N3));
-    }
-    for (int p = 0; p < nk; ++p) {
+    HIPCHK(fame_convert(y, 8, ev, 8, n2, n2, ev_pstride, nk, c->stream));
+    HIPCHK(fame_scale_real_mod(y, y, c->sig, n2, n2, N, N, nk, 8, c->stream));
+    CHK(c->qr_field(y, 0, nk, 0, nullptr));
+    HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->invB64, N3, N3, 0, N3, nk, 8, c->stream));
+    HIPCHK(fame_norm2_partials(c->F64, 8, c->partials, NBLK_VEC, N3, N3, nk, c->stream));
+    HIPCHK(fame_reduce_partials(c->partials, NBLK_VEC, 1, c->scal, nk, c->stream));
+    HIPCHK(hipMemcpyAsync(nr.data(), c->scal, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int p = 0; p < nk; ++p) nr[p] = std::sqrt(nr[p]);   /* Zdscal by the norm itself (:119-121) */
+    HIPCHK(hipMemcpyAsync(c->scal, nr.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->scal, N3, N3, 1, 1, nk, 8, c->stream));
+    for (int p = 0; p < nk; ++p)
+        CHK(c->field_to_ref(c->F64 + (size_t)p * N3, (double2*)E_dev + ((size_t)p * nwant + i) * N3, 1, N3));
+    return FAME_OK;
+}
+
+/* Gamma rule on the fields: bands move up by two, E_0 = E_1 = 1/sqrt(3N)
+ * (FAME_Fast_Algorithms_Isotropic.cu:71-79) */
+static fame_status restore_gamma(fame_ctx* c, int nwant, void* E_dev) {
+    const size_t N3 = 3 * (size_t)c->N;
+    for (int p = 0; p < c->nk; ++p) {
         if (!c->prob_h[p].gamma) continue;
         double2* E = (double2*)E_dev + (size_t)p * nwant * N3;
         for (int i = nwant - 1; i >= 2; --i)
@@ -354,3 +359,25 @@ fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev, size_t 
     return FAME_OK;
 }
 
+/* ev: padded reduced vectors with per-problem stride ev_pstride and per-band stride 2N */
+fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev, size_t ev_pstride, void* E_dev) {
+    for (int i = 0; i < nwant; ++i) CHK(restore_band(c, i, nwant, ev + (size_t)i * 2 * c->N, ev_pstride, E_dev));
+    return restore_gamma(c, nwant, E_dev);
+}
+
+extern "C" fame_status fame_restore(fame_ctx* c, int nwant, const void* evr_dev, void* E_dev) {
+    if (!c || !evr_dev || !E_dev || nwant < 1) FAIL(FAME_ERR_ARG, "fame_restore: bad arguments");
+    if (!c->have_eps || c->nk < 1) FAIL(FAME_ERR_STATE, "fame_restore: permittivity and wave vectors must be set");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n2 = 2 * (size_t)c->N;
+    double2* stage = (double2*)c->ref64[2];
+    for (int i = 0; i < nwant; ++i) {
+        for (int p = 0; p < c->nk; ++p)
+            HIPCHK(fame_launch_pack((const double2*)evr_dev + ((size_t)p * nwant + i) * n2, stage + (size_t)p * n2, c->N,
+                                    c->nd[p], +1, 8, c->stream));
+        CHK(restore_band(c, i, nwant, stage, n2, E_dev));
+    }
+    CHK(restore_gamma(c, nwant, E_dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FAME_OK;
+}

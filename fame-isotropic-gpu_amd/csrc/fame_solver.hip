@@ -23,6 +23,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -94,6 +95,17 @@ static bool add_transform(LinePlanDev& d, const std::vector<int>& rad, int M, in
     return passes(-1, prechirp ? (sign < 0 ? PRE_CHIRP : PRE_CHIRP_CONJ) : PRE_NONE) &&
            passes(+1, sign < 0 ? PRE_BM : PRE_BP);
 }
+/* append one direct DFT (matrix cores) of length n */
+static bool add_direct(LinePlanDev& d, int n, int sign) {
+    if (d.nst >= FAME_MAX_STAGES) return false;
+    StageDev& s = d.st[d.nst++];
+    std::memset(&s, 0, sizeof(s));
+    s.R = DFT_MFMA; s.sign = sign;
+    s.dn = n; s.dm = (n - 1) / 2; s.dQ = n / 2 + 1;
+    s.dK = std::max(4, (s.dm + 3) / 4 * 4);
+    s.dqt = (s.dQ + 15) / 16;
+    return true;
+}
 static bool add_mid(LinePlanDev& d) {
     if (d.nst >= FAME_MAX_STAGES) return false;
     StageDev& s = d.st[d.nst++];
@@ -101,19 +113,24 @@ static bool add_mid(LinePlanDev& d) {
     return true;
 }
 
-fame_status AxisPlan::build(int n_) {
+fame_status AxisPlan::build(int n_, int engine) {
     n = n_;
+    /* lengths 16..127: direct DFT on the matrix cores (any n, no Bluestein);
+     * otherwise Stockham passes, or Bluestein when n has a prime factor > 13
+     * or the chirp-z transform is cheaper */
+    mfma = (engine == FFT_ENGINE_MFMA && n >= 2 && n <= 127) || (engine == FFT_ENGINE_AUTO && n >= 16 && n <= 127);
     std::vector<int> rad;
     double direct = 1e30;
     if (factor_gpu(n, &rad)) direct = n * plan_cost(rad);
     int bestM = 0; std::vector<int> bestR; double bestC = 1e30;
-    for (int m = 2 * n - 1; m <= 4 * n + 8; ++m) {
+    for (int m = 2 * n - 1; m <= 4 * n + 8 && !mfma; ++m) {
         std::vector<int> r2;
         if (!factor_gpu(m, &r2)) continue;
         double c = 2.0 * m * plan_cost(r2) + 12.0 * m;
         if (c < bestC) { bestC = c; bestM = m; bestR = r2; }
     }
-    if (direct <= bestC && direct < 1e29) { blue = 0; M = n; radices = rad; }
+    if (mfma) { blue = 0; M = n; radices.clear(); }
+    else if (direct <= bestC && direct < 1e29) { blue = 0; M = n; radices = rad; }
     else if (bestM) { blue = 1; M = bestM; radices = bestR; }
     else FAIL(FAME_ERR_UNSUPPORTED, "no FFT plan for length " + std::to_string(n));
     /* tables in long double, rounded once to each precision */
@@ -167,13 +184,24 @@ fame_status AxisPlan::build(int n_) {
         std::memset(&base, 0, sizeof(base));
         base.n = n; base.M = M; base.blue = blue;
         base.ntab = blue ? 4 * M : M;
+        base.wsz = mfma ? dft_ws_entries(std::max(4, ((n - 1) / 2 + 3) / 4 * 4)) : 0;
         base.mM = fame_magic32(M); base.mN = fame_magic32(n);
         CHK(upload(W, prec, &base.W));
         CHK(upload(ch, prec, &base.chirp));
         CHK(upload(bm, prec, &base.bm));
         CHK(upload(bp, prec, &base.bp));
         bool ok = true;
-        for (int l = 0; l < NLIST; ++l) {
+        for (int l = 0; l < NLIST && mfma; ++l) {
+            LinePlanDev& d = lists[pi][l];
+            d = base;
+            switch (l) {
+                case LIST_FWD: ok = ok && add_direct(d, n, -1); break;
+                case LIST_INV: ok = ok && add_direct(d, n, +1); break;
+                case LIST_EPS: ok = ok && add_direct(d, n, +1) && add_mid(d) && add_direct(d, n, -1); break;
+                default: ok = ok && add_direct(d, n, -1) && add_mid(d) && add_direct(d, n, +1); break;
+            }
+        }
+        for (int l = 0; l < NLIST && !mfma; ++l) {
             LinePlanDev& d = lists[pi][l];
             d = base;
             switch (l) {
@@ -192,9 +220,9 @@ fame_status AxisPlan::build(int n_) {
         if (!ok) FAIL(FAME_ERR_UNSUPPORTED, "too many FFT passes for length " + std::to_string(n));
         /* register budget per thread: 8 (FP64) / 16 (FP32) complex elements */
         const int maxe = prec == 8 ? MaxE<double>::v : MaxE<float>::v;
-        int c = 1 << 30;
+        int c = 1 << 12;
         for (int r : radices) c = std::min(c, r * nbt_of(r, maxe));
-        cap[pi] = FFT_THREADS * c;
+        cap[pi] = FFT_THREADS * c;     /* direct DFT stages hold no lines in registers */
     }
     return FAME_OK;
 }
@@ -255,15 +283,22 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     g.m_nb = fame_magic32(g.nb);
     c->N = g.N;
     fame_status st;
-    if ((st = c->pa.build(g.na)) != FAME_OK || (st = c->pb.build(g.nb)) != FAME_OK || (st = c->pc.build(g.nc)) != FAME_OK) {
+    /* FAME_FFT=stockham|mfma|auto (default auto) selects the line-transform engine */
+    int engine = FFT_ENGINE_AUTO;
+    if (const char* e = std::getenv("FAME_FFT")) {
+        if (!std::strcmp(e, "stockham")) engine = FFT_ENGINE_STOCKHAM;
+        else if (!std::strcmp(e, "mfma")) engine = FFT_ENGINE_MFMA;
+    }
+    if ((st = c->pa.build(g.na, engine)) != FAME_OK || (st = c->pb.build(g.nb, engine)) != FAME_OK ||
+        (st = c->pc.build(g.nc, engine)) != FAME_OK) {
         delete c; return st;
     }
     for (int pi = 0; pi < 2; ++pi) {
         int prec = pi == 0 ? 8 : 4;
         const int align = prec == 8 ? 8 : 16;
         pick_tile(c->pa, 3, prec, MAX_LINES_INNER, 1, 0, &c->Lin[pi], &c->Pin[pi]);
-        pick_tile(c->pb, 1, prec, 64, align, g.nb, &c->Lmid[pi], &c->Pmid[pi]);
-        pick_tile(c->pc, 1, prec, 64, align, 0, &c->Lout[pi], &c->Pout[pi]);
+        pick_tile(c->pb, 1, prec, c->pb.mfma ? 32 : 64, align, g.nb, &c->Lmid[pi], &c->Pmid[pi]);
+        pick_tile(c->pc, 1, prec, c->pc.mfma ? 32 : 64, align, 0, &c->Lout[pi], &c->Pout[pi]);
         if (c->Lin[pi] < 1 || c->Lmid[pi] < 1 || c->Lout[pi] < 1) {
             delete c;
             FAIL(FAME_ERR_UNSUPPORTED, "FFT length too large for the LDS-resident line transforms");

@@ -305,6 +305,7 @@ class Solver:
         nd = np.zeros(kk.shape[0], np.int32)
         _check(lib().fame_set_wave_vectors(self._ctx, kk.shape[0], _ptr(kk), _ptr(nd)), "fame_set_wave_vectors")
         self.nd = [int(v) for v in nd]
+        self.k = kk.copy()
         return self.nd
 
     def fft_lengths(self):
@@ -348,6 +349,15 @@ class Solver:
                       qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time) for p in prof]
         return freq, evr, E, profs
 
+    def restore(self, evr):
+        """E fields (nk, nwant, 3N) recomputed from reduced Ritz vectors (nk, nwant, 2N)."""
+        torch = _torch()
+        nk, nwant, _ = evr.shape
+        E = torch.empty((nk, nwant, 3 * self.N), dtype=torch.complex128, device=evr.device)
+        _check(lib().fame_restore(self._ctx, nwant, C.c_void_p(evr.contiguous().data_ptr()), C.c_void_p(E.data_ptr())),
+               "fame_restore")
+        return E
+
     def residuals(self, freq, E):
         nk, nwant = freq.shape
         res = np.zeros((nk, nwant))
@@ -373,3 +383,66 @@ class Solver:
         return g
 
 OP_INVAR_MIXED, OP_QBQ32 = 6, 7
+
+
+# --------------------------------------------------------------------------- adjoint
+def _make_eig_function():
+    torch = _torch()
+
+    class BandEigenvalues(torch.autograd.Function):
+        """lambda = omega^2 of the `nwant` lowest bands at every wave vector of
+        `solver`, as a differentiable function of the permittivity on the Yee
+        points (3N, reference layout).  The backward pass is the Hellmann-Feynman
+        adjoint d lambda / d eps_e = -lambda |E_e|^2 / (E^H B E) evaluated on the
+        GPU through fame_eig_grad; inside a degenerate cluster it returns the
+        derivative of the cluster mean (the only basis-independent quantity)."""
+
+        @staticmethod
+        def forward(ctx, eps, solver, nwant, nstep, tol, precision):
+            e = eps.detach().to("cpu", torch.float64).numpy().copy()
+            solver.set_eps(e)
+            freq, evr, _, prof = solver.eigs(nwant, nstep, tol=tol, precision=precision, want_reduced=True)
+            if not all(p["converged"] for p in prof):
+                raise FameError("eigensolve did not converge")
+            ctx.solver, ctx.freq, ctx.evr = solver, freq, evr
+            ctx.eps_np, ctx.k = e, solver.k.copy()
+            ctx.eps_device = eps.device
+            return torch.as_tensor(freq ** 2, dtype=torch.float64, device=eps.device)
+
+        @staticmethod
+        def backward(ctx, grad_lam):
+            S = ctx.solver
+            # the fields are recomputed from the reduced eigenvectors (not stored)
+            S.set_eps(ctx.eps_np)
+            if not np.array_equal(S.k, ctx.k):
+                S.set_wave_vectors(ctx.k)
+            E = S.restore(ctx.evr)
+            w = grad_lam.detach().to("cpu", torch.float64).numpy()
+            g = S.eig_vjp(ctx.freq, E, w)
+            del E
+            return g.to(ctx.eps_device), None, None, None, None, None
+
+    return BandEigenvalues
+
+
+_EIG_FN = None
+
+
+def band_eigenvalues(solver: "Solver", eps, nwant: int, nstep: int, tol: float = 1e-12, precision: int = PREC_F64):
+    """Differentiable lambda[k, band] (torch float64) of eps (torch float64, 3N)."""
+    global _EIG_FN
+    if _EIG_FN is None:
+        _EIG_FN = _make_eig_function()
+    return _EIG_FN.apply(eps, solver, nwant, nstep, tol, precision)
+
+
+def cell_to_yee(rho_eps):
+    """Permittivity on the Yee edges from a periodic per-cell field (torch,
+    shape (Nz, Ny, Nx), x fastest like the reference's real-space layout): the
+    edge along axis d between cells r and r + e_d takes the mean of the two.
+    Returns 3N entries [x-edges; y-edges; z-edges]; differentiable."""
+    torch = _torch()
+    ex = 0.5 * (rho_eps + torch.roll(rho_eps, -1, dims=2))
+    ey = 0.5 * (rho_eps + torch.roll(rho_eps, -1, dims=1))
+    ez = 0.5 * (rho_eps + torch.roll(rho_eps, -1, dims=0))
+    return torch.cat([ex.reshape(-1), ey.reshape(-1), ez.reshape(-1)])

@@ -117,6 +117,14 @@ fame_status fame_cg(fame_ctx* ctx, int ik, const void* rhs_dev, void* x_dev, int
 fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                       void* evr_dev, void* E_dev, fame_profile* prof);
 
+/* Restored E fields (device, nk*nwant*3N, reference scaling and Gamma rule, as
+ * fame_eigs' E_dev) recomputed from the reduced Ritz vectors evr_dev (device,
+ * nk*nwant*2N, as fame_eigs' evr_dev) with the permittivity and wave vectors
+ * currently set.  Replaces Eigen_Restoration_Isotropic
+ * (FAME_Main/FAME_Fast_Algorithms_Isotropic.cu:90) as a separate step, so that a
+ * caller (the adjoint) keeps 2N instead of 3N complex values per band.      */
+fame_status fame_restore(fame_ctx* ctx, int nwant, const void* evr_dev, void* E_dev);
+
 /* Relative residuals ||C^H C E - omega^2 B E|| / ||E|| of fields produced by
  * fame_eigs (device, nk*nwant*3N) -> res (host, nk*nwant).  GPU counterpart of
  * Check_Residual (FAME_Main/FAME_Main_Code.cu:383), reported relative.       */

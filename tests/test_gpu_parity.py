@@ -5,7 +5,12 @@ restarted Lanczos eigensolve (FP64 and FP32+FP64-refinement), for Simple (SC)
 and General (FCC, BCC, hexagonal) lattices, at a generic k and at Gamma.
 Bars: operators 1e-11 relative (FP64 FFT rounding), eigenvalues 1e-8 relative
 residual ||A E - lambda B E||/||E|| small.
+Both line-transform engines are covered: Stockham/Bluestein passes and the
+direct DFT on the matrix cores (FAME_FFT selects the engine at context
+creation; the default picks the matrix cores for axis lengths 16..127).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -27,9 +32,20 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
 
 
+@pytest.fixture(params=["stockham", "mfma"])
+def engine(request):
+    old = os.environ.get("FAME_FFT")
+    os.environ["FAME_FFT"] = request.param
+    yield request.param
+    if old is None:
+        del os.environ["FAME_FFT"]
+    else:
+        os.environ["FAME_FFT"] = old
+
+
 @pytest.mark.parametrize("kname", list(KPTS))
 @pytest.mark.parametrize("sname", list(STRUCTS))
-def test_operators(gpu, sname, kname):
+def test_operators(gpu, engine, sname, kname):
     import torch
     lo, fl, ol, eps = make(sname)
     k = KPTS[kname]
@@ -57,7 +73,7 @@ def test_operators(gpu, sname, kname):
 
 
 @pytest.mark.parametrize("sname", ["sc", "fcc"])
-def test_cg(gpu, sname):
+def test_cg(gpu, engine, sname):
     import torch
     lo, fl, ol, eps = make(sname)
     k = KPTS["k"]
@@ -75,7 +91,7 @@ def test_cg(gpu, sname):
 
 @pytest.mark.parametrize("prec", [F.PREC_F64, F.PREC_F32_REFINE])
 @pytest.mark.parametrize("sname", list(STRUCTS))
-def test_eigs_batched(gpu, sname, prec):
+def test_eigs_batched(gpu, engine, sname, prec):
     lo, fl, ol, eps = make(sname)
     ks = np.stack([KPTS["k"], KPTS["gamma"], np.array([0.05, 0.31, -0.12])])
     nwant, nstep = 6, 20
@@ -95,4 +111,34 @@ def test_eigs_batched(gpu, sname, prec):
         # by es.tol = 1e-12; in lambda space that bound is amplified by lambda*||A_r||
         # (||A_r|| ~ max|Lambda|^2 / eps_min), so the field residual is checked at 1e-6 lambda.
         assert np.all(res[i][nz] < 1e-6 * lam_g[nz]), res[i]
+    S.close()
+
+
+@pytest.mark.parametrize("sname,nx", [("fcc", 22), ("sc", 20)])
+def test_operators_default_engine_medium(gpu, sname, nx):
+    """Grids whose axes the default engine puts on the matrix cores (16..127),
+    one with a prime axis (FCC 22 -> 22x19x18: 19 is prime)."""
+    import torch
+    lt, _, kw, mat, ein = __import__("cases").STRUCTS[sname]
+    lo = F.build_lattice(lt, nx, **kw)
+    fl = F.to_fame_lattice(lo)
+    eps = F.material_eps(lo, mat, [ein], 1.0)
+    ol = O.lattice_from(fl)
+    k = KPTS["k"]
+    old = os.environ.pop("FAME_FFT", None)
+    try:
+        S = F.Solver(fl, max_batch=1)
+    finally:
+        if old is not None:
+            os.environ["FAME_FFT"] = old
+    S.set_eps(eps)
+    S.set_wave_vectors(k)
+    t = O.Tables(ol, k)
+    x = _rand(2 * t.Nd, 5)
+    xd = torch.tensor(x, device=gpu)
+    invB = 1.0 / eps
+    assert _rel(S.apply(F.OP_QR, 0, xd).cpu().numpy(), t.qr(x)) < 1e-11
+    assert _rel(S.apply(F.OP_QBQ, 0, xd).cpu().numpy(), t.qbq(invB, x)) < 1e-11
+    # the FP32 operator against the FP64 oracle: single-precision rounding only
+    assert _rel(S.apply(F.OP_QBQ32, 0, xd).cpu().numpy(), t.qbq(invB, x)) < 2e-6
     S.close()

@@ -9,6 +9,7 @@ What each check can catch:
   tridiagonal eigensolver vs LAPACK -> ordering, eigenvector columns
   GVqrrq vs a dense shifted QR step -> Givens sign convention, symmetrisation
   Lanczos eigenvalues vs dense eigh -> anything in CG / Lanczos / lock-purge / Gamma rule
+  CG solves QBQ x = b, invAr inverts Sigma QBQ Sigma -> a wrong CG update or Sigma placement
   a planted Lambda error            -> shows the curl check is sensitive to the tables
 """
 import json
@@ -161,3 +162,20 @@ def test_oracle_eigs_vs_dense(case):
     nz = ref > 0
     assert np.all(lam[~nz] == 0)
     assert np.max(np.abs(lam[nz] - ref[nz]) / ref[nz]) < 1e-10
+
+
+@pytest.mark.parametrize("sname", ["sc", "fcc"])
+def test_cg_and_invar_solve(sname):
+    """orc_cg returns x with ||QBQ x - b|| at the requested absolute tolerance
+    (CG.cu stops on <r,r> <= tol^2), and orc_invar returns y with
+    Sigma QBQ Sigma y = x (A_r^{-1}, ..._Isotropic_invAr.cu:8-48)."""
+    lo, fl, ol, eps = make(sname)
+    t = O.Tables(ol, KPTS["k"])
+    invB = 1.0 / eps
+    b = _rand(2 * t.Nd, 4) * 1e-2
+    x, it, res = t.cg(invB, b, 2000, 1e-12)
+    assert 1 < it < 2000
+    assert np.linalg.norm(t.qbq(invB, x) - b) <= 2e-12
+    sig = np.tile(t.get(3), 2)
+    y, _ = t.invar(invB, b, 2000, 1e-13)
+    assert np.linalg.norm(sig * t.qbq(invB, sig * y) - b) <= 1e-10 * np.linalg.norm(b)
