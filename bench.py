@@ -74,6 +74,33 @@ def cpu_baseline(fl, eps, k, qbq_per_k, seconds):
                       f"GPU solve; CPU CG/Lanczos vector work not included"}
 
 
+COPY_BYTES = 1 << 30
+
+
+def passes_only(F, fl, eps, ks, args, torch, dev):
+    """Per-pass QBQ timings of the batch and a 1 GiB device copy (reads and
+    writes COPY_BYTES once each), printed as one JSON line."""
+    S = F.Solver(fl, max_batch=args.nk)
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    tune = {"fp32": S.autotune(4), "fp64": S.autotune(8)}
+    us32 = S.time_qbq_passes(4, args.reps)
+    us64 = S.time_qbq_passes(8, args.reps)
+    S.close()
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    cp_us = (time.perf_counter() - t0) / args.reps * 1e6
+    names = ["inner_expand", "mid_inverse", "outer_eps", "mid_forward", "inner_contract", "qbq"]
+    print(json.dumps({"grid": list(fl.grid_nums), "nk": args.nk,
+                      "fp32_us": dict(zip(names, map(float, us32))), "fp64_us": dict(zip(names, map(float, us64))),
+                      "copy_bytes": COPY_BYTES, "copy_us": cp_us, "tuned_tiles": tune}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -86,6 +113,10 @@ def main():
     ap.add_argument("--plain-nk", type=int, default=30, help="wave vectors timed with the plain FP64 solver")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--passes-only", action="store_true",
+                    help="only time the five QBQ passes on the batch (FP32 and FP64) and one plain copy of "
+                         "known size (the reference kernel for HBM counter scaling); for profiler runs")
+    ap.add_argument("--reps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver shards over wave vectors with no communication, "
@@ -98,6 +129,10 @@ def main():
     tol, maxit = 1e-12, 300
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
     log(f"grid {tuple(fl.grid_nums)} N={fl.grid_nums[0]*fl.grid_nums[1]*fl.grid_nums[2]} nk={args.nk}")
+
+    if args.passes_only:
+        passes_only(F, fl, eps, ks, args, torch, dev)
+        return
 
     # ---- batched FP32 + FP64 refinement (the headline step) ----
     S = F.Solver(fl, max_batch=args.nk)
@@ -116,6 +151,7 @@ def main():
     log(f"batched FP32+refine: {t_b:.2f}s for {args.steps} step(s) -> {kps_b:.3f} k-pts/s")
 
     # ---- roofline of the dominant QBQ pass (FP32 passes dominate the batched solve) ----
+    tiles = {"fp32": S.autotune(4), "fp64": S.autotune(8)}    # the choices the solve used (same batch shape)
     us32 = S.time_qbq_passes(4, 10)
     us64 = S.time_qbq_passes(8, 10)
     N = S.N
@@ -193,6 +229,7 @@ def main():
                             "all_converged": bool(all(p["converged"] for p in prof_b))},
         "qbq_fp32_us_per_batch": us32[5], "qbq_fp64_us_per_batch": us64[5],
         "qbq_path_gbs": qbq_gbs,
+        "tuned_tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": f"k_{names[dom]} (FP32, {args.nk} problems per launch)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

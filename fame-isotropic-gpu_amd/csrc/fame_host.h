@@ -10,6 +10,9 @@
 #define NBLK_VEC 1024   /* blocks per problem of the vector/reduction kernels */
 
 enum { FFT_ENGINE_AUTO = 0, FFT_ENGINE_STOCKHAM = 1, FFT_ENGINE_MFMA = 2 };
+enum { AX_INNER = 0, AX_MID = 1, AX_OUTER = 2 };
+
+struct PassTile { int eng = 0, L = 0, P = 0; };
 
 struct AxisPlan {
     int n = 0, M = 0, blue = 0, mfma = 0;
@@ -27,9 +30,14 @@ struct fame_ctx {
     fame_lattice lat;
     fame::GridDev g;
     int N = 0, maxb = 0;
-    AxisPlan pa, pb, pc;
-    int Lin[2] = {0, 0}, Pin[2] = {0, 0}, Lmid[2] = {0, 0}, Pmid[2] = {0, 0}, Lout[2] = {0, 0}, Pout[2] = {0, 0};
-    int nblk_in[2] = {0, 0};
+    /* plans per axis (AX_*) and engine (0 Stockham/Bluestein, 1 direct DFT on the
+     * matrix cores; ax[a][1].n == 0 when not available for that length) */
+    AxisPlan ax[3][2];
+    PassTile tile[3][2];          /* chosen engine and lines per block, [axis][precision index] */
+    int nblk_in[2] = {0, 0};      /* blocks of the inner-axis kernels per problem */
+    int eng_forced = -1;          /* FAME_FFT: -1 auto, else the only engine allowed */
+    bool tuned[2] = {false, false};
+    bool autotune = true;
     /* permittivity, kernel layout */
     double* invB64 = nullptr; float* invB32 = nullptr; double* eps_int = nullptr; double* eps_ref = nullptr;
     bool have_eps = false;
@@ -73,6 +81,8 @@ struct fame_ctx {
     fame_status cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
                    const int* mask_dev, int* iters);
     fame_status invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls, int* iters);
+    fame_status tune(int prec);
+    const AxisPlan& plan(int axis, int prec) const { return ax[axis][tile[axis][prec == 8 ? 0 : 1].eng]; }
     ~fame_ctx();
 };
 

@@ -184,6 +184,11 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
         FAIL(FAME_ERR_ARG, "fame_eigs: need 1 <= nwant < nstep <= 62 and positive maxit");
     if (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE) FAIL(FAME_ERR_ARG, "fame_eigs: unknown precision");
     HIPCHK(hipSetDevice(c->device));
+    /* first solve of a context: tune the pass kernels on this batch shape */
+    if (c->autotune) {
+        if (!c->tuned[0]) CHK(c->tune(8));
+        if (precision == FAME_PREC_F32_REFINE && !c->tuned[1]) CHK(c->tune(4));
+    }
     auto t0 = std::chrono::steady_clock::now();
     c->ls_last = *ls;
     const int nk = c->nk, nstep = es->nstep, nwant = es->nwant, m = nstep + 1;

@@ -227,20 +227,31 @@ fame_status AxisPlan::build(int n_, int engine) {
     return FAME_OK;
 }
 
-/* lines per block: as many as the per-thread register budget allows (cap
- * elements of ncomp x L lines of length M), at most maxL, rounded down to a
- * multiple of `align` (a full 128-byte row of the strided axes) when possible;
- * the LDS line pitch is odd so that strided columns spread over the banks.  */
-static void pick_tile(const AxisPlan& p, int ncomp, int prec, int maxL, int align, int extra, int* L, int* P) {
-    const int pi = prec == 8 ? 0 : 1;
-    int l = p.cap[pi] / (ncomp * p.M);
-    if (l > maxL) l = maxL;
+/* LDS entries of one block beyond the lines: the middle-axis kernels keep one
+ * twiddle per line position */
+static int tile_extra(int axis, const GridDev& g) { return axis == AX_MID ? g.nb : 0; }
+static int tile_ncomp(int axis) { return axis == AX_INNER ? 3 : 1; }
+
+/* is L lines per block feasible for this plan: register budget of the Stockham
+ * passes (ncomp x L lines of M elements within cap), at most 32 lines for the
+ * inner kernels, LDS within 60 KiB (+ the kernels' static LDS < 64 KiB) */
+static bool tile_ok(const AxisPlan& p, int axis, int prec, int L, const GridDev& g) {
+    const int pi = prec == 8 ? 0 : 1, nc = tile_ncomp(axis);
+    if (L < 1 || (axis == AX_INNER && L > MAX_LINES_INNER)) return false;
+    if (!p.mfma && (long long)nc * L * p.M > p.cap[pi]) return false;
+    return fame_line_lds(p.lists[pi][0], prec, nc * L, p.M | 1, tile_extra(axis, g)) <= 60 * 1024;
+}
+
+/* default lines per block before tuning: as many as feasible up to maxL, rounded
+ * down to a full 128-byte row (8 FP64 / 16 FP32 lines) on the strided axes */
+static int default_lines(const AxisPlan& p, int axis, int prec, const GridDev& g) {
+    const int maxL = axis == AX_INNER ? MAX_LINES_INNER : (p.mfma ? 32 : 64);
+    const int align = axis == AX_INNER ? 1 : (prec == 8 ? 8 : 16);
+    int l = maxL;
+    while (l > 1 && !tile_ok(p, axis, prec, l, g)) --l;
+    if (!tile_ok(p, axis, prec, l, g)) return 0;
     if (align > 1 && l >= align) l = (l / align) * align;
-    const int pitch = p.M | 1;
-    const size_t lds_max = 60 * 1024;   /* + the kernels' static LDS stays within 64 KiB */
-    while (l >= 1 && fame_line_lds(p.lists[pi][0], prec, ncomp * l, pitch, extra) > lds_max) --l;
-    *L = l < 1 ? 0 : l;
-    *P = pitch;
+    return l;
 }
 
 /* ======================================================================== */
@@ -283,30 +294,38 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     g.m_nb = fame_magic32(g.nb);
     c->N = g.N;
     fame_status st;
-    /* FAME_FFT=stockham|mfma|auto (default auto) selects the line-transform engine */
-    int engine = FFT_ENGINE_AUTO;
+    /* FAME_FFT=stockham|mfma restricts the line-transform engine (default: both
+     * are planned, the matrix cores are preferred for lengths 16..127 and the
+     * autotuner may change the choice); FAME_AUTOTUNE=0 disables tuning */
     if (const char* e = std::getenv("FAME_FFT")) {
-        if (!std::strcmp(e, "stockham")) engine = FFT_ENGINE_STOCKHAM;
-        else if (!std::strcmp(e, "mfma")) engine = FFT_ENGINE_MFMA;
+        if (!std::strcmp(e, "stockham")) c->eng_forced = 0;
+        else if (!std::strcmp(e, "mfma")) c->eng_forced = 1;
     }
-    if ((st = c->pa.build(g.na, engine)) != FAME_OK || (st = c->pb.build(g.nb, engine)) != FAME_OK ||
-        (st = c->pc.build(g.nc, engine)) != FAME_OK) {
-        delete c; return st;
+    if (const char* e = std::getenv("FAME_AUTOTUNE")) c->autotune = std::strcmp(e, "0") != 0;
+    const int len[3] = {g.na, g.nb, g.nc};
+    for (int a = 0; a < 3; ++a) {
+        if ((st = c->ax[a][0].build(len[a], FFT_ENGINE_STOCKHAM)) != FAME_OK) { delete c; return st; }
+        if (c->eng_forced != 0 && len[a] >= 2 && len[a] <= 127 &&
+            (st = c->ax[a][1].build(len[a], FFT_ENGINE_MFMA)) != FAME_OK) { delete c; return st; }
     }
     for (int pi = 0; pi < 2; ++pi) {
-        int prec = pi == 0 ? 8 : 4;
-        const int align = prec == 8 ? 8 : 16;
-        pick_tile(c->pa, 3, prec, MAX_LINES_INNER, 1, 0, &c->Lin[pi], &c->Pin[pi]);
-        pick_tile(c->pb, 1, prec, c->pb.mfma ? 32 : 64, align, g.nb, &c->Lmid[pi], &c->Pmid[pi]);
-        pick_tile(c->pc, 1, prec, c->pc.mfma ? 32 : 64, align, 0, &c->Lout[pi], &c->Pout[pi]);
-        if (c->Lin[pi] < 1 || c->Lmid[pi] < 1 || c->Lout[pi] < 1) {
-            delete c;
-            FAIL(FAME_ERR_UNSUPPORTED, "FFT length too large for the LDS-resident line transforms");
+        const int prec = pi == 0 ? 8 : 4;
+        for (int a = 0; a < 3; ++a) {
+            int eng = 0;
+            if (c->ax[a][1].n > 0 && (c->eng_forced == 1 || (c->eng_forced < 0 && len[a] >= 16))) eng = 1;
+            PassTile& t = c->tile[a][pi];
+            t.eng = eng;
+            t.L = default_lines(c->ax[a][eng], a, prec, g);
+            t.P = c->ax[a][eng].M | 1;
+            if (t.L < 1) {
+                delete c;
+                FAIL(FAME_ERR_UNSUPPORTED, "FFT length too large for the LDS-resident line transforms");
+            }
         }
     }
     const size_t N = g.N, B = max_batch;
-    c->nblk_in[0] = (g.nb * g.nc + c->Lin[0] - 1) / c->Lin[0];
-    c->nblk_in[1] = (g.nb * g.nc + c->Lin[1] - 1) / c->Lin[1];
+    c->nblk_in[0] = (g.nb * g.nc + c->tile[AX_INNER][0].L - 1) / c->tile[AX_INNER][0].L;
+    c->nblk_in[1] = (g.nb * g.nc + c->tile[AX_INNER][1].L - 1) / c->tile[AX_INNER][1].L;
 #define A(ptr, bytes) do { fame_status s_ = dalloc(c, (void**)&(ptr), (bytes)); if (s_ != FAME_OK) { delete c; return s_; } } while (0)
     A(c->prob_d, sizeof(ProbDev) * B);
     A(c->sig, sizeof(double) * N * B);
@@ -320,7 +339,8 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     for (int i = 0; i < 4; ++i) A(c->cg64[i], 16 * 2 * N * B);
     for (int i = 0; i < 5; ++i) A(c->cg32[i], 8 * 2 * N * B);
     for (int i = 0; i < 6; ++i) A(c->ref64[i], 16 * 2 * N * B);
-    int nbmax = std::max(std::max(c->nblk_in[0], c->nblk_in[1]), NBLK_VEC);
+    /* inner-kernel partials: room for one line per block (any tuned tile) */
+    int nbmax = std::max(g.nb * g.nc, NBLK_VEC);
     A(c->partials, sizeof(double) * 4 * nbmax * B);
     A(c->partials2, sizeof(double) * 2 * nbmax * B);
     A(c->cgsc, sizeof(CgScal) * B);
@@ -351,7 +371,7 @@ extern "C" fame_status fame_query(const fame_ctx* c, int* N, int* nk, int* len3)
     if (!c) FAIL(FAME_ERR_ARG, "null context");
     if (N) *N = c->N;
     if (nk) *nk = c->nk;
-    if (len3) { len3[0] = c->pa.M; len3[1] = c->pb.M; len3[2] = c->pc.M; }
+    if (len3) for (int a = 0; a < 3; ++a) len3[a] = c->plan(a, 8).M;
     return FAME_OK;
 }
 
@@ -440,26 +460,29 @@ static inline int pix(int prec) { return prec == 8 ? 0 : 1; }
 KInner fame_ctx::kin(int prec, int p0, const int* act, int list) const {
     KInner a;
     std::memset(&a, 0, sizeof(a));
-    a.g = g; a.prob = prob_d + p0; a.plan = pa.lists[pix(prec)][list]; a.active = act;
+    const PassTile& t = tile[AX_INNER][pix(prec)];
+    a.g = g; a.prob = prob_d + p0; a.plan = ax[AX_INNER][t.eng].lists[pix(prec)][list]; a.active = act;
     a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
     a.xstride = 2 * (size_t)N; a.fstride = 3 * (size_t)N;
-    a.L = Lin[pix(prec)]; a.P = Pin[pix(prec)];
+    a.L = t.L; a.P = t.P;
     return a;
 }
 KAxis fame_ctx::kmid(int prec, int p0, const int* act, int list) const {
     KAxis a;
     std::memset(&a, 0, sizeof(a));
-    a.g = g; a.prob = prob_d + p0; a.plan = pb.lists[pix(prec)][list]; a.active = act;
+    const PassTile& t = tile[AX_MID][pix(prec)];
+    a.g = g; a.prob = prob_d + p0; a.plan = ax[AX_MID][t.eng].lists[pix(prec)][list]; a.active = act;
     a.f = prec == 8 ? (void*)(F64 + (size_t)p0 * 3 * N) : (void*)(F32 + (size_t)p0 * 3 * N);
-    a.fstride = 3 * (size_t)N; a.L = Lmid[pix(prec)]; a.P = Pmid[pix(prec)];
+    a.fstride = 3 * (size_t)N; a.L = t.L; a.P = t.P;
     a.mL = fame_magic32(a.L);
     return a;
 }
 KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
     KAxis a = kmid(prec, p0, act, list);
-    a.plan = pc.lists[pix(prec)][list];
+    const PassTile& t = tile[AX_OUTER][pix(prec)];
+    a.plan = ax[AX_OUTER][t.eng].lists[pix(prec)][list];
     a.invB = prec == 8 ? (const void*)invB64 : (const void*)invB32;
-    a.L = Lout[pix(prec)]; a.P = Pout[pix(prec)];
+    a.L = t.L; a.P = t.P;
     a.mL = fame_magic32(a.L);
     return a;
 }
@@ -473,7 +496,7 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
     HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
-    ki.plan = pa.lists[pix(prec)][LIST_FWD];
+    ki.plan = plan(AX_INNER, prec).lists[pix(prec)][LIST_FWD];
     ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
     HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
     if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
@@ -570,6 +593,88 @@ extern "C" fame_status fame_apply(fame_ctx* c, int op, int ik, const void* x, vo
         default: FAIL(FAME_ERR_ARG, "fame_apply: unknown operator");
     }
     HIPCHK(hipStreamSynchronize(c->stream));
+    return FAME_OK;
+}
+
+/* ======================================================================== */
+/*  Autotuner                                                                */
+/* ======================================================================== */
+/* For each pass kernel (inner, middle, outer) of precision prec, time every
+ * feasible (engine, lines per block) on the current batch (all nk problems, the
+ * shape of the production launches) and keep the fastest.  The field and
+ * vector buffers are zeroed first; the results are discarded.               */
+fame_status fame_ctx::tune(int prec) {
+    const int pi = pix(prec);
+    if (nk < 1) FAIL(FAME_ERR_STATE, "autotune: wave vectors must be set");
+    const size_t es = prec == 8 ? 16 : 8;
+    void* F = prec == 8 ? (void*)F64 : (void*)F32;
+    void* x = prec == 8 ? cg64[1] : cg32[1];
+    void* y = prec == 8 ? cg64[2] : cg32[2];
+    HIPCHK(hipMemsetAsync(F, 0, es * 3 * (size_t)N * nk, stream));
+    HIPCHK(hipMemsetAsync(x, 0, es * 2 * (size_t)N * nk, stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    static const int candL_inner[] = {2, 4, 6, 8, 10, 12, 16, 20, 24, 32};
+    static const int candL_axis[] = {4, 8, 16, 24, 32, 48, 64};
+    fame_status st = FAME_OK;
+    for (int a = 0; a < 3 && st == FAME_OK; ++a) {
+        PassTile best = tile[a][pi];
+        float best_ms = 1e30f;
+        for (int eng = 0; eng < 2; ++eng) {
+            if (ax[a][eng].n == 0 || (eng_forced >= 0 && eng != eng_forced)) continue;
+            const int* cl = a == AX_INNER ? candL_inner : candL_axis;
+            const int ncl = a == AX_INNER ? 10 : 7;
+            for (int ci = 0; ci < ncl; ++ci) {
+                const int L = cl[ci];
+                if (!tile_ok(ax[a][eng], a, prec, L, g)) continue;
+                const PassTile save = tile[a][pi];
+                tile[a][pi].eng = eng; tile[a][pi].L = L; tile[a][pi].P = ax[a][eng].M | 1;
+                float ms = 0;
+                for (int rep = -1; rep < 3; ++rep) {
+                    if (rep == 0 && hipEventRecord(e0, stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
+                    hipError_t e = hipSuccess;
+                    if (a == AX_INNER) {
+                        KInner ki = kin(prec, 0, nullptr, LIST_INV);
+                        ki.x = x; ki.pikind = 0;
+                        e = fame_launch_inner(ki, 1, prec, nk, stream);
+                        KInner kf = kin(prec, 0, nullptr, LIST_FWD);
+                        kf.y = y; kf.pikind = 0;
+                        if (e == hipSuccess) e = fame_launch_inner(kf, 0, prec, nk, stream);
+                    } else if (a == AX_MID) {
+                        e = fame_launch_mid(kmid(prec, 0, nullptr, LIST_INV), +1, prec, nk, stream);
+                        if (e == hipSuccess) e = fame_launch_mid(kmid(prec, 0, nullptr, LIST_FWD), -1, prec, nk, stream);
+                    } else {
+                        e = fame_launch_outer(kout(prec, 0, nullptr, LIST_EPS), OUTER_EPS, prec, nk, stream);
+                    }
+                    if (e != hipSuccess) { st = FAME_ERR_HIP; break; }
+                }
+                if (st == FAME_OK && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                      hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) st = FAME_ERR_HIP;
+                if (st != FAME_OK) { tile[a][pi] = save; break; }
+                if (ms < best_ms) { best_ms = ms; best = tile[a][pi]; }
+                tile[a][pi] = save;
+            }
+            if (st != FAME_OK) break;
+        }
+        tile[a][pi] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st != FAME_OK) FAIL(st, std::string("autotune: HIP error: ") + hipGetErrorString(hipGetLastError()));
+    nblk_in[pi] = (g.nb * g.nc + tile[AX_INNER][pi].L - 1) / tile[AX_INNER][pi].L;
+    tuned[pi] = true;
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_autotune(fame_ctx* c, int prec, int* choice) {
+    if (!c || (prec != 8 && prec != 4)) FAIL(FAME_ERR_ARG, "fame_autotune: bad arguments");
+    HIPCHK(hipSetDevice(c->device));
+    CHK(c->tune(prec));
+    if (choice) {
+        const int pi = prec == 8 ? 0 : 1;
+        for (int a = 0; a < 3; ++a) { choice[2 * a] = c->tile[a][pi].eng; choice[2 * a + 1] = c->tile[a][pi].L; }
+    }
     return FAME_OK;
 }
 

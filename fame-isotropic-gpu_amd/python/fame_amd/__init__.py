@@ -365,6 +365,12 @@ class Solver:
         _check(lib().fame_residuals(self._ctx, nwant, _ptr(f), C.c_void_p(E.data_ptr()), _ptr(res)), "fame_residuals")
         return res
 
+    def autotune(self, prec_bytes: int = 8):
+        """Tune the pass kernels on the current batch; returns {pass: (engine, lines)}."""
+        ch = np.zeros(6, np.int32)
+        _check(lib().fame_autotune(self._ctx, prec_bytes, _ptr(ch)), "fame_autotune")
+        return {name: (int(ch[2 * i]), int(ch[2 * i + 1])) for i, name in enumerate(("inner", "mid", "outer"))}
+
     def time_qbq_passes(self, prec_bytes: int = 8, reps: int = 20):
         """hipEvent-timed average microseconds of the 5 QBQ passes and one QBQ (all wave vectors)."""
         us = np.zeros(6)

@@ -149,6 +149,15 @@ fame_status fame_eig_grad(fame_ctx* ctx, int nwant, const double* freq, const vo
  * counterpart.                                                                 */
 fame_status fame_time_qbq_passes(fame_ctx* ctx, int prec, int reps, double* us6);
 
+/* Autotuner: time every feasible (line-transform engine, lines per block) of the
+ * inner, middle and outer pass kernels of precision prec (8 or 4) on the
+ * current batch and keep the fastest.  fame_eigs runs it once per context and
+ * precision unless FAME_AUTOTUNE=0.  choice (host, 6 ints, may be NULL)
+ * receives {engine, lines} for the inner, middle and outer passes (engine 0
+ * Stockham/Bluestein, 1 direct DFT on the matrix cores).  Replaces the fixed
+ * cuFFT plans of FAME_Create_cublas (FAME_Main/FAME_Create_cublas.cu:16-25).  */
+fame_status fame_autotune(fame_ctx* ctx, int prec, int* choice);
+
 /* Stream used by all launches (a hipStream_t passed as void*; NULL = default). */
 fame_status fame_set_stream(fame_ctx* ctx, void* stream);
 
