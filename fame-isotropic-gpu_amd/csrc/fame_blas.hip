@@ -526,6 +526,15 @@ __global__ void k_lz_coeffs(const double* hdot, double* T0d, const double* T1d, 
     else { h[0] = a; h[1] = 0; h[2] = 0; h[3] = 0; }
 }
 
+/* out[p] = in[p] && sum_i h[p][i]^2 > tau2 (h: n doubles per problem) */
+__global__ void k_lz_second_pass_mask(const double* h, int n, double tau2, const int* in, int* out) {
+    const int p = blockIdx.x;
+    double v[1] = {0.0};
+    for (int i = threadIdx.x; i < n; i += blockDim.x) { const double x = h[(size_t)p * n + i]; v[0] += x * x; }
+    block_sum_all<1>(v);
+    if (threadIdx.x == 0) out[p] = (!in || in[p]) && v[0] > tau2;
+}
+
 /* beta_j = ||U_{j+1}|| and its inverse (Lanczos_decomp_gpu.cu:171-174) */
 __global__ void k_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, const int* mask) {
     const int p = blockIdx.x;
@@ -554,6 +563,11 @@ __global__ void k_cg_init_scal2(CgScal* sc, const double* partials, int nblk, do
 hipError_t fame_lz_coeffs(const double* hdot, double* T0d, const double* T1d, int ld, int j, double* hout, int nprob,
                           const int* mask, hipStream_t s) {
     hipLaunchKernelGGL(k_lz_coeffs, dim3(nprob), dim3(64), 0, s, hdot, T0d, T1d, ld, j, hout, mask);
+    return hipGetLastError();
+}
+hipError_t fame_lz_second_pass_mask(const double* h, int n, double tau2, const int* in, int* out, int nprob,
+                                    hipStream_t s) {
+    hipLaunchKernelGGL(k_lz_second_pass_mask, dim3(nprob), dim3(64), 0, s, h, n, tau2, in, out);
     return hipGetLastError();
 }
 hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, int nprob, const int* mask,

@@ -42,6 +42,15 @@ __device__ __forceinline__ double2 lam_of(double x, double inv_h) {
 }
 
 
+/* Workgroups go to the 8 XCDs round robin by linear block id; with gridDim.x a
+ * multiple of 8, block x runs on XCD x % 8.  xcd_tile() renumbers the blocks so
+ * that each XCD gets a run of consecutive tiles: tiles that share boundary
+ * cache lines then meet in the same L2 instead of fetching them twice.     */
+__device__ __forceinline__ int xcd_tile() {
+    return (int)(blockIdx.x & 7u) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+}
+__host__ __device__ inline int xcd_round(int nblocks) { return (nblocks + 7) & ~7; }
+
 /* quotient n / d from m = ceil(2^32 / d) (m = 0 encodes d = 1); exact for n*d < 2^32 */
 __device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulhi((unsigned)n, m) : n; }
 
@@ -344,17 +353,17 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
     const bool inv = st.sign > 0;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kq = lane >> 4;
-    /* A operands: lane holds row q = 16w + row, column k = 4 ks + kq (t = k + 1);
-     * r = q t mod n advances by 4q mod n per k step and indexes the twiddle table */
+    /* A operands: lane holds row q = 16w + row, column k = 4 ks + kq (t = k + 1)
+     * of the cos / sin matrices; r = q t mod n advances by 4q mod n per k step
+     * and indexes the twiddle table exp(-2 pi i r / n) */
     const int q = w * 16 + row;
     const bool qon = w < nqt && q < Q;
     const int dr = (4 * q) % n;
     const int r0 = (q * (kq + 1)) % n;
-    T* SPr = reinterpret_cast<T*>(tb.ws);
-    T* SPi = SPr + K * 16;
-    T* SMr = SPr + 2 * K * 16;
-    T* SMi = SPr + 3 * K * 16;
-    T2* xh = reinterpret_cast<T2*>(SPr + 4 * K * 16);    /* x_0 [16], h [16] */
+    /* workspace: s_t and d_t of 16 lines as complex [k][line], then x_0 [16], h [16] */
+    T2* SP = tb.ws;
+    T2* SM = SP + K * 16;
+    T2* xh = SM + K * 16;
     const bool even = (n & 1) == 0;
     const int ngr = (nl + 15) >> 4;
     for (int gi = 0; gi < ngr; ++gi) {
@@ -366,7 +375,7 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
                 const T2 a = S[line * P + t], b = S[line * P + n - t];
                 sp = cadd(a, b); sm = csub(a, b);
             }
-            SPr[idx] = sp.x; SPi[idx] = sp.y; SMr[idx] = sm.x; SMi[idx] = sm.y;
+            SP[idx] = sp; SM[idx] = sm;
         }
         if (threadIdx.x < 16) {
             const int line = gi * 16 + threadIdx.x;
@@ -386,17 +395,18 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
                 const int t = ks * 4 + kq + 1;
                 T a_c = 0, a_s = 0;
                 if (qon && t <= m) {
-                    const T2 wv = tb.W[r];      /* exp(-2 pi i r / n) = cos - i sin */
+                    const T2 wv = tb.W[r];
                     a_c = wv.x; a_s = -wv.y;
                 }
                 r += dr;
                 if (r >= n) r -= n;
                 /* B operand: lane holds row k = 4 ks + kq, column (line) row */
                 const int o = (ks * 4 + kq) * 16 + row;
-                cr = Mfma<T>::mac(a_c, SPr[o], cr);
-                ci = Mfma<T>::mac(a_c, SPi[o], ci);
-                sr = Mfma<T>::mac(a_s, SMr[o], sr);
-                si = Mfma<T>::mac(a_s, SMi[o], si);
+                const T2 bs = SP[o], bd = SM[o];
+                cr = Mfma<T>::mac(a_c, bs.x, cr);
+                ci = Mfma<T>::mac(a_c, bs.y, ci);
+                sr = Mfma<T>::mac(a_s, bd.x, sr);
+                si = Mfma<T>::mac(a_s, bd.y, si);
             }
             /* result: lane holds rows q = 16w + Mfma::row(kq, v), column (line) row */
             const int line = gi * 16 + row;

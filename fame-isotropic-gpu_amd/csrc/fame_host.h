@@ -2,6 +2,8 @@
  * fame_host.h -- internal host-side state of a solver context (not public).
  */
 #pragma once
+#include <map>
+#include <tuple>
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "fame_hip.h"
@@ -69,6 +71,14 @@ struct fame_ctx {
     fame_ls ls_last = {0, 0.0};
     long long cnt_qbq64 = 0, cnt_qbq32 = 0;
     int last_iters = 0;
+    /* one CG iteration captured as a hipGraph per (precision, first problem,
+     * problem count, solution vector); replayed chunk times between the host
+     * checks of the active flags.  FAME_GRAPH=0 launches eagerly instead.   */
+    bool use_graphs = true;
+    hipStream_t cap_stream = nullptr;
+    std::map<std::tuple<int, int, int, const void*>, hipGraphExec_t> cg_graphs;
+    void drop_graphs();
+    fame_status cg_iteration(int prec, void* x, int p0, int nprob);
 
     KInner kin(int prec, int p0, const int* act, int list) const;
     KAxis kmid(int prec, int p0, const int* act, int list) const;

@@ -112,6 +112,8 @@ hipError_t fame_launch_sigma(const fame::GridDev& g, const fame::ProbDev* prob, 
 /* Lanczos scalar helpers (fame_blas.hip) */
 hipError_t fame_lz_coeffs(const double* hdot, double* T0d, const double* T1d, int ld, int j, double* hout,
                           int nprob, const int* mask, hipStream_t s);
+hipError_t fame_lz_second_pass_mask(const double* h, int n, double tau2, const int* in, int* out, int nprob,
+                                    hipStream_t s);
 hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, int nprob, const int* mask,
                         hipStream_t s);
 hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,

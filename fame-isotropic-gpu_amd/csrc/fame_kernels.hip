@@ -53,6 +53,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y;
     if (args.active && !args.active[p]) return;
+    const int nlines = args.g.nb * args.g.nc;
+    const int nblk = (nlines + args.L - 1) / args.L;
+    const int tile = xcd_tile();
+    if (tile >= nblk) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LineSym lsym[MAX_LINES_INNER];
     const GridDev& g = args.g;
@@ -62,8 +66,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
     T2* S = base + lds_head(pl);
-    const int nlines = g.nb * g.nc;
-    const int l0 = blockIdx.x * L;
+    const int l0 = tile * L;
     if ((int)threadIdx.x < L) {
         const int gl = l0 + threadIdx.x;
         if (gl < nlines) {
@@ -168,7 +171,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             }
         }
         if (D) {
-            /* block sum of conj(d) . y into partials[p][blockIdx.x]: wave
+            /* block sum of conj(d) . y into partials[p][tile]: wave
              * shuffles, then the four wave sums in a fixed order */
             __shared__ double red[2][FFT_THREADS / 64];
 #pragma unroll
@@ -178,7 +181,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             if (threadIdx.x == 0) {
                 double sr = 0, si = 0;
                 for (int w = 0; w < FFT_THREADS / 64; ++w) { sr += red[0][w]; si += red[1][w]; }
-                double* part = args.partials + ((size_t)p * gridDim.x + blockIdx.x) * 2;
+                double* part = args.partials + ((size_t)p * nblk + tile) * 2;
                 part[0] = sr; part[1] = si;
             }
         }
@@ -196,6 +199,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
+    const int ntile = (args.g.na + args.L - 1) / args.L;
+    const int tile = xcd_tile();
+    if (tile >= ntile * args.g.nc) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
@@ -205,8 +211,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
     T2* twb = base + lds_head(pl);
     T2* S = twb + nb;
-    const int ntile = (na + L - 1) / L;
-    const int cf = blockIdx.x / ntile, a0 = (blockIdx.x - cf * ntile) * L;
+    const int cf = tile / ntile, a0 = (tile - cf * ntile) * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
     const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)nb;
     for (int b = threadIdx.x; b < nb; b += FFT_THREADS) twb[b] = tw_expipi<T2>(phc * (double)b);
@@ -248,6 +253,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
+    const int tile = xcd_tile();
+    if (tile * args.L >= args.g.na * args.g.nb) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
@@ -257,7 +264,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
     T2* S = base + lds_head(pl);
-    const int q0 = blockIdx.x * L;
+    const int q0 = tile * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
     const int Wd = pl.blue ? pl.M : nc;
     for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
@@ -388,7 +395,7 @@ size_t fame_line_lds(const LinePlanDev& pl, int prec, int nl, int P, int extra) 
 template <typename T, int ENG>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
-    dim3 grid((nlines + a.L - 1) / a.L, nprob);
+    dim3 grid(xcd_round((nlines + a.L - 1) / a.L), nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, 0);
     if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     else hipLaunchKernelGGL((k_inner<T, 0, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
@@ -398,7 +405,7 @@ static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream
 template <typename T, int ENG>
 static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) {
     const int ntile = (a.g.na + a.L - 1) / a.L;
-    dim3 grid(ntile * a.g.nc, 3 * nprob);
+    dim3 grid(xcd_round(ntile * a.g.nc), 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
     if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     else hipLaunchKernelGGL((k_mid<T, -1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
@@ -408,7 +415,7 @@ static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) 
 template <typename T, int ENG>
 static hipError_t launch_outer(const KAxis& a, int mode, int nprob, hipStream_t s) {
     const int plane = a.g.na * a.g.nb;
-    dim3 grid((plane + a.L - 1) / a.L, 3 * nprob);
+    dim3 grid(xcd_round((plane + a.L - 1) / a.L), 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, 0);
     switch (mode) {
         case OUTER_EPS: hipLaunchKernelGGL((k_outer<T, OUTER_EPS, ENG>), grid, dim3(FFT_THREADS), lds, s, a); break;

@@ -63,6 +63,7 @@ struct LzBuffers {
     double* G = nullptr;    /* nk x 2 m^2 */
     double* Q = nullptr;    /* nk x m x (nwant+1) */
     int* mask = nullptr;    /* nk */
+    int* mask2 = nullptr;   /* nk: problems that take the second Gram-Schmidt pass */
 };
 
 fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
@@ -90,6 +91,7 @@ fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
     CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
     CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
     CHK(al((void**)&b.mask, sizeof(int) * nk));
+    CHK(al((void**)&b.mask2, sizeof(int) * nk));
     return FAME_OK;
 }
 
@@ -128,14 +130,23 @@ static fame_status lz_three_term(fame_ctx* c, LzBuffers& b, int j, const int* ma
     return FAME_OK;
 }
 
-/* classical Gram-Schmidt twice against U_0..U_j */
+/* classical Gram-Schmidt against U_0..U_j (Lanczos_decomp_gpu.cu:81-87), with a
+ * second pass for the problems whose first-pass coefficients h = U^H w are not
+ * negligible (||h|| > 1e-8 for the unit vector w): after the three-term step h
+ * is normally O(machine eps), and a second pass only pays once orthogonality
+ * has started to degrade, which is when "twice is enough" is needed. */
 static fame_status lz_reorth(fame_ctx* c, LzBuffers& b, int j, const int* mask_dev) {
     const size_t ps = b.n2 * (b.nstep + 1);
     const int m = j + 1;
+    const int* mk = mask_dev;
     for (int rep = 0; rep < 2; ++rep) {
-        HIPCHK(fame_gemv_h(col(b, 0), col(b, j + 1), b.gpart, GEMV_BLK, m, b.n2, b.n2, ps, ps, b.nk, mask_dev, c->stream));
+        HIPCHK(fame_gemv_h(col(b, 0), col(b, j + 1), b.gpart, GEMV_BLK, m, b.n2, b.n2, ps, ps, b.nk, mk, c->stream));
         HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 2 * m, b.hh, b.nk, c->stream));
-        HIPCHK(fame_gemv_n(col(b, 0), col(b, j + 1), b.hh, m, b.n2, b.n2, ps, ps, 2 * m, b.nk, mask_dev, c->stream));
+        HIPCHK(fame_gemv_n(col(b, 0), col(b, j + 1), b.hh, m, b.n2, b.n2, ps, ps, 2 * m, b.nk, mk, c->stream));
+        if (rep == 0) {
+            HIPCHK(fame_lz_second_pass_mask(b.hh, 2 * m, 1e-16, mask_dev, b.mask2, b.nk, c->stream));
+            mk = b.mask2;
+        }
     }
     return FAME_OK;
 }

@@ -267,7 +267,14 @@ static fame_status dalloc(fame_ctx* c, void** p, size_t bytes) {
     return FAME_OK;
 }
 
+void fame_ctx::drop_graphs() {
+    for (auto& kv : cg_graphs) (void)hipGraphExecDestroy(kv.second);
+    cg_graphs.clear();
+}
+
 fame_ctx::~fame_ctx() {
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     for (void* p : allocs) if (p) (void)hipFree(p);
     for (void* p : big) if (p) (void)hipFree(p);
 }
@@ -302,6 +309,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
         else if (!std::strcmp(e, "mfma")) c->eng_forced = 1;
     }
     if (const char* e = std::getenv("FAME_AUTOTUNE")) c->autotune = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("FAME_GRAPH")) c->use_graphs = std::strcmp(e, "0") != 0;
     const int len[3] = {g.na, g.nb, g.nc};
     for (int a = 0; a < 3; ++a) {
         if ((st = c->ax[a][0].build(len[a], FFT_ENGINE_STOCKHAM)) != FAME_OK) { delete c; return st; }
@@ -364,6 +372,7 @@ extern "C" fame_status fame_destroy(fame_ctx* c) {
 extern "C" fame_status fame_set_stream(fame_ctx* c, void* stream) {
     if (!c) FAIL(FAME_ERR_ARG, "null context");
     c->stream = (hipStream_t)stream;
+    c->drop_graphs();
     return FAME_OK;
 }
 
@@ -664,6 +673,7 @@ fame_status fame_ctx::tune(int prec) {
     if (st != FAME_OK) FAIL(st, std::string("autotune: HIP error: ") + hipGetErrorString(hipGetLastError()));
     nblk_in[pi] = (g.nb * g.nc + tile[AX_INNER][pi].L - 1) / tile[AX_INNER][pi].L;
     tuned[pi] = true;
+    drop_graphs();            /* captured launches carry the old tiles */
     return FAME_OK;
 }
 
@@ -691,20 +701,12 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     const size_t es = prec == 8 ? 16 : 8;
     char* r = (char*)V[0] + es * st * p0;
     char* p = (char*)V[1] + es * st * p0;
-    char* Ap = (char*)V[2] + es * st * p0;
-    const int nbi = nblk_in[pix(prec)];
     HIPCHK(fame_cg_begin(x, r, p, b, partials, NBLK_VEC, n, st, nprob, prec, mask_dev, stream));
     HIPCHK(fame_cg_init_scal2(cgsc, partials, NBLK_VEC, tol, reltol, maxit, nprob, active, mask_dev, stream));
     int done_it = 0;
     int chunk = std::max(1, std::min(maxit, cg_predict[pix(prec)] - 1));
     while (true) {
-        for (int it = 0; it < chunk; ++it) {
-            CHK(qbq(prec, p, Ap, p0, nprob, active, p));
-            HIPCHK(fame_cg_alpha(cgsc, partials2, nbi, nprob, active, stream));
-            HIPCHK(fame_cg_update_xr(x, r, p, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, stream));
-            HIPCHK(fame_cg_beta(cgsc, partials, NBLK_VEC, nprob, active, stream));
-            HIPCHK(fame_cg_update_p(p, r, cgsc, n, st, nprob, prec, active, stream));
-        }
+        for (int it = 0; it < chunk; ++it) CHK(cg_iteration(prec, x, p0, nprob));
         done_it += chunk;
         HIPCHK(hipMemcpyAsync(active_h, active, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -718,6 +720,54 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     int mx = 0;
     for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc[i].iters; mx = std::max(mx, sc[i].iters); }
     cg_predict[pix(prec)] = std::max(2, mx - 1);
+    return FAME_OK;
+}
+
+/* one CG iteration (CG.cu:53-78) on the masked batch: Ap = QBQ p with the
+ * fused <p,Ap> partials, alpha, x += alpha p and r -= alpha Ap with the <r,r>
+ * partials, beta and the active flags, p = r + beta p.  All scalars stay on
+ * the device, so the nine launches are captured once as a graph.          */
+fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
+    void** V = prec == 8 ? cg64 : cg32;
+    const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
+    const size_t es = prec == 8 ? 16 : 8;
+    char* r = (char*)V[0] + es * st * p0;
+    char* p = (char*)V[1] + es * st * p0;
+    char* Ap = (char*)V[2] + es * st * p0;
+    auto launch = [&](hipStream_t s) -> fame_status {
+        const hipStream_t keep = stream;
+        stream = s;
+        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p);
+        stream = keep;
+        CHK(q);
+        const int nbi = nblk_in[pix(prec)];
+        HIPCHK(fame_cg_alpha(cgsc, partials2, nbi, nprob, active, s));
+        HIPCHK(fame_cg_update_xr(x, r, p, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, s));
+        HIPCHK(fame_cg_beta(cgsc, partials, NBLK_VEC, nprob, active, s));
+        HIPCHK(fame_cg_update_p(p, r, cgsc, n, st, nprob, prec, active, s));
+        return FAME_OK;
+    };
+    if (!use_graphs) return launch(stream);
+    const auto key = std::make_tuple(prec, p0, nprob, (const void*)x);
+    auto itg = cg_graphs.find(key);
+    if (itg == cg_graphs.end()) {
+        if (!cap_stream) HIPCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+        const long long c64 = cnt_qbq64, c32 = cnt_qbq32;
+        fame_status q = launch(cap_stream);
+        cnt_qbq64 = c64; cnt_qbq32 = c32;
+        hipGraph_t gr = nullptr;
+        hipError_t e = hipStreamEndCapture(cap_stream, &gr);
+        if (q != FAME_OK) { if (gr) (void)hipGraphDestroy(gr); return q; }
+        HIPCHK(e);
+        hipGraphExec_t ex = nullptr;
+        e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e);
+        itg = cg_graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(itg->second, stream));
+    if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
     return FAME_OK;
 }
 

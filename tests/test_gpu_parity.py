@@ -142,3 +142,44 @@ def test_operators_default_engine_medium(gpu, sname, nx):
     # the FP32 operator against the FP64 oracle: single-precision rounding only
     assert _rel(S.apply(F.OP_QBQ32, 0, xd).cpu().numpy(), t.qbq(invB, x)) < 2e-6
     S.close()
+
+
+def test_graph_replay_matches_eager(gpu):
+    """The CG iteration replayed from a captured hipGraph gives bit-identical
+    results to eager launches: several solves in a row (each a fresh replay
+    sequence with host synchronisation and eager work in between), and a full
+    batched eigensolve in both precisions."""
+    import torch
+    lo, fl, ol, eps = make("fcc")
+    ks = np.stack([KPTS["k"], KPTS["gamma"]])
+    out = {}
+    for graphs in ("1", "0"):
+        # the autotuner is off so that both contexts use the same tiles
+        saved = {k: os.environ.get(k) for k in ("FAME_GRAPH", "FAME_AUTOTUNE")}
+        os.environ["FAME_GRAPH"] = graphs
+        os.environ["FAME_AUTOTUNE"] = "0"
+        try:
+            S = F.Solver(fl, max_batch=2)
+        finally:
+            for k, v in saved.items():
+                if v is None:
+                    del os.environ[k]
+                else:
+                    os.environ[k] = v
+        S.set_eps(eps)
+        S.set_wave_vectors(ks)
+        res = []
+        for seed in range(3):
+            b = torch.tensor(_rand(2 * S.nd[0], 10 + seed) * 1e-2, device=gpu)
+            x, it = S.cg(0, b, maxit=500, tol=1e-12)
+            res.append((x.cpu().numpy(), it))
+            torch.cuda.synchronize()
+            _ = (b * 2).sum().item()                      # eager work between replays
+        f64 = S.eigs(6, 20, precision=F.PREC_F64)[0]
+        f32 = S.eigs(6, 20, precision=F.PREC_F32_REFINE)[0]
+        out[graphs] = (res, f64, f32)
+        S.close()
+    for (xg, ig), (xe, ie) in zip(out["1"][0], out["0"][0]):
+        assert ig == ie and np.array_equal(xg, xe)
+    assert np.array_equal(out["1"][1], out["0"][1])
+    assert np.array_equal(out["1"][2], out["0"][2])
