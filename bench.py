@@ -75,6 +75,24 @@ def cpu_baseline(fl, eps, k, qbq_per_k, seconds):
 
 
 COPY_BYTES = 1 << 30
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_fcc96_30k.json")
+
+
+def measured_traffic(kernel_key, tiles, grid, nk):
+    """HBM bytes per launch of a pass kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the 1 GiB streaming-kernel
+    calibration), used when this run's tuned tiles are the profiled ones."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None, "no traffic profile"
+    want = ";".join(f"{p}:" + ",".join(str(v) for name in ("inner", "mid", "outer") for v in tiles[p][name])
+                    for p in ("fp32", "fp64"))
+    if prof.get("tiles") != want or list(prof.get("grid", [])) != list(grid) or prof.get("nk") != nk:
+        return None, f"profiled tiles {prof.get('tiles')} differ from this run's {want}"
+    k = prof["kernels"].get(kernel_key)
+    return (k["traffic_bytes"], "rocprofv3 FETCH_SIZE+WRITE_SIZE, scaled") if k else (None, "kernel not profiled")
 
 
 def passes_only(F, fl, eps, ks, args, torch, dev):
@@ -84,7 +102,15 @@ def passes_only(F, fl, eps, ks, args, torch, dev):
     S = F.Solver(fl, max_batch=args.nk)
     S.set_eps(eps)
     S.set_wave_vectors(ks)
-    tune = {"fp32": S.autotune(4), "fp64": S.autotune(8)}
+    if args.tiles:
+        tune = {}
+        for part in args.tiles.split(";"):
+            key, vals = part.split(":")
+            v = [int(t) for t in vals.split(",")]
+            tune[key] = {"inner": v[0:2], "mid": v[2:4], "outer": v[4:6]}
+            S.set_tiles(4 if key == "fp32" else 8, tune[key])
+    else:
+        tune = {"fp32": S.autotune(4), "fp64": S.autotune(8)}
     us32 = S.time_qbq_passes(4, args.reps)
     us64 = S.time_qbq_passes(8, args.reps)
     S.close()
@@ -118,6 +144,9 @@ def main():
                     help="only time the five QBQ passes on the batch (FP32 and FP64) and one plain copy of "
                          "known size (the reference kernel for HBM counter scaling); for profiler runs")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--tiles", type=str, default=None,
+                    help="with --passes-only: fixed tiles instead of tuning, 'fp32:e,L,e,L,e,L;fp64:e,L,e,L,e,L' "
+                         "(engine, lines per block of the inner, middle and outer passes, as bench.py reports them)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver shards over wave vectors with no communication, "
@@ -163,6 +192,7 @@ def main():
     dom = int(np.argmax(tot32))
     alg_bytes = bytes_pass(s32)[dom] * args.nk
     achieved = alg_bytes / (us32[dom] * 1e-6) / 1e9
+    traffic, traffic_note = measured_traffic(f"k_{names[dom]}_fp32", tiles, list(fl.grid_nums), args.nk)
     qbq_alg = (10 * N * s32 + 3 * N * r32) * args.nk
     qbq_gbs = qbq_alg / (us32[5] * 1e-6) / 1e9
     S.close()
@@ -233,7 +263,7 @@ def main():
         "tuned_tiles": tiles,
         "roofline": {"bound": "hbm", "kernel": f"k_{names[dom]} (FP32, {args.nk} problems per launch)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "avg_launch_us": us32[dom], "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }

@@ -677,6 +677,24 @@ fame_status fame_ctx::tune(int prec) {
     return FAME_OK;
 }
 
+extern "C" fame_status fame_set_tiles(fame_ctx* c, int prec, const int* choice) {
+    if (!c || !choice || (prec != 8 && prec != 4)) FAIL(FAME_ERR_ARG, "fame_set_tiles: bad arguments");
+    const int pi = prec == 8 ? 0 : 1;
+    for (int a = 0; a < 3; ++a) {
+        const int eng = choice[2 * a], L = choice[2 * a + 1];
+        if (eng < 0 || eng > 1 || c->ax[a][eng].n == 0 || !tile_ok(c->ax[a][eng], a, prec, L, c->g))
+            FAIL(FAME_ERR_ARG, "fame_set_tiles: engine or lines per block not available for pass " + std::to_string(a));
+    }
+    for (int a = 0; a < 3; ++a) {
+        PassTile& t = c->tile[a][pi];
+        t.eng = choice[2 * a]; t.L = choice[2 * a + 1]; t.P = c->ax[a][t.eng].M | 1;
+    }
+    c->nblk_in[pi] = (c->g.nb * c->g.nc + c->tile[AX_INNER][pi].L - 1) / c->tile[AX_INNER][pi].L;
+    c->tuned[pi] = true;
+    c->drop_graphs();
+    return FAME_OK;
+}
+
 extern "C" fame_status fame_autotune(fame_ctx* c, int prec, int* choice) {
     if (!c || (prec != 8 && prec != 4)) FAIL(FAME_ERR_ARG, "fame_autotune: bad arguments");
     HIPCHK(hipSetDevice(c->device));

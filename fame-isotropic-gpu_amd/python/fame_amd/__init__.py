@@ -371,6 +371,11 @@ class Solver:
         _check(lib().fame_autotune(self._ctx, prec_bytes, _ptr(ch)), "fame_autotune")
         return {name: (int(ch[2 * i]), int(ch[2 * i + 1])) for i, name in enumerate(("inner", "mid", "outer"))}
 
+    def set_tiles(self, prec_bytes: int, tiles):
+        """Fix the pass tiles: {pass: (engine, lines)} as returned by autotune."""
+        ch = np.array([v for name in ("inner", "mid", "outer") for v in tiles[name]], np.int32)
+        _check(lib().fame_set_tiles(self._ctx, prec_bytes, _ptr(ch)), "fame_set_tiles")
+
     def time_qbq_passes(self, prec_bytes: int = 8, reps: int = 20):
         """hipEvent-timed average microseconds of the 5 QBQ passes and one QBQ (all wave vectors)."""
         us = np.zeros(6)

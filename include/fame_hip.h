@@ -158,6 +158,12 @@ fame_status fame_time_qbq_passes(fame_ctx* ctx, int prec, int reps, double* us6)
  * cuFFT plans of FAME_Create_cublas (FAME_Main/FAME_Create_cublas.cu:16-25).  */
 fame_status fame_autotune(fame_ctx* ctx, int prec, int* choice);
 
+/* Set the pass tiles of precision prec explicitly, in the format of
+ * fame_autotune's choice (e.g. to reproduce a tuned configuration); marks the
+ * precision as tuned.  FAME_ERR_ARG when an engine or line count is not
+ * available for this grid.  No reference counterpart.                       */
+fame_status fame_set_tiles(fame_ctx* ctx, int prec, const int* choice);
+
 /* Stream used by all launches (a hipStream_t passed as void*; NULL = default). */
 fame_status fame_set_stream(fame_ctx* ctx, void* stream);
 
