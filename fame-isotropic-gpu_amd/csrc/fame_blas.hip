@@ -175,9 +175,10 @@ __global__ void __launch_bounds__(RB) k_cg_update_p(typename V2<T>::t* pv, const
 /* ------------------------------------------------------------ helpers ---- */
 template <typename T>
 __global__ void k_scale_real(typename V2<T>::t* y, const typename V2<T>::t* x, const double* s, size_t n,
-                             size_t xstride, size_t sstride, size_t smod) {
+                             size_t xstride, size_t sstride, size_t smod, const int* mask) {
     typedef typename V2<T>::t T2;
     const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
     for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
         T2 v = x[(size_t)p * xstride + i];
         T f = (T)s[(size_t)p * sstride + (i % smod)];
@@ -187,8 +188,10 @@ __global__ void k_scale_real(typename V2<T>::t* y, const typename V2<T>::t* x, c
 }
 
 template <typename TD, typename TS>
-__global__ void k_convert(typename V2<TD>::t* d, const typename V2<TS>::t* s, size_t n, size_t dstride, size_t sstride) {
+__global__ void k_convert(typename V2<TD>::t* d, const typename V2<TS>::t* s, size_t n, size_t dstride, size_t sstride,
+                          const int* mask) {
     const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
     for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
         typename V2<TS>::t v = s[(size_t)p * sstride + i];
         typename V2<TD>::t w; w.x = (TD)v.x; w.y = (TD)v.y;
@@ -294,60 +297,75 @@ __global__ void k_gemv_n(const double2* V, double2* w, const double* h, int m, s
     }
 }
 
-/* Gram matrix G[j][l] = sum_i conj(V[j][i]) V[l][i] over the upper triangle.
- * A block stages GR rows of all m columns in LDS; each thread owns a fixed set
- * of (j,l) pairs and accumulates over the staged rows.  Partial per block:
- * 2 m^2 doubles (lower triangle left zero).                                  */
-#define GR 32
+/* Gram matrix G[j][l] = sum_i conj(V[j][i]) V[l][i] over the upper triangle,
+ * on the matrix cores (v_mfma_f64_16x16x4f64).  A block stages GK rows of all
+ * m columns (m <= 64, padded to 16-column tiles) in LDS with an odd pitch, and
+ * each wave accumulates a fixed set of upper-triangle 16x16 tile pairs:
+ *   Re G += Vr^T Vr + Vi^T Vi,   Im G += Vr^T Vi - Vi^T Vr.
+ * Operands: lane l holds A[l % 16][l / 16] = conj V_j(k) and B[l / 16][l % 16]
+ * = V_l(k); result element v of lane l is row l / 16 + 4 v, column l % 16.
+ * Partial per block: 2 m^2 doubles (lower triangle zero).                    */
+#define GK 32
+typedef double gacc_t __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials, int m, size_t n, size_t cstride,
                                              size_t pstride, const int* mask) {
     const int p = blockIdx.y;
-    extern __shared__ double2 gtile[];   /* m x GR */
-    const bool on = !mask || mask[p];
-    const double2* Vp = V + (size_t)p * pstride;
-    const int npair = m * (m + 1) / 2;
-    const int PPT = 8;                   /* pairs per thread (m <= 64 -> 2080 pairs <= 8*256) */
-    int pj[PPT], pl[PPT];
-    double accr[PPT], acci[PPT];
-    for (int q = 0; q < PPT; ++q) {
-        int t = threadIdx.x + q * RB;
-        pj[q] = -1; pl[q] = -1; accr[q] = 0; acci[q] = 0;
+    if (mask && !mask[p]) return;
+    __shared__ double2 tile[64 * (GK + 1)];
+    const int mt = (m + 15) >> 4, mp = mt * 16;
+    const int npair = mt * (mt + 1) / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    /* tile pairs of this wave: w, w + 4, w + 8 (at most 10 pairs for m <= 64) */
+    int pj[3], pl[3];
+    for (int q = 0; q < 3; ++q) {
+        int t = w + 4 * q, j = 0;
+        pj[q] = -1; pl[q] = -1;
         if (t < npair) {
-            int j = 0, rem = t;
-            while (rem >= m - j) { rem -= m - j; ++j; }
-            pj[q] = j; pl[q] = j + rem;
+            while (t >= mt - j) { t -= mt - j; ++j; }
+            pj[q] = j; pl[q] = j + t;
         }
     }
-    const size_t chunk = (size_t)gridDim.x * GR;
-    for (size_t base = (size_t)blockIdx.x * GR; on && base < n; base += chunk) {
+    gacc_t re[3], im[3];
+    for (int q = 0; q < 3; ++q) { re[q] = (gacc_t){0, 0, 0, 0}; im[q] = (gacc_t){0, 0, 0, 0}; }
+    const double2* Vp = V + (size_t)p * pstride;
+    const size_t chunk = (size_t)gridDim.x * GK;
+    for (size_t base = (size_t)blockIdx.x * GK; base < n; base += chunk) {
         __syncthreads();
-        for (int t = threadIdx.x; t < m * GR; t += RB) {
-            int j = t / GR, r = t - j * GR;
-            size_t i = base + r;
-            gtile[t] = (i < n) ? Vp[(size_t)j * cstride + i] : make_double2(0, 0);
+        for (int t = threadIdx.x; t < mp * GK; t += RB) {
+            const int j = t / GK, r = t - j * GK;
+            const size_t i = base + r;
+            tile[j * (GK + 1) + r] = (j < m && i < n) ? Vp[(size_t)j * cstride + i] : make_double2(0, 0);
         }
         __syncthreads();
-        for (int q = 0; q < PPT; ++q) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
             if (pj[q] < 0) continue;
-            const double2* a = gtile + pj[q] * GR;
-            const double2* b = gtile + pl[q] * GR;
-            double re = accr[q], im = acci[q];
-#pragma unroll 8
-            for (int r = 0; r < GR; ++r) {
-                double2 x = a[r], y = b[r];
-                re += x.x * y.x + x.y * y.y;
-                im += x.x * y.y - x.y * y.x;
+            const double2* A = tile + (pj[q] * 16 + row) * (GK + 1) + kq;
+            const double2* B = tile + (pl[q] * 16 + row) * (GK + 1) + kq;
+#pragma unroll
+            for (int k0 = 0; k0 < GK; k0 += 4) {
+                const double2 a = A[k0], bb = B[k0];
+                re[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bb.x, re[q], 0, 0, 0);
+                re[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bb.y, re[q], 0, 0, 0);
+                im[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bb.y, im[q], 0, 0, 0);
+                im[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, bb.x, im[q], 0, 0, 0);
             }
-            accr[q] = re; acci[q] = im;
         }
     }
     double* o = partials + ((size_t)p * gridDim.x + blockIdx.x) * 2 * m * m;
     for (int t = threadIdx.x; t < 2 * m * m; t += RB) o[t] = 0;
     __syncthreads();
-    for (int q = 0; q < PPT; ++q) {
+    for (int q = 0; q < 3; ++q) {
         if (pj[q] < 0) continue;
-        o[2 * (pj[q] * m + pl[q])] = accr[q];
-        o[2 * (pj[q] * m + pl[q]) + 1] = acci[q];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int j = pj[q] * 16 + kq + 4 * v, l = pl[q] * 16 + row;
+            if (j < m && l < m && j <= l) {
+                o[2 * (j * m + l)] = re[q][v];
+                o[2 * (j * m + l) + 1] = im[q][v];
+            }
+        }
     }
 }
 
@@ -435,12 +453,12 @@ hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, 
 }
 
 hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
-                               size_t smod, int nprob, int prec, hipStream_t s) {
+                               size_t smod, int nprob, int prec, hipStream_t s, const int* mask) {
     dim3 g(1024, nprob);
     if (prec == 8)
-        hipLaunchKernelGGL(k_scale_real<double>, g, dim3(RB), 0, s, (double2*)y, (const double2*)x, sreal, n, xstride, sstride, smod);
+        hipLaunchKernelGGL(k_scale_real<double>, g, dim3(RB), 0, s, (double2*)y, (const double2*)x, sreal, n, xstride, sstride, smod, mask);
     else
-        hipLaunchKernelGGL(k_scale_real<float>, g, dim3(RB), 0, s, (float2*)y, (const float2*)x, sreal, n, xstride, sstride, smod);
+        hipLaunchKernelGGL(k_scale_real<float>, g, dim3(RB), 0, s, (float2*)y, (const float2*)x, sreal, n, xstride, sstride, smod, mask);
     return hipGetLastError();
 }
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
@@ -448,16 +466,16 @@ hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n
     return fame_scale_real_mod(y, x, sreal, n, xstride, sstride, n, nprob, prec, s);
 }
 hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t n, size_t dstride, size_t sstride,
-                        int nprob, hipStream_t s) {
+                        int nprob, hipStream_t s, const int* mask) {
     dim3 g(1024, nprob);
     if (dprec == 4 && sprec == 8)
-        hipLaunchKernelGGL((k_convert<float, double>), g, dim3(RB), 0, s, (float2*)dst, (const double2*)src, n, dstride, sstride);
+        hipLaunchKernelGGL((k_convert<float, double>), g, dim3(RB), 0, s, (float2*)dst, (const double2*)src, n, dstride, sstride, mask);
     else if (dprec == 8 && sprec == 4)
-        hipLaunchKernelGGL((k_convert<double, float>), g, dim3(RB), 0, s, (double2*)dst, (const float2*)src, n, dstride, sstride);
+        hipLaunchKernelGGL((k_convert<double, float>), g, dim3(RB), 0, s, (double2*)dst, (const float2*)src, n, dstride, sstride, mask);
     else if (dprec == 8)
-        hipLaunchKernelGGL((k_convert<double, double>), g, dim3(RB), 0, s, (double2*)dst, (const double2*)src, n, dstride, sstride);
+        hipLaunchKernelGGL((k_convert<double, double>), g, dim3(RB), 0, s, (double2*)dst, (const double2*)src, n, dstride, sstride, mask);
     else
-        hipLaunchKernelGGL((k_convert<float, float>), g, dim3(RB), 0, s, (float2*)dst, (const float2*)src, n, dstride, sstride);
+        hipLaunchKernelGGL((k_convert<float, float>), g, dim3(RB), 0, s, (float2*)dst, (const float2*)src, n, dstride, sstride, mask);
     return hipGetLastError();
 }
 hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
@@ -501,8 +519,8 @@ hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n,
 }
 hipError_t fame_gram(const void* V, double* partials, int nblk, int m, size_t n, size_t cstride, size_t pstride, int nprob,
                      const int* mask, hipStream_t s) {
-    size_t lds = sizeof(double2) * m * GR;
-    hipLaunchKernelGGL(k_gram, dim3(nblk, nprob), dim3(RB), lds, s, (const double2*)V, partials, m, n, cstride, pstride, mask);
+    if (m > 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gram, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, partials, m, n, cstride, pstride, mask);
     return hipGetLastError();
 }
 hipError_t fame_gemm_vq(const void* V, const double* Q, void* W, int m, int q, size_t n, size_t cstride, size_t pstride,

@@ -83,8 +83,9 @@ hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, 
 /* generic helpers (complex double unless prec says float) */
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
                            int nprob, int prec, hipStream_t s);
+/* mask (device, optional): problems with mask 0 are left untouched */
 hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t n, size_t dstride, size_t sstride,
-                        int nprob, hipStream_t s);
+                        int nprob, hipStream_t s, const int* mask = nullptr);
 hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
                            int nprob, const int* mask, hipStream_t s);
 hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
@@ -117,6 +118,6 @@ hipError_t fame_lz_second_pass_mask(const double* h, int n, double tau2, const i
 hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* invb, int nprob, const int* mask,
                         hipStream_t s);
 hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
-                               size_t smod, int nprob, int prec, hipStream_t s);
+                               size_t smod, int nprob, int prec, hipStream_t s, const int* mask = nullptr);
 hipError_t fame_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
                               int nprob, int* active, const int* mask, hipStream_t s);

@@ -43,7 +43,8 @@ int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double*
 void fame_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s, double shift, int n);
 
 #define GEMV_BLK 256
-#define GRAM_BLK 64
+/* Gram blocks per problem: enough to fill the GPU for small batches */
+static int gram_blk(int nk) { return std::max(32, std::min(512, 2048 / std::max(nk, 1))); }
 
 namespace {
 
@@ -86,7 +87,7 @@ fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
     CHK(al((void**)&b.hh, sizeof(double) * 2 * m * nk));
     CHK(al((void**)&b.nrm, sizeof(double) * nk));
     CHK(al((void**)&b.invb, sizeof(double) * nk));
-    size_t gp = std::max((size_t)GEMV_BLK * 2 * m, (size_t)GRAM_BLK * 2 * m * m);
+    size_t gp = std::max((size_t)GEMV_BLK * 2 * m, (size_t)gram_blk(nk) * 2 * m * m);
     CHK(al((void**)&b.gpart, sizeof(double) * gp * nk));
     CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
     CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
@@ -109,7 +110,9 @@ static fame_status lz_invar(fame_ctx* c, LzBuffers& b, int j, int mode, const fa
     HIPCHK(fame_convert(xin, 8, col(b, j), 8, b.n2, b.n2, ps, b.nk, c->stream));
     std::vector<int> it(b.nk, 0);
     CHK(c->invar(mode, xin, yout, 0, b.nk, mask_dev, ls, it.data()));
-    HIPCHK(fame_convert(col(b, j + 1), 8, yout, 8, b.n2, ps, b.n2, b.nk, c->stream));
+    /* only the problems still iterating: a converged problem keeps its basis,
+     * whose columns its final Ritz vectors are built from */
+    HIPCHK(fame_convert(col(b, j + 1), 8, yout, 8, b.n2, ps, b.n2, b.nk, c->stream, mask_dev));
     for (int p = 0; p < b.nk; ++p) if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; }
     return FAME_OK;
 }
@@ -126,7 +129,7 @@ static fame_status lz_three_term(fame_ctx* c, LzBuffers& b, int j, const int* ma
     HIPCHK(fame_norm2_partials(col(b, j + 1), 8, b.gpart, GEMV_BLK, b.n2, ps, b.nk, c->stream));
     HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 1, b.nrm, b.nk, c->stream));
     HIPCHK(fame_lz_beta(b.nrm, b.T1d, ld, j, b.invb, b.nk, mask_dev, c->stream));
-    HIPCHK(fame_scale_real_mod(col(b, j + 1), col(b, j + 1), b.invb, b.n2, ps, 1, 1, b.nk, 8, c->stream));
+    HIPCHK(fame_scale_real_mod(col(b, j + 1), col(b, j + 1), b.invb, b.n2, ps, 1, 1, b.nk, 8, c->stream, mask_dev));
     return FAME_OK;
 }
 
@@ -240,8 +243,8 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
         /* Gram matrices of the bases that restart */
         HIPCHK(hipMemcpyAsync(b.mask, rmask.data(), sizeof(int) * nk, hipMemcpyHostToDevice, c->stream));
         const size_t ps = b.n2 * m;
-        HIPCHK(fame_gram(b.U, b.gpart, GRAM_BLK, m, b.n2, b.n2, ps, nk, b.mask, c->stream));
-        HIPCHK(fame_reduce_partials(b.gpart, GRAM_BLK, 2 * m * m, b.G, nk, c->stream));
+        HIPCHK(fame_gram(b.U, b.gpart, gram_blk(nk), m, b.n2, b.n2, ps, nk, b.mask, c->stream));
+        HIPCHK(fame_reduce_partials(b.gpart, gram_blk(nk), 2 * m * m, b.G, nk, c->stream));
         HIPCHK(hipMemcpyAsync(G.data(), b.G, sizeof(double) * 2 * m * m * nk, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int p = 0; p < nk; ++p) {

@@ -95,6 +95,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise FameError(f"native library missing: {LIB_PATH} (run __graft_entry__.build())")
+    # PyTorch ships its own HIP runtime; load it first so that this library and
+    # torch share one runtime (the library's libamdhip64 dependency then
+    # resolves to the copy already in the process)
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     L.fame_create.argtypes = [C.POINTER(fame_lattice), i, i, C.POINTER(vp)]

@@ -183,3 +183,32 @@ def test_graph_replay_matches_eager(gpu):
         assert ig == ie and np.array_equal(xg, xe)
     assert np.array_equal(out["1"][1], out["0"][1])
     assert np.array_equal(out["1"][2], out["0"][2])
+
+
+def test_batch_with_uneven_convergence(gpu):
+    """Problems of one batch that converge at different restarts keep their own
+    Lanczos bases: every band's field must satisfy the eigen-equation, and the
+    batched frequencies must equal one-at-a-time solves."""
+    lo, fl, ol, eps = make("fcc")
+    ks = np.array([[0.21, -0.13, 0.07], [0.0, 0.0, 0.0], [0.4, 0.1, 0.2], [0.02, 0.01, -0.01],
+                   [0.33, 0.33, 0.0], [-0.15, 0.27, 0.4]])
+    nwant, nstep = 8, 14
+    S = F.Solver(fl, max_batch=len(ks))
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    freq, _, E, prof = S.eigs(nwant, nstep, want_fields=True)
+    restarts = [p["restarts"] for p in prof]
+    assert len(set(restarts)) > 1, restarts          # the scenario this test is about
+    res = S.residuals(freq, E)
+    for i in range(len(ks)):
+        nz = freq[i] > 0
+        assert np.all(res[i][nz] < 1e-6 * freq[i][nz] ** 2), (i, restarts, res[i])
+    one = F.Solver(fl, max_batch=1)
+    one.set_eps(eps)
+    for i in range(len(ks)):
+        one.set_wave_vectors(ks[i:i + 1])
+        f1 = one.eigs(nwant, nstep)[0][0]
+        nz = f1 > 0
+        assert np.max(np.abs(f1[nz] ** 2 - freq[i][nz] ** 2) / f1[nz] ** 2) < 1e-9
+    one.close()
+    S.close()
