@@ -27,6 +27,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <thread>
 
 using namespace fame;
 
@@ -186,6 +187,47 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
 extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                                  void* evr_dev, void* E_dev, fame_profile* prof) {
     return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
+}
+
+/* Several structures (contexts of any grid, batch and band count) solved in
+ * one call: each context runs fame_eigs from its own host thread on its own
+ * non-blocking stream (created here when the context has none), so that the
+ * kernels of different structures overlap on the GPU. */
+extern "C" fame_status fame_eigs_many(int n, fame_ctx* const* ctxs, const int* precision, const fame_es* es,
+                                      const fame_ls* ls, double* const* freq, void* const* evr_dev,
+                                      void* const* E_dev, fame_profile* const* prof) {
+    if (n < 1 || !ctxs || !precision || !es || !ls || !freq) FAIL(FAME_ERR_ARG, "fame_eigs_many: bad arguments");
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < i; ++j)
+            if (ctxs[i] == ctxs[j] || !ctxs[i]) FAIL(FAME_ERR_ARG, "fame_eigs_many: contexts must be distinct and non-null");
+    std::vector<fame_status> st(n, FAME_OK);
+    std::vector<std::string> err(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (int i = 0; i < n; ++i) {
+        th.emplace_back([&, i]() {
+            fame_ctx* c = ctxs[i];
+            hipStream_t own = nullptr;
+            if (!c->stream) {
+                if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&own, hipStreamNonBlocking) != hipSuccess) {
+                    st[i] = FAME_ERR_HIP; err[i] = "stream creation failed"; return;
+                }
+                fame_set_stream(c, own);
+            }
+            st[i] = fame_eigs(c, precision[i], &es[i], &ls[i], freq[i], evr_dev ? evr_dev[i] : nullptr,
+                              E_dev ? E_dev[i] : nullptr, prof ? prof[i] : nullptr);
+            if (st[i] != FAME_OK) err[i] = fame_last_error();
+            if (own) {
+                (void)hipStreamSynchronize(own);
+                fame_set_stream(c, nullptr);
+                (void)hipStreamDestroy(own);
+            }
+        });
+    }
+    for (auto& t : th) t.join();
+    for (int i = 0; i < n; ++i)
+        if (st[i] != FAME_OK) FAIL(st[i], "fame_eigs_many: structure " + std::to_string(i) + ": " + err[i]);
+    return FAME_OK;
 }
 
 fame_status fame_restore_impl(fame_ctx* c, int nwant, const double2* ev_padded, size_t ev_pstride, void* E_dev);

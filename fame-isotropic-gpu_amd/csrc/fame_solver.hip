@@ -406,6 +406,9 @@ extern "C" fame_status fame_set_eps(fame_ctx* c, const double* eps) {
                     size_t q = comp * N + int_index(c->g, x, y, z);
                     e[q] = v; ib[q] = 1.0 / v; ib32[q] = (float)(1.0 / v);
                 }
+    /* the context's stream may still be running kernels that read the old values */
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->eps_int, e.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->eps_ref, eps, sizeof(double) * 3 * N, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->invB64, ib.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
@@ -455,6 +458,7 @@ extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* 
         c->nd[i] = c->prob_h[i].gamma ? c->N - 1 : c->N;
         if (nd_out) nd_out[i] = c->nd[i];
     }
+    HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(c->prob_d, c->prob_h.data(), sizeof(ProbDev) * nk, hipMemcpyHostToDevice));
     HIPCHK(fame_launch_sigma(c->g, c->prob_d, c->sig, c->sig_inv, nk, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -734,7 +738,8 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
         chunk = 2;
     }
     std::vector<CgScal> sc(nprob);
-    HIPCHK(hipMemcpy(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
     int mx = 0;
     for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc[i].iters; mx = std::max(mx, sc[i].iters); }
     cg_predict[pix(prec)] = std::max(2, mx - 1);
@@ -826,7 +831,10 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
         HIPCHK(hipMemsetAsync(xr, 0, 16 * st * nprob, stream));
         HIPCHK(hipMemcpyAsync(rr, b, 16 * st * nprob, hipMemcpyDeviceToDevice, stream));
         std::vector<int> msk(nprob, 1);
-        if (mask_dev) HIPCHK(hipMemcpy(msk.data(), mask_dev, sizeof(int) * nprob, hipMemcpyDeviceToHost));
+        if (mask_dev) {
+            HIPCHK(hipMemcpyAsync(msk.data(), mask_dev, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
         double tol2 = ls.tol * ls.tol;
         for (int sweep = 0; sweep < refine_max_sweeps; ++sweep) {
             HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));

@@ -403,6 +403,28 @@ class Solver:
 OP_INVAR_MIXED, OP_QBQ32 = 6, 7
 
 
+def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxit=1000, precision=PREC_F64):
+    """Solve several structures (Solvers of any grid and batch) in one call;
+    nwant / nstep / precision may be per-solver lists.  Returns [(freq, profs)]."""
+    n = len(solvers)
+    per = lambda v: list(v) if isinstance(v, (list, tuple)) else [v] * n
+    nw, ns, pr = per(nwant), per(nstep), per(precision)
+    ctxs = (C.c_void_p * n)(*[s._ctx.value for s in solvers])
+    esa = (fame_es * n)(*[fame_es(nw[i], ns[i], maxit, tol) for i in range(n)])
+    lsa = (fame_ls * n)(*[fame_ls(ls_maxit, ls_tol) for _ in range(n)])
+    prec = (C.c_int * n)(*pr)
+    freqs = [np.zeros((len(s.nd), nw[i])) for i, s in enumerate(solvers)]
+    profs = [(fame_profile * len(s.nd))() for s in solvers]
+    fp = (C.c_void_p * n)(*[f.ctypes.data for f in freqs])
+    pp = (C.c_void_p * n)(*[C.addressof(p) for p in profs])
+    _check(lib().fame_eigs_many(n, ctxs, prec, esa, lsa, fp, None, None, pp), "fame_eigs_many")
+    out = []
+    for f, ps in zip(freqs, profs):
+        out.append((f, [dict(ls_iter=p.ls_iter, es_iter=p.es_iter, restarts=p.restarts, nd=p.nd, converged=p.converged,
+                             qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time) for p in ps]))
+    return out
+
+
 # --------------------------------------------------------------------------- adjoint
 def _make_eig_function():
     torch = _torch()

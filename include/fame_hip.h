@@ -125,6 +125,15 @@ fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fam
  * caller (the adjoint) keeps 2N instead of 3N complex values per band.      */
 fame_status fame_restore(fame_ctx* ctx, int nwant, const void* evr_dev, void* E_dev);
 
+/* n structures in one call (uneven grids, batch sizes and band counts): arrays
+ * of n contexts and per-structure arguments as for fame_eigs (evr_dev, E_dev
+ * and prof may be NULL, or hold NULL entries).  The contexts run concurrently,
+ * each on its own stream.  Replaces the reference's one-structure-per-run
+ * FAME_Main_Code loop (FAME_Main/FAME_Main_Code.cu:110) for batches of
+ * structures; no direct counterpart.                                         */
+fame_status fame_eigs_many(int n, fame_ctx* const* ctxs, const int* precision, const fame_es* es, const fame_ls* ls,
+                           double* const* freq, void* const* evr_dev, void* const* E_dev, fame_profile* const* prof);
+
 /* Relative residuals ||C^H C E - omega^2 B E|| / ||E|| of fields produced by
  * fame_eigs (device, nk*nwant*3N) -> res (host, nk*nwant).  GPU counterpart of
  * Check_Residual (FAME_Main/FAME_Main_Code.cu:383), reported relative.       */

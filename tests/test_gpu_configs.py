@@ -156,3 +156,34 @@ def test_c4_sc64_adjoint_fd(gpu):
         an = float(g[c])
         assert abs(fd - an) <= 2e-3 * abs(an), (int(c), fd, an)
     S.close()
+
+
+def test_c5_uneven_batch(gpu):
+    """C5 at reduced size: seeded structures of different lattices, grids,
+    sphere counts, permittivities and band counts solved concurrently in one
+    fame_eigs_many call, against the same structures solved one after another."""
+    rng = np.random.default_rng(5)
+    lattices = [("simple_cubic", {}), ("face_centered_cubic", {}), ("body_centered_cubic", {}), ("hexagonal", {"c": 1.2})]
+    solvers, nw, ns, ks_all, eps_all, fls = [], [], [], [], [], []
+    for i in range(6):
+        lt, kw = lattices[i % 4]
+        nx = int(rng.choice([16, 20, 24]))
+        lo = F.build_lattice(lt, nx, **kw)
+        fl = F.to_fame_lattice(lo)
+        nsph = int(rng.integers(1, 4))
+        mat = F.Material(sphere_centers=[[tuple(c) for c in rng.random((nsph, 3))]],
+                         sphere_radius=[list(rng.uniform(0.08, 0.2, nsph))])
+        eps = F.material_eps(lo, mat, [float(rng.uniform(4, 13))], 1.0)
+        ks = rng.uniform(-0.4, 0.4, (3, 3))
+        S = F.Solver(fl, max_batch=3)
+        S.set_eps(eps)
+        S.set_wave_vectors(ks)
+        w = int(rng.integers(6, 13))
+        solvers.append(S); nw.append(w); ns.append(min(3 * w, 40)); ks_all.append(ks); eps_all.append(eps); fls.append(fl)
+    out = F.eigs_many(solvers, nw, ns)
+    for i, (f, prof) in enumerate(out):
+        assert all(p["converged"] for p in prof)
+        ref = solvers[i].eigs(nw[i], ns[i])[0]
+        assert _rel_err(f ** 2, ref ** 2) < 1e-9, i
+    for S in solvers:
+        S.close()
