@@ -116,16 +116,19 @@ def passes_only(F, fl, eps, ks, args, torch, dev):
     S.close()
     a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
     b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    torch.neg(a, out=b)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(args.reps):
         torch.neg(a, out=b)
-    torch.cuda.synchronize()
-    cp_us = (time.perf_counter() - t0) / args.reps * 1e6
+    e1.record()
+    e1.synchronize()
+    cp_us = e0.elapsed_time(e1) / args.reps * 1e3
     names = ["inner_expand", "mid_inverse", "outer_eps", "mid_forward", "inner_contract", "qbq"]
     print(json.dumps({"grid": list(fl.grid_nums), "nk": args.nk,
                       "fp32_us": dict(zip(names, map(float, us32))), "fp64_us": dict(zip(names, map(float, us64))),
-                      "copy_bytes": COPY_BYTES, "copy_us": cp_us, "tuned_tiles": tune}))
+                      "copy_bytes": COPY_BYTES, "copy_us": cp_us, "copy_gbs": 2 * COPY_BYTES / cp_us / 1e3,
+                      "tuned_tiles": tune}))
 
 
 def main():

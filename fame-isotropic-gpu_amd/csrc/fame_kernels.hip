@@ -75,7 +75,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         }
     }
     __syncthreads();
-    const size_t N = (size_t)g.N;
+    /* per-problem base pointers (uniform) with 32-bit element offsets below
+     * (3N < 2^31 is checked at context creation) */
+    const unsigned N = (unsigned)g.N;
     const T2* X = reinterpret_cast<const T2*>(args.x) + (size_t)p * args.xstride;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride;
     const double inv_na = 1.0 / (double)na;
@@ -89,7 +91,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             const int gl = l0 + line;
             double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
             if (gl < nlines && a < na) {
-                const size_t f = (size_t)gl * na + a;
+                const unsigned f = (unsigned)(gl * na + a);
                 if (!(pr.gamma && f == 0)) {
                     double2 lx, ly, lz, p1[3], p2[3];
                     lambdas_line(g, lsym[line], a, lx, ly, lz);
@@ -112,7 +114,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             const int gl = l0 + line;
             if (gl >= nlines) continue;
             const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
-            const size_t f = (size_t)gl * na + a;
+            const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
                 F[c * N + f] = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w);
@@ -126,7 +128,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             for (int c = 0; c < 3; ++c) { v[c].x = 0; v[c].y = 0; }
             if (gl < nlines && a < na) {
                 const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
-                const size_t f = (size_t)gl * na + a;
+                const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + f], w);
             }
@@ -142,7 +144,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             const int line = udiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
-            const size_t f = (size_t)gl * na + a;
+            const unsigned f = (unsigned)(gl * na + a);
             double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
             if (!(pr.gamma && f == 0)) {
                 double2 lx, ly, lz, p1[3], p2[3];
@@ -213,6 +215,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     T2* S = twb + nb;
     const int cf = tile / ntile, a0 = (tile - cf * ntile) * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
+    /* 32-bit element offsets from the uniform plane pointer */
     const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)nb;
     for (int b = threadIdx.x; b < nb; b += FFT_THREADS) twb[b] = tw_expipi<T2>(phc * (double)b);
     __syncthreads();
@@ -222,7 +225,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
         T2 v;
         v.x = 0; v.y = 0;
         if (a < na && b < nb) {
-            v = F[(size_t)b * na + a];
+            v = F[(unsigned)(b * na + a)];
             if (DIR < 0) v = cmulc(v, twb[b]);
         }
         S[ta * P + b] = v;
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
         if (a >= na) continue;
         T2 v = post_mul<T2, T>(pl, tb, b, S[ta * P + b]);
         if (DIR > 0) v = cmul(v, twb[b]);
-        F[(size_t)b * na + a] = v;
+        F[(unsigned)(b * na + a)] = v;
     }
 }
 
@@ -272,7 +275,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         T2 v;
         v.x = 0; v.y = 0;
         if (q < plane && c < nc) {
-            v = F[(size_t)c * plane + q];
+            v = F[(unsigned)(c * plane + q)];
             if (MODE == OUTER_QRS) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
         }
         S[tq * P + c] = v;
@@ -285,7 +288,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         auto eps_step = [&](T2* Sx) {
             for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
                 const int c = udiv(idx, mL), tq = idx - c * L, q = q0 + tq;
-                const T s = (q < plane && c < nc) ? invB[(size_t)c * plane + q] * scale : (T)0;
+                const T s = (q < plane && c < nc) ? invB[(unsigned)(c * plane + q)] * scale : (T)0;
                 Sx[tq * P + c] = cscale(Sx[tq * P + c], s);
             }
             __syncthreads();
@@ -300,7 +303,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         if (q >= plane) continue;
         T2 v = post_mul<T2, T>(pl, tb, c, S[tq * P + c]);
         if (MODE == OUTER_QR) v = cmul(v, from_d<T2>(cscale(expipi(2.0 * pr.r0 * (double)c / (double)nc), invN)));
-        F[(size_t)c * plane + q] = v;
+        F[(unsigned)(c * plane + q)] = v;
     }
 }
 

@@ -295,6 +295,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     g.general = lat->general ? 1 : 0;
     if (g.general) { g.na = Nz; g.nb = Ny; g.nc = Nx; g.ha = lat->mesh_len[2]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[0]; }
     else { g.na = Nx; g.nb = Ny; g.nc = Nz; g.ha = lat->mesh_len[0]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[2]; }
+    if ((long long)Nx * Ny * Nz * 3 >= (1LL << 31)) { delete c; FAIL(FAME_ERR_ARG, "grid too large: 3N must stay below 2^31"); }
     g.N = Nx * Ny * Nz;
     g.iha = 1.0 / g.ha; g.ihb = 1.0 / g.hb; g.ihc = 1.0 / g.hc;
     g.m_na = fame_magic32(g.na);
