@@ -319,9 +319,9 @@ __device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& s
  *   X_{n-q} = x_0 + (-1)^q h + A_q - sign i B_q,    q = 0..n/2,
  *   A_q = sum_t cos(2 pi q t / n) s_t,   B_q = sum_t sin(2 pi q t / n) d_t.
  * A and B are two real matrix products (Q x m) x (m x 2 columns per line) run
- * as 16x16x4 MFMAs: wave w owns output rows q = 16w..16w+15 and keeps its cos /
- * sin operands in registers; the s / d columns of 16 lines at a time are staged
- * in the LDS workspace.  About a quarter of the flops of the complex DFT-GEMM. */
+ * as 16x16x4 MFMAs: wave w owns output rows q = 16w..16w+15 and reads its cos /
+ * sin operands from the twiddle table; the s / d columns of 16 lines at a time
+ * are staged in the LDS workspace.  About a quarter of the flops of the complex DFT-GEMM. */
 template <typename T> struct Mfma;
 /* Operands of both 16x16x4 variants: lane l holds A[l % 16][l / 16] and
  * B[l / 16][l % 16].  Result row of accumulator element v in lane l (column

@@ -6,8 +6,9 @@
  * the Yee-edge material mask (FAME_Matrix_B_Isotropic.cu:4-74), the wave-vector
  * loop -- here batched so that many wave vectors share each launch --, the
  * restored fields, the residual check with the reference's absolute 1e-10
- * threshold and sign flag, the PROFILE counters and Data_Omega_array.txt
- * (FAME_Tools/FAME_Create_Frequency_txt.cpp:3-21).
+ * threshold and sign flag, the PROFILE counters, the per-band eigenvector files
+ * when flag.save_eigen_vector is set (FAME_Main_Code.cu:124-127) and
+ * Data_Omega_array.txt (FAME_Tools/FAME_Create_Frequency_txt.cpp:3-21).
  */
 #include "fame_main.h"
 
@@ -43,7 +44,45 @@ fame_lattice lattice_of(const PAR& P) {
     return L;
 }
 
+/* Data_EV_kk_bb.txt of one band (FAME_Tools/FAME_Save_Eigenvector.cpp:7-47):
+ * "Ele_field_mtx{k}(:, b) = [", one "(re) + (im * 1i)" line per entry, "];" */
+bool save_field(const std::vector<double>& E, size_t N3, int k, int b) {
+    char name[64];
+    std::snprintf(name, sizeof(name), "Data_EV_%02d_%02d.txt", k + 1, b + 1);
+    FILE* fp = std::fopen(name, "w");
+    if (!fp) return false;
+    std::fprintf(fp, "Ele_field_mtx{%d}(:, %d) = [\n", k + 1, b + 1);
+    for (size_t j = 0; j < N3; ++j) std::fprintf(fp, "(% e) + (% e * 1i)\n", E[2 * j], E[2 * j + 1]);
+    std::fprintf(fp, "];\n");
+    std::fclose(fp);
+    return true;
+}
+
+/* per-k profile line (FAME_Tools/FAME_Profile.cpp:12-25) */
+void print_profile(const PROFILE& P) {
+    std::printf("= = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = =\n");
+    std::printf("               idx     LS_iter     LS_time     ES_iter     ES_time\n");
+    std::printf("               %3d%12d%12.2f%12d%12.2f\n", P.idx + 1, P.ls_iter[P.idx], P.ls_time[P.idx],
+                P.es_iter[P.idx], P.es_time[P.idx]);
+    std::printf("= = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = = =\n");
+}
+
 }  // namespace
+
+/* Data_Profile.txt (FAME_Tools/FAME_Profile.cpp:27-39): rows 1..idx+1 of the profile */
+extern "C" int FAME_Create_Profile_txt(PROFILE Profile) {
+    if (!Profile.ls_iter || !Profile.es_iter || !Profile.ls_time || !Profile.es_time) return -1;
+    FILE* fp = std::fopen("Data_Profile.txt", "w");
+    if (!fp) return -1;
+    std::fprintf(fp, "Compute_Info = [\n");
+    std::fprintf(fp, "%%idx   LS_iter     LS_time   ES_iter     ES_time\n");
+    for (int i = 0; i <= Profile.idx; i++)
+        std::fprintf(fp, " %3d, %8d, %10.2f, %8d, %10.2f;\n", i + 1, Profile.ls_iter[i], Profile.ls_time[i],
+                     Profile.es_iter[i], Profile.es_time[i]);
+    std::fprintf(fp, "];\n");
+    std::fclose(fp);
+    return 0;
+}
 
 extern "C" int Lanczos_Isotropic(double* Freq_array, void* ev_dev, fame_ctx* ctx, ES es, LS ls, int Nx, int Ny,
                                  int Nz, const char* flag_CompType, PROFILE* Profile) {
@@ -137,13 +176,19 @@ extern "C" int FAME_Main_Code(PAR Par, PROFILE* Profile) {
                 const double res = rel[(size_t)p * Nwant + i] * std::sqrt(nrm);
                 std::printf("                 Freq(%2d) = %10.8f, residual = %e.\n", i, freq[(size_t)p * Nwant + i], res);
                 if (res > 1e-10) freq[(size_t)p * Nwant + i] = -freq[(size_t)p * Nwant + i];
+                if (Par.flag.save_eigen_vector && !save_field(Eh, N3, k0 + p, i)) {
+                    std::fprintf(stderr, "FAME_Main_Code: cannot write the eigenvector files\n");
+                    st = FAME_ERR_ARG; break;
+                }
             }
+            if (st != FAME_OK) break;
             if (Profile && Profile->ls_iter) {
                 Profile->idx = k0 + p;
                 Profile->ls_iter[k0 + p] = prof[p].ls_iter;
                 Profile->es_iter[k0 + p] = prof[p].es_iter;
                 Profile->es_time[k0 + p] = secs / nb;
                 Profile->ls_time[k0 + p] = secs / nb;
+                print_profile(*Profile);
             }
         }
     }

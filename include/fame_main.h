@@ -94,10 +94,16 @@ typedef struct {
 /* Replaces FAME_Main_Code (FAME_Main/FAME_Main_Code.cu:37): all wave vectors of
  * Par.recip_lattice, in batches sized to the free device memory, FP64 (or the
  * mixed-precision solver when FAME_PRECISION=mixed is set in the environment).
- * Writes Data_Omega_array.txt and fills Profile like the reference; a frequency
+ * Writes Data_Omega_array.txt (and Data_EV_kk_bb.txt per band when
+ * Par.flag.save_eigen_vector is set) and fills Profile like the reference; a frequency
  * whose absolute residual (reference field scaling) exceeds 1e-10 is negated
  * (FAME_Main_Code.cu:383-421).  Returns 0, or a fame_status on failure.      */
 int FAME_Main_Code(PAR Par, PROFILE* Profile);
+
+/* Writes Data_Profile.txt for rows 0..Profile.idx, in the format of the
+ * reference's FAME_Create_Profile_txt (FAME_Tools/FAME_Profile.cpp:27-39),
+ * which its Main.cpp calls after FAME_Main_Code.  Returns 0, or -1.        */
+int FAME_Create_Profile_txt(PROFILE Profile);
 
 /* Replaces Lanczos_Isotropic (FAME_Main/Lanczos_Isotropic.cu:7) for the wave
  * vectors already set in ctx: Freq_array (host, nk*nwant) receives omega =
