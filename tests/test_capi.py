@@ -25,8 +25,28 @@ def test_exports_every_declared_symbol():
 
 def test_drop_in_surface_exported():
     lib = C.CDLL(F.LIB_PATH)
-    for s in ("FAME_Main_Code", "Lanczos_Isotropic"):
+    for s in ("FAME_Main_Code", "Lanczos_Isotropic", "FAME_Create_Profile_txt"):
         assert hasattr(lib, s), s
+
+
+def test_profile_txt_format(tmp_path, monkeypatch):
+    """Data_Profile.txt in the reference's format (FAME_Tools/FAME_Profile.cpp:27-39);
+    host-only, no device needed."""
+    class PROFILE(C.Structure):
+        _fields_ = [("idx", C.c_int), ("ls_iter", C.POINTER(C.c_int)), ("es_iter", C.POINTER(C.c_int)),
+                    ("ls_time", C.POINTER(C.c_double)), ("es_time", C.POINTER(C.c_double))]
+    li = np.array([120, 7], np.int32); ei = np.array([33, 4], np.int32)
+    lt = np.array([1.5, 0.25]); et = np.array([2.0, 0.125])
+    pr = PROFILE(1, li.ctypes.data_as(C.POINTER(C.c_int)), ei.ctypes.data_as(C.POINTER(C.c_int)),
+                 lt.ctypes.data_as(C.POINTER(C.c_double)), et.ctypes.data_as(C.POINTER(C.c_double)))
+    lib = C.CDLL(F.LIB_PATH)
+    lib.FAME_Create_Profile_txt.argtypes = [PROFILE]
+    monkeypatch.chdir(tmp_path)
+    assert lib.FAME_Create_Profile_txt(pr) == 0
+    assert open(tmp_path / "Data_Profile.txt").read() == (
+        "Compute_Info = [\n%idx   LS_iter     LS_time   ES_iter     ES_time\n"
+        "   1,      120,       1.50,       33,       2.00;\n"
+        "   2,        7,       0.25,        4,       0.12;\n];\n")
 
 
 def test_no_device_needed_for_errors():
