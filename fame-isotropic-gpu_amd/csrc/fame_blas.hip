@@ -97,7 +97,7 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
     c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
     c.tol2 = tol * tol; c.iters = 1; c.maxit = maxit;
     c.active = (!mask || mask[p]) && (s > c.tol2) && (1 <= maxit);
-    c.pad = 0;
+    c.pend = 0;
     sc[p] = c;
     active[p] = c.active;
 }
@@ -119,13 +119,15 @@ __global__ void k_cg_alpha(CgScal* sc, const double* partials, int nblk, const i
     double temp = re * re + im * im;
     c.alpha_re = c.rr * re / temp;
     c.alpha_im = -c.rr * im / temp;
+    c.pend = 1;
 }
 
+/* r -= alpha Ap with the <r,r> partials.  The matching x += alpha p is deferred
+ * to the next iteration's first QBQ pass, or to k_cg_finish_x.              */
 template <typename T>
-__global__ void __launch_bounds__(RB) k_cg_update_xr(typename V2<T>::t* x, typename V2<T>::t* r,
-                                                     const typename V2<T>::t* pv, const typename V2<T>::t* Ap,
-                                                     const CgScal* sc, double* partials, size_t n, size_t stride,
-                                                     const int* active) {
+__global__ void __launch_bounds__(RB) k_cg_update_r(typename V2<T>::t* r, const typename V2<T>::t* Ap,
+                                                    const CgScal* sc, double* partials, size_t n, size_t stride,
+                                                    const int* active) {
     typedef typename V2<T>::t T2;
     const int p = blockIdx.y;
     T acc = 0;
@@ -133,10 +135,9 @@ __global__ void __launch_bounds__(RB) k_cg_update_xr(typename V2<T>::t* x, typen
         const T ar = (T)sc[p].alpha_re, ai = (T)sc[p].alpha_im;
         const size_t o = (size_t)p * stride;
         for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-            T2 pp = pv[o + i], ap = Ap[o + i], xx = x[o + i], rr = r[o + i];
-            xx.x += ar * pp.x - ai * pp.y; xx.y += ar * pp.y + ai * pp.x;
+            T2 ap = Ap[o + i], rr = r[o + i];
             rr.x -= ar * ap.x - ai * ap.y; rr.y -= ar * ap.y + ai * ap.x;
-            x[o + i] = xx; r[o + i] = rr;
+            r[o + i] = rr;
             acc += rr.x * rr.x + rr.y * rr.y;
         }
     }
@@ -157,18 +158,19 @@ __global__ void k_cg_beta(CgScal* sc, const double* partials, int nblk, int* act
     active[p] = c.active;
 }
 
+/* the solution update x += alpha p still pending at the end of a solve */
 template <typename T>
-__global__ void __launch_bounds__(RB) k_cg_update_p(typename V2<T>::t* pv, const typename V2<T>::t* r, const CgScal* sc,
-                                                    size_t n, size_t stride, const int* active) {
+__global__ void __launch_bounds__(RB) k_cg_finish_x(typename V2<T>::t* x, const typename V2<T>::t* pv,
+                                                    const CgScal* sc, size_t n, size_t stride) {
     typedef typename V2<T>::t T2;
     const int p = blockIdx.y;
-    if (!active[p]) return;
-    const T be = (T)sc[p].beta;
+    if (!sc[p].pend) return;
+    const T ar = (T)sc[p].alpha_re, ai = (T)sc[p].alpha_im;
     const size_t o = (size_t)p * stride;
     for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-        T2 pp = pv[o + i], rr = r[o + i];
-        pp.x = rr.x + be * pp.x; pp.y = rr.y + be * pp.y;
-        pv[o + i] = pp;
+        T2 pp = pv[o + i], xx = x[o + i];
+        xx.x += ar * pp.x - ai * pp.y; xx.y += ar * pp.y + ai * pp.x;
+        x[o + i] = xx;
     }
 }
 
@@ -426,29 +428,29 @@ hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob
     hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
-hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, const CgScal* sc, double* partials, int nblk,
-                             size_t n, size_t stride, int nprob, int prec, const int* active, hipStream_t s) {
+hipError_t fame_cg_update_r(void* r, const void* Ap, const CgScal* sc, double* partials, int nblk, size_t n,
+                            size_t stride, int nprob, int prec, const int* active, hipStream_t s) {
     if (prec == 8)
-        hipLaunchKernelGGL(k_cg_update_xr<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)x, (double2*)r,
-                           (const double2*)p, (const double2*)Ap, sc, partials, n, stride, active);
+        hipLaunchKernelGGL(k_cg_update_r<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)r,
+                           (const double2*)Ap, sc, partials, n, stride, active);
     else
-        hipLaunchKernelGGL(k_cg_update_xr<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x, (float2*)r,
-                           (const float2*)p, (const float2*)Ap, sc, partials, n, stride, active);
+        hipLaunchKernelGGL(k_cg_update_r<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)r,
+                           (const float2*)Ap, sc, partials, n, stride, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
-hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
-                            const int* active, hipStream_t s) {
+hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
+                            hipStream_t s) {
     const int nblk = 1024;
     if (prec == 8)
-        hipLaunchKernelGGL(k_cg_update_p<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)p, (const double2*)r, sc, n,
-                           stride, active);
+        hipLaunchKernelGGL(k_cg_finish_x<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)x, (const double2*)p,
+                           sc, n, stride);
     else
-        hipLaunchKernelGGL(k_cg_update_p<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)p, (const float2*)r, sc, n,
-                           stride, active);
+        hipLaunchKernelGGL(k_cg_finish_x<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x, (const float2*)p, sc,
+                           n, stride);
     return hipGetLastError();
 }
 
@@ -573,7 +575,7 @@ __global__ void k_cg_init_scal2(CgScal* sc, const double* partials, int nblk, do
     c.rr = s; c.rr0 = s; c.pap_re = c.pap_im = 0; c.alpha_re = c.alpha_im = 0; c.beta = 0;
     c.tol2 = fmax(tol * tol, reltol * reltol * s); c.iters = 1; c.maxit = maxit;
     c.active = (!mask || mask[p]) && (s > c.tol2) && (1 <= maxit);
-    c.pad = 0;
+    c.pend = 0;
     sc[p] = c;
     active[p] = c.active;
 }

@@ -515,6 +515,19 @@ __device__ __forceinline__ void lambdas_line(const GridDev& g, const LineSym& s,
     else { lx = La; ly = s.Lb; lz = s.Lc; }
 }
 
+/* lambdas_line with exp(i pi (a_f + phi) / na) formed as the product of the
+ * tabulated ea = exp(i pi a_f / na) and the line's ephi = exp(i pi phi / na):
+ * one complex product instead of a division and a sincospi per mode.  The
+ * product's absolute error is a few ulp, so sin stays accurate to a few ulp
+ * relative to the largest symbol.                                           */
+__device__ __forceinline__ void lambdas_line_tab(const GridDev& g, const LineSym& s, double2 ea, double2 ephi,
+                                                 double2& lx, double2& ly, double2& lz) {
+    const double2 e = cmul(ea, ephi);
+    const double2 La = make_double2(-2.0 * e.y * e.y * g.iha, 2.0 * e.y * e.x * g.iha);
+    if (g.general) { lx = s.Lc; ly = s.Lb; lz = La; }
+    else { lx = La; ly = s.Lb; lz = s.Lc; }
+}
+
 /* Fourier symbols of mode (a_f, b_f, c_f) */
 __device__ __forceinline__ void lambdas(const GridDev& g, const ProbDev& p, int af, int bf, int cf,
                                         double2& lx, double2& ly, double2& lz) {
@@ -526,6 +539,17 @@ __device__ __forceinline__ void lambdas(const GridDev& g, const ProbDev& p, int 
  * (Simple: FAME_Matrix_Lambdas.cu:184-215, General: :441-461).  Both radicands
  * are real and non-negative (Cauchy-Schwarz); reciprocal square roots replace
  * the reference's complex square roots and divisions.                         */
+/* 1/sqrt(x): FP64, or (FAST) the FP32 hardware estimate (about 1 ulp of FP32),
+ * which is all a basis that is rounded to FP32 afterwards needs.  A relative
+ * error d in the norms scales each reduced mode of the FP32 operator by about
+ * (1 + 2d), a perturbation that the FP64 refinement removes at rate ~2d.   */
+template <bool FAST>
+__device__ __forceinline__ double rsq(double x) {
+    if (FAST) return (double)rsqrtf((float)x);
+    return rsqrt(x);
+}
+
+template <bool FAST = false>
 __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 ly, double2 lz,
                                          double2 p1[3], double2 p2[3]) {
     const double Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
@@ -535,8 +559,8 @@ __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 l
         double2 t2 = csub(lx, cscale(lz, al));
         double2 t3 = csub(cscale(ly, al), cscale(lx, be));
         const double n2 = t1.x * t1.x + t1.y * t1.y + t2.x * t2.x + t2.y * t2.y + t3.x * t3.x + t3.y * t3.y;
-        const double i2 = rsqrt(n2);
-        const double i1 = i2 * rsqrt(Lq);
+        const double i2 = rsq<FAST>(n2);
+        const double i1 = i2 * rsq<FAST>(Lq);
         p1[0] = cscale(csub(cmulc(t3, ly), cmulc(t2, lz)), i1);
         p1[1] = cscale(csub(cmulc(t1, lz), cmulc(t3, lx)), i1);
         p1[2] = cscale(csub(cmulc(t2, lx), cmulc(t1, ly)), i1);
@@ -546,8 +570,8 @@ __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 l
     } else {
         const double2 Ls = cadd(cadd(lx, ly), lz);
         const double Lp = Ls.x * Ls.x + Ls.y * Ls.y;
-        const double i1 = rsqrt(3.0 * Lq * Lq - Lq * Lp);
-        const double i2 = rsqrt(3.0 * Lq - Lp);
+        const double i1 = rsq<FAST>(3.0 * Lq * Lq - Lq * Lp);
+        const double i2 = rsq<FAST>(3.0 * Lq - Lp);
         const double2 lxs = cmulc(lx, Ls), lys = cmulc(ly, Ls), lzs = cmulc(lz, Ls);
         p1[0] = make_double2((Lq - lxs.x) * i1, -lxs.y * i1);
         p1[1] = make_double2((Lq - lys.x) * i1, -lys.y * i1);

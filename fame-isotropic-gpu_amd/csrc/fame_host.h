@@ -83,7 +83,8 @@ struct fame_ctx {
     KInner kin(int prec, int p0, const int* act, int list) const;
     KAxis kmid(int prec, int p0, const int* act, int list) const;
     KAxis kout(int prec, int p0, const int* act, int list) const;
-    fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec);
+    fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
+                    void* cg_r = nullptr, void* cg_x = nullptr);
     fame_status qr_field(const void* x, int p0, int nprob, int pikind, const int* act);
     fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
     fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);

@@ -17,6 +17,20 @@ enum { OUTER_EPS = 0, OUTER_QR = 1, OUTER_QRS = 2 };
  * FI   FWD, kernel elementwise step, INV (curl-curl inner pass).             */
 enum { LIST_FWD = 0, LIST_INV = 1, LIST_EPS = 2, LIST_FI = 3, NLIST = 4 };
 
+/* per-problem CG scalars, device resident */
+struct CgScal {
+    double rr;        /* current <r,r>                                         */
+    double rr0;       /* <b,b> of the solve                                    */
+    double pap_re, pap_im;
+    double alpha_re, alpha_im;
+    double beta;
+    double tol2;      /* absolute tolerance squared                           */
+    int iters;        /* reference-style counter k (starts at 1)              */
+    int maxit;
+    int active;       /* 1 while iterating                                      */
+    int pend;         /* 1 when x still lacks alpha p of the last iteration    */
+};
+
 struct KInner {
     fame::GridDev g;
     const fame::ProbDev* prob;   /* device, one per problem                    */
@@ -30,6 +44,14 @@ struct KInner {
     size_t xstride, fstride;
     int L, P;                    /* lines per block, LDS line pitch            */
     int pikind;                  /* 0 Pi_Qr, 1 Pi_Pr                           */
+    /* EXPAND inside CG (cg_r != nullptr): x is the search direction p, and
+     * before it is expanded each element is updated in place,
+     *   x_sol += alpha p  (when pend),   p = r + beta p,
+     * with alpha, beta, pend of the problem's CgScal: the previous iteration's
+     * solution and direction updates ride on this pass's reads.             */
+    void* cg_r;
+    void* cg_x;
+    const CgScal* cgsc;
 };
 
 struct KAxis {
@@ -44,6 +66,14 @@ struct KAxis {
     unsigned mL;                 /* udiv magic for / L                         */
 };
 
+/* inner-axis kernels keep a table of na complex doubles after their lines:
+ * its offset in line entries (rounded to 16 bytes for FP32 entries) and the
+ * entries it adds to the LDS size                                            */
+__host__ __device__ inline int inner_tab_offset(int entries, int prec) {
+    return prec == 8 ? entries : ((entries + 1) & ~1);
+}
+__host__ __device__ inline int inner_tab_extra(int na, int prec) { return prec == 8 ? na : 2 * na + 1; }
+
 /* LDS bytes of a line-transform launch: tables + nl lines + extra entries */
 size_t fame_line_lds(const fame::LinePlanDev& pl, int prec, int nl, int P, int extra);
 
@@ -55,18 +85,6 @@ hipError_t fame_launch_xz_transpose(const void* in, void* out, int nx, int ny, i
 hipError_t fame_launch_pack(const void* in, void* out, int N, int Nd, int dir, int prec, hipStream_t s);
 
 /* ---- vector / reduction kernels (fame_blas.hip) ------------------------- */
-struct CgScal {
-    double rr;        /* current <r,r>                                         */
-    double rr0;       /* <b,b> of the solve                                    */
-    double pap_re, pap_im;
-    double alpha_re, alpha_im;
-    double beta;
-    double tol2;      /* absolute tolerance squared                           */
-    int iters;        /* reference-style counter k (starts at 1)              */
-    int maxit;
-    int active;       /* 1 while iterating                                      */
-    int pad;
-};
 
 /* y = alpha * x (+ y if accumulate), complex vectors of n per problem, real alpha per element table */
 hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* partials, int nblk,
@@ -74,11 +92,11 @@ hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* parti
 hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob,
                              int* active, hipStream_t s);
 hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob, const int* active, hipStream_t s);
-hipError_t fame_cg_update_xr(void* x, void* r, const void* p, const void* Ap, const CgScal* sc, double* partials,
-                             int nblk, size_t n, size_t stride, int nprob, int prec, const int* active, hipStream_t s);
+hipError_t fame_cg_update_r(void* r, const void* Ap, const CgScal* sc, double* partials, int nblk, size_t n,
+                            size_t stride, int nprob, int prec, const int* active, hipStream_t s);
 hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
-hipError_t fame_cg_update_p(void* p, const void* r, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
-                            const int* active, hipStream_t s);
+hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
+                            hipStream_t s);
 
 /* generic helpers (complex double unless prec says float) */
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,

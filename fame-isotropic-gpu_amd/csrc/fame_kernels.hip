@@ -59,6 +59,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     if (tile >= nblk) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LineSym lsym[MAX_LINES_INNER];
+    __shared__ double2 ephi[MAX_LINES_INNER];
     const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
     const LinePlanDev& pl = args.plan;
@@ -66,14 +67,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     T2* base = reinterpret_cast<T2*>(smem_raw);
     const Tabs<T2> tb = load_tabs<T2>(pl, base);
     T2* S = base + lds_head(pl);
+    /* exp(i pi a / na) for a < na behind the lines, 16-byte aligned */
+    double2* ea = reinterpret_cast<double2*>(base + inner_tab_offset(lds_head(pl) + 3 * L * P, (int)sizeof(T)));
     const int l0 = tile * L;
     if ((int)threadIdx.x < L) {
         const int gl = l0 + threadIdx.x;
         if (gl < nlines) {
             const int cf = udiv(gl, g.m_nb), bf = gl - cf * g.nb;
-            lsym[threadIdx.x] = line_sym(g, pr, bf, cf);
+            const LineSym ls = line_sym(g, pr, bf, cf);
+            lsym[threadIdx.x] = ls;
+            ephi[threadIdx.x] = expipi(ls.phi / (double)na);
         }
     }
+    for (int a = threadIdx.x; a < na; a += FFT_THREADS) ea[a] = expipi((double)a / (double)na);
     __syncthreads();
     /* per-problem base pointers (uniform) with 32-bit element offsets below
      * (3N < 2^31 is checked at context creation) */
@@ -86,17 +92,41 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     const unsigned mW = pl.blue ? pl.mM : g.m_na;
 
     if (EXPAND) {
+        /* CG fusion: per-problem scalars of the direction / solution update */
+        const bool cgf = args.cg_r != nullptr;
+        T beta = 0, ar = 0, ai = 0;
+        bool pend = false;
+        if (cgf) {
+            const CgScal& cs = args.cgsc[p];
+            beta = (T)cs.beta; ar = (T)cs.alpha_re; ai = (T)cs.alpha_im; pend = cs.pend != 0;
+        }
         for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
             const int line = udiv(idx, mW), a = idx - line * Wd;
             const int gl = l0 + line;
             double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
             if (gl < nlines && a < na) {
                 const unsigned f = (unsigned)(gl * na + a);
+                T2 xv1 = X[f], xv2 = X[N + f];
+                if (cgf) {
+                    T2* Pw = const_cast<T2*>(X);
+                    const T2* R = reinterpret_cast<const T2*>(args.cg_r) + (size_t)p * args.xstride;
+                    if (pend) {
+                        T2* Xs = reinterpret_cast<T2*>(args.cg_x) + (size_t)p * args.xstride;
+                        T2 s1 = Xs[f], s2 = Xs[N + f];
+                        s1.x += ar * xv1.x - ai * xv1.y; s1.y += ar * xv1.y + ai * xv1.x;
+                        s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
+                        Xs[f] = s1; Xs[N + f] = s2;
+                    }
+                    const T2 r1 = R[f], r2 = R[N + f];
+                    xv1.x = r1.x + beta * xv1.x; xv1.y = r1.y + beta * xv1.y;
+                    xv2.x = r2.x + beta * xv2.x; xv2.y = r2.y + beta * xv2.y;
+                    Pw[f] = xv1; Pw[N + f] = xv2;
+                }
                 if (!(pr.gamma && f == 0)) {
                     double2 lx, ly, lz, p1[3], p2[3];
-                    lambdas_line(g, lsym[line], a, lx, ly, lz);
-                    pi_basis(g, lx, ly, lz, p1, p2);
-                    const double2 x1 = to_d(X[f]), x2 = to_d(X[N + f]);
+                    lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                    pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
+                    const double2 x1 = to_d(xv1), x2 = to_d(xv2);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
@@ -148,8 +178,8 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
             if (!(pr.gamma && f == 0)) {
                 double2 lx, ly, lz, p1[3], p2[3];
-                lambdas_line(g, lsym[line], a, lx, ly, lz);
-                pi_basis(g, lx, ly, lz, p1, p2);
+                lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
@@ -399,7 +429,7 @@ template <typename T, int ENG>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
     dim3 grid(xcd_round((nlines + a.L - 1) / a.L), nprob);
-    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, 0);
+    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, inner_tab_extra(a.g.na, (int)sizeof(T)));
     if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     else hipLaunchKernelGGL((k_inner<T, 0, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
     return hipGetLastError();

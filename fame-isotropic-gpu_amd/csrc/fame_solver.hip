@@ -228,8 +228,10 @@ fame_status AxisPlan::build(int n_, int engine) {
 }
 
 /* LDS entries of one block beyond the lines: the middle-axis kernels keep one
- * twiddle per line position */
-static int tile_extra(int axis, const GridDev& g) { return axis == AX_MID ? g.nb : 0; }
+ * twiddle per line position, the inner-axis kernels a phase per mode */
+static int tile_extra(int axis, const GridDev& g, int prec) {
+    return axis == AX_MID ? g.nb : (axis == AX_INNER ? inner_tab_extra(g.na, prec) : 0);
+}
 static int tile_ncomp(int axis) { return axis == AX_INNER ? 3 : 1; }
 
 /* is L lines per block feasible for this plan: register budget of the Stockham
@@ -239,7 +241,7 @@ static bool tile_ok(const AxisPlan& p, int axis, int prec, int L, const GridDev&
     const int pi = prec == 8 ? 0 : 1, nc = tile_ncomp(axis);
     if (L < 1 || (axis == AX_INNER && L > MAX_LINES_INNER)) return false;
     if (!p.mfma && (long long)nc * L * p.M > p.cap[pi]) return false;
-    return fame_line_lds(p.lists[pi][0], prec, nc * L, p.M | 1, tile_extra(axis, g)) <= 60 * 1024;
+    return fame_line_lds(p.lists[pi][0], prec, nc * L, p.M | 1, tile_extra(axis, g, prec)) <= 60 * 1024;
 }
 
 /* default lines per block before tuning: as many as feasible up to maxL, rounded
@@ -503,10 +505,13 @@ KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
 
 /* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
  * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
-fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec) {
+fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
+                          void* cg_r, void* cg_x) {
     KInner ki = kin(prec, p0, act, LIST_INV);
     ki.x = x; ki.pikind = 0;
+    if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; }
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
+    ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr;
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
     HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
@@ -738,6 +743,7 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
         if (!any || done_it >= maxit) break;
         chunk = 2;
     }
+    HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
     std::vector<CgScal> sc(nprob);
     HIPCHK(hipMemcpyAsync(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -747,10 +753,15 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     return FAME_OK;
 }
 
-/* one CG iteration (CG.cu:53-78) on the masked batch: Ap = QBQ p with the
- * fused <p,Ap> partials, alpha, x += alpha p and r -= alpha Ap with the <r,r>
- * partials, beta and the active flags, p = r + beta p.  All scalars stay on
- * the device, so the nine launches are captured once as a graph.          */
+/* one CG iteration (CG.cu:53-78) on the masked batch:
+ *   first QBQ pass: x += alpha p (previous iteration's, when pending) and
+ *                   p = r + beta p, then Ap = QBQ p with the fused <p,Ap> partials;
+ *   alpha; r -= alpha Ap with the <r,r> partials; beta and the active flags.
+ * The reference's order of updates is kept; the x and p updates of iteration k
+ * are applied while iteration k+1 reads p anyway (k_cg_begin sets p = r and
+ * beta = 0, so the first pass leaves p = r).  The last x update is applied by
+ * k_cg_finish_x after the loop.  All scalars stay on the device, so the eight
+ * launches are captured once as a graph.                                   */
 fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
@@ -761,14 +772,13 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
     auto launch = [&](hipStream_t s) -> fame_status {
         const hipStream_t keep = stream;
         stream = s;
-        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p);
+        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x);
         stream = keep;
         CHK(q);
         const int nbi = nblk_in[pix(prec)];
         HIPCHK(fame_cg_alpha(cgsc, partials2, nbi, nprob, active, s));
-        HIPCHK(fame_cg_update_xr(x, r, p, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, s));
+        HIPCHK(fame_cg_update_r(r, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, s));
         HIPCHK(fame_cg_beta(cgsc, partials, NBLK_VEC, nprob, active, s));
-        HIPCHK(fame_cg_update_p(p, r, cgsc, n, st, nprob, prec, active, s));
         return FAME_OK;
     };
     if (!use_graphs) return launch(stream);
