@@ -289,7 +289,20 @@ __global__ void k_gemv_n(const double2* V, double2* w, const double* h, int m, s
     const double* hp = h + (size_t)p * hstride;
     for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
         double2 acc = wp[i];
-        for (int j = 0; j < m; ++j) {
+        int j = 0;
+        /* four column loads in flight, accumulated in column order */
+        for (; j + 4 <= m; j += 4) {
+            double2 v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = Vp[(size_t)(j + q) * cstride + i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double hr = hp[2 * (j + q)], hi = hp[2 * (j + q) + 1];
+                acc.x -= v[q].x * hr - v[q].y * hi;
+                acc.y -= v[q].x * hi + v[q].y * hr;
+            }
+        }
+        for (; j < m; ++j) {
             double2 v = Vp[(size_t)j * cstride + i];
             double hr = hp[2 * j], hi = hp[2 * j + 1];
             acc.x -= v.x * hr - v.y * hi;

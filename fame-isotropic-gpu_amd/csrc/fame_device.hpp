@@ -72,6 +72,30 @@ __device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulh
 #define FAME_MAX_STAGES 24
 #define FFT_THREADS 256
 
+/* A block-strided loop over idx = threadIdx.x + k * FFT_THREADS < total in two
+ * phases per group of U elements: ld(idx) issues the global loads of each
+ * element and returns them, then use(idx, value) consumes them.  The U loads
+ * of a thread are in flight together instead of one load-wait-store round
+ * trip per element, which is what keeps HBM busy in the strided gathers.   */
+template <int U, typename Ld, typename Use>
+__device__ __forceinline__ void batched_loop(int total, Ld ld, Use use) {
+    typedef decltype(ld(0)) R;
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += U * FFT_THREADS) {
+        R r[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = i0 + u * FFT_THREADS;
+            if (idx < total) r[u] = ld(idx);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = i0 + u * FFT_THREADS;
+            if (idx < total) use(idx, r[u]);
+        }
+    }
+}
+
+
 enum { PRE_NONE = 0, PRE_CHIRP = 1, PRE_CHIRP_CONJ = 2, PRE_BM = 3, PRE_BP = 4 };
 enum { POST_NONE = 0, POST_CHIRP = 1, POST_CHIRP_CONJ = 2 };
 

@@ -23,6 +23,11 @@ template <typename T> struct Vec;
 template <> struct Vec<double> { typedef double2 T2; };
 template <> struct Vec<float>  { typedef float2 T2; };
 
+/* global loads in flight per thread in the gathers of the axis kernels */
+template <typename T> struct Inflight { static constexpr int v = sizeof(T) == 4 ? 8 : 4; };
+/* elements in flight per thread in the inner kernels (several vectors each) */
+template <typename T> struct InflightInner { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
+
 template <typename T2> __device__ __forceinline__ T2 from_d(double2 v);
 template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
 template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
@@ -100,43 +105,59 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             const CgScal& cs = args.cgsc[p];
             beta = (T)cs.beta; ar = (T)cs.alpha_re; ai = (T)cs.alpha_im; pend = cs.pend != 0;
         }
-        for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
-            const int line = udiv(idx, mW), a = idx - line * Wd;
-            const int gl = l0 + line;
-            double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-            if (gl < nlines && a < na) {
-                const unsigned f = (unsigned)(gl * na + a);
-                T2 xv1 = X[f], xv2 = X[N + f];
-                if (cgf) {
-                    T2* Pw = const_cast<T2*>(X);
-                    const T2* R = reinterpret_cast<const T2*>(args.cg_r) + (size_t)p * args.xstride;
-                    if (pend) {
-                        T2* Xs = reinterpret_cast<T2*>(args.cg_x) + (size_t)p * args.xstride;
-                        T2 s1 = Xs[f], s2 = Xs[N + f];
-                        s1.x += ar * xv1.x - ai * xv1.y; s1.y += ar * xv1.y + ai * xv1.x;
-                        s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
-                        Xs[f] = s1; Xs[N + f] = s2;
-                    }
-                    const T2 r1 = R[f], r2 = R[N + f];
-                    xv1.x = r1.x + beta * xv1.x; xv1.y = r1.y + beta * xv1.y;
-                    xv2.x = r2.x + beta * xv2.x; xv2.y = r2.y + beta * xv2.y;
-                    Pw[f] = xv1; Pw[N + f] = xv2;
-                }
-                if (!(pr.gamma && f == 0)) {
-                    double2 lx, ly, lz, p1[3], p2[3];
-                    lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                    pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
-                    const double2 x1 = to_d(xv1), x2 = to_d(xv2);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
-                        else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+        T2* const Pw = const_cast<T2*>(X);
+        const T2* const R = cgf ? reinterpret_cast<const T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
+        T2* const Xs = cgf ? reinterpret_cast<T2*>(args.cg_x) + (size_t)p * args.xstride : nullptr;
+        struct XIn { T2 x1, x2, r1, r2, s1, s2; };
+        batched_loop<InflightInner<T>::v>(
+            L * Wd,
+            [&](int idx) -> XIn {
+                const int line = udiv(idx, mW), a = idx - line * Wd;
+                const int gl = l0 + line;
+                XIn d;
+                if (gl < nlines && a < na) {
+                    const unsigned f = (unsigned)(gl * na + a);
+                    d.x1 = X[f]; d.x2 = X[N + f];
+                    if (cgf) {
+                        d.r1 = R[f]; d.r2 = R[N + f];
+                        if (pend) { d.s1 = Xs[f]; d.s2 = Xs[N + f]; }
                     }
                 }
-            }
+                return d;
+            },
+            [&](int idx, XIn d) {
+                const int line = udiv(idx, mW), a = idx - line * Wd;
+                const int gl = l0 + line;
+                double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+                if (gl < nlines && a < na) {
+                    const unsigned f = (unsigned)(gl * na + a);
+                    T2 xv1 = d.x1, xv2 = d.x2;
+                    if (cgf) {
+                        if (pend) {
+                            T2 s1 = d.s1, s2 = d.s2;
+                            s1.x += ar * xv1.x - ai * xv1.y; s1.y += ar * xv1.y + ai * xv1.x;
+                            s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
+                            Xs[f] = s1; Xs[N + f] = s2;
+                        }
+                        xv1.x = d.r1.x + beta * xv1.x; xv1.y = d.r1.y + beta * xv1.y;
+                        xv2.x = d.r2.x + beta * xv2.x; xv2.y = d.r2.y + beta * xv2.y;
+                        Pw[f] = xv1; Pw[N + f] = xv2;
+                    }
+                    if (!(pr.gamma && f == 0)) {
+                        double2 lx, ly, lz, p1[3], p2[3];
+                        lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                        pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
+                        const double2 x1 = to_d(xv1), x2 = to_d(xv2);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
-        }
+                        for (int c = 0; c < 3; ++c) {
+                            if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
+                            else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+                        }
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
+            });
         __syncthreads();
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
@@ -150,58 +171,83 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 F[c * N + f] = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w);
         }
     } else {
-        for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
-            const int line = udiv(idx, mW), a = idx - line * Wd;
-            const int gl = l0 + line;
-            T2 v[3];
+        struct FIn { T2 v[3]; };
+        batched_loop<InflightInner<T>::v>(
+            L * Wd,
+            [&](int idx) -> FIn {
+                const int line = udiv(idx, mW), a = idx - line * Wd;
+                const int gl = l0 + line;
+                FIn d;
 #pragma unroll
-            for (int c = 0; c < 3; ++c) { v[c].x = 0; v[c].y = 0; }
-            if (gl < nlines && a < na) {
-                const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
-                const unsigned f = (unsigned)(gl * na + a);
+                for (int c = 0; c < 3; ++c) { d.v[c].x = 0; d.v[c].y = 0; }
+                if (gl < nlines && a < na) {
+                    const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) v[c] = cmulc(F[c * N + f], w);
-            }
+                    for (int c = 0; c < 3; ++c) d.v[c] = F[c * N + f];
+                }
+                return d;
+            },
+            [&](int idx, FIn d) {
+                const int line = udiv(idx, mW), a = idx - line * Wd;
+                const int gl = l0 + line;
+                if (gl < nlines && a < na) {
+                    const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = v[c];
-        }
+                    for (int c = 0; c < 3; ++c) d.v[c] = cmulc(d.v[c], w);
+                }
+#pragma unroll
+                for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = d.v[c];
+            });
         __syncthreads();
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0;
-        for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
-            const int line = udiv(idx, g.m_na), a = idx - line * na;
-            const int gl = l0 + line;
-            if (gl >= nlines) continue;
-            const unsigned f = (unsigned)(gl * na + a);
-            double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
-            if (!(pr.gamma && f == 0)) {
-                double2 lx, ly, lz, p1[3], p2[3];
-                lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
+        struct DIn { T2 d1, d2; };
+        batched_loop<InflightInner<T>::v>(
+            L * na,
+            [&](int idx) -> DIn {
+                const int line = udiv(idx, g.m_na), a = idx - line * na;
+                const int gl = l0 + line;
+                DIn d;
+                if (D && gl < nlines) {
+                    const unsigned f = (unsigned)(gl * na + a);
+                    d.d1 = D[f]; d.d2 = D[N + f];
+                }
+                return d;
+            },
+            [&](int idx, DIn dd) {
+                const int line = udiv(idx, g.m_na), a = idx - line * na;
+                const int gl = l0 + line;
+                if (gl >= nlines) return;
+                const unsigned f = (unsigned)(gl * na + a);
+                double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
+                if (!(pr.gamma && f == 0)) {
+                    double2 lx, ly, lz, p1[3], p2[3];
+                    lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                    pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
-                    if (args.pikind == 0) {
-                        y1 = cadd(y1, cmul(cconj(p1[c]), w));
-                        y2 = cadd(y2, cmul(cconj(p2[c]), w));
-                    } else {
-                        y1 = csub(y1, cmul(p2[c], w));
-                        y2 = cadd(y2, cmul(p1[c], w));
+                    for (int c = 0; c < 3; ++c) {
+                        const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
+                        if (args.pikind == 0) {
+                            y1 = cadd(y1, cmul(cconj(p1[c]), w));
+                            y2 = cadd(y2, cmul(cconj(p2[c]), w));
+                        } else {
+                            y1 = csub(y1, cmul(p2[c], w));
+                            y2 = cadd(y2, cmul(p1[c], w));
+                        }
                     }
                 }
-            }
-            const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
-            Y[f] = o1;
-            Y[N + f] = o2;
-            if (D) {
-                const double2 d1 = to_d(D[f]), d2 = to_d(D[N + f]);
-                const double2 q1 = to_d(o1), q2 = to_d(o2);
-                dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
-                dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
-            }
-        }
+                const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
+                Y[f] = o1;
+                Y[N + f] = o2;
+                if (D) {
+                    const double2 d1 = to_d(dd.d1), d2 = to_d(dd.d2);
+                    const double2 q1 = to_d(o1), q2 = to_d(o2);
+                    dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
+                    dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                }
+            });
         if (D) {
             /* block sum of conj(d) . y into partials[p][tile]: wave
              * shuffles, then the four wave sums in a fixed order */
@@ -250,16 +296,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     for (int b = threadIdx.x; b < nb; b += FFT_THREADS) twb[b] = tw_expipi<T2>(phc * (double)b);
     __syncthreads();
     const int Wd = pl.blue ? pl.M : nb;
-    for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
-        const int b = udiv(idx, args.mL), ta = idx - b * L, a = a0 + ta;
-        T2 v;
-        v.x = 0; v.y = 0;
-        if (a < na && b < nb) {
-            v = F[(unsigned)(b * na + a)];
-            if (DIR < 0) v = cmulc(v, twb[b]);
-        }
-        S[ta * P + b] = v;
-    }
+    batched_loop<Inflight<T>::v>(
+        L * Wd,
+        [&](int idx) -> T2 {
+            const int b = udiv(idx, args.mL), a = a0 + (idx - b * L);
+            T2 v;
+            v.x = 0; v.y = 0;
+            if (a < na && b < nb) v = F[(unsigned)(b * na + a)];
+            return v;
+        },
+        [&](int idx, T2 v) {
+            const int b = udiv(idx, args.mL), ta = idx - b * L;
+            if (DIR < 0 && b < nb) v = cmulc(v, twb[b]);
+            S[ta * P + b] = v;
+        });
     __syncthreads();
     run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
     for (int idx = threadIdx.x; idx < L * nb; idx += FFT_THREADS) {
@@ -300,27 +350,36 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     const int q0 = tile * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
     const int Wd = pl.blue ? pl.M : nc;
-    for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
-        const int c = udiv(idx, args.mL), tq = idx - c * L, q = q0 + tq;
-        T2 v;
-        v.x = 0; v.y = 0;
-        if (q < plane && c < nc) {
-            v = F[(unsigned)(c * plane + q)];
-            if (MODE == OUTER_QRS) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
-        }
-        S[tq * P + c] = v;
-    }
+    batched_loop<Inflight<T>::v>(
+        L * Wd,
+        [&](int idx) -> T2 {
+            const int c = udiv(idx, args.mL), q = q0 + (idx - c * L);
+            T2 v;
+            v.x = 0; v.y = 0;
+            if (q < plane && c < nc) v = F[(unsigned)(c * plane + q)];
+            return v;
+        },
+        [&](int idx, T2 v) {
+            const int c = udiv(idx, args.mL), tq = idx - c * L;
+            if (MODE == OUTER_QRS && c < nc) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
+            S[tq * P + c] = v;
+        });
     __syncthreads();
     if (MODE == OUTER_EPS) {
         const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
         const T scale = (T)(1.0 / (double)g.N / (pl.blue ? (double)pl.M : 1.0));
         const unsigned mL = args.mL;
         auto eps_step = [&](T2* Sx) {
-            for (int idx = threadIdx.x; idx < L * Wd; idx += FFT_THREADS) {
-                const int c = udiv(idx, mL), tq = idx - c * L, q = q0 + tq;
-                const T s = (q < plane && c < nc) ? invB[(unsigned)(c * plane + q)] * scale : (T)0;
-                Sx[tq * P + c] = cscale(Sx[tq * P + c], s);
-            }
+            batched_loop<2 * Inflight<T>::v>(
+                L * Wd,
+                [&](int idx) -> T {
+                    const int c = udiv(idx, mL), q = q0 + (idx - c * L);
+                    return (q < plane && c < nc) ? invB[(unsigned)(c * plane + q)] : (T)0;
+                },
+                [&](int idx, T e) {
+                    const int c = udiv(idx, mL), tq = idx - c * L;
+                    Sx[tq * P + c] = cscale(Sx[tq * P + c], e * scale);
+                });
             __syncthreads();
         };
         run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
