@@ -415,18 +415,19 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
         if (w < nqt) {
             acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
             int r = r0;
+            /* k step ks: A = cos / -sin (zero outside the q and t ranges), B = the
+             * s / d columns (lane holds row k = 4 ks + kq, column (line) row).
+             * The table is read unconditionally (r < n) and masked afterwards, so
+             * that no branch separates the LDS reads from the MFMAs.           */
             for (int ks = 0; ks < nks; ++ks) {
-                const int t = ks * 4 + kq + 1;
-                T a_c = 0, a_s = 0;
-                if (qon && t <= m) {
-                    const T2 wv = tb.W[r];
-                    a_c = wv.x; a_s = -wv.y;
-                }
-                r += dr;
-                if (r >= n) r -= n;
-                /* B operand: lane holds row k = 4 ks + kq, column (line) row */
+                const T2 wv = tb.W[r];
                 const int o = (ks * 4 + kq) * 16 + row;
                 const T2 bs = SP[o], bd = SM[o];
+                const bool on = qon && (ks * 4 + kq + 1 <= m);
+                r += dr;
+                if (r >= n) r -= n;
+                T a_c = 0, a_s = 0;
+                if (on) { a_c = wv.x; a_s = -wv.y; }
                 cr = Mfma<T>::mac(a_c, bs.x, cr);
                 ci = Mfma<T>::mac(a_c, bs.y, ci);
                 sr = Mfma<T>::mac(a_s, bd.x, sr);

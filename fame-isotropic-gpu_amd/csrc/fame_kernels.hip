@@ -23,10 +23,13 @@ template <typename T> struct Vec;
 template <> struct Vec<double> { typedef double2 T2; };
 template <> struct Vec<float>  { typedef float2 T2; };
 
-/* global loads in flight per thread in the gathers of the axis kernels */
+/* global loads in flight per thread: gathers of the axis kernels, the 1/eps
+ * step of the outer kernel, and the inner kernels (several vectors per
+ * element).  Larger batches cost registers, and so waves per SIMD; these are
+ * the measured best on gfx950 for the 96x83x78 grid at 30 problems.        */
 template <typename T> struct Inflight { static constexpr int v = sizeof(T) == 4 ? 8 : 4; };
-/* elements in flight per thread in the inner kernels (several vectors each) */
-template <typename T> struct InflightInner { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
+template <typename T> struct InflightEps { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
+template <typename T> struct InflightInner { static constexpr int v = 2; };
 
 template <typename T2> __device__ __forceinline__ T2 from_d(double2 v);
 template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
@@ -370,7 +373,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         const T scale = (T)(1.0 / (double)g.N / (pl.blue ? (double)pl.M : 1.0));
         const unsigned mL = args.mL;
         auto eps_step = [&](T2* Sx) {
-            batched_loop<2 * Inflight<T>::v>(
+            batched_loop<InflightEps<T>::v>(
                 L * Wd,
                 [&](int idx) -> T {
                     const int c = udiv(idx, mL), q = q0 + (idx - c * L);
