@@ -240,6 +240,85 @@ __device__ __forceinline__ void dft4(T2& v0, T2& v1, T2& v2, T2& v3, bool inv) {
     v0 = cadd(a0, a2); v2 = csub(a0, a2); v1 = cadd(a1, j3); v3 = csub(a1, j3);
 }
 
+/* cos / sin (2 pi e / R), e = 0..R-1, for the inner twiddles of the compound radices */
+template <int R> struct CTab;
+template <> struct CTab<6> {
+    static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    static constexpr double s[6] = {0.0, 0.8660254037844386, 0.8660254037844386, 0.0, -0.8660254037844386, -0.8660254037844386};
+};
+template <> struct CTab<9> {
+    static constexpr double c[9] = {1.0, 0.766044443118978, 0.17364817766693036, -0.5, -0.9396926207859084, -0.9396926207859084, -0.5, 0.17364817766693036, 0.766044443118978};
+    static constexpr double s[9] = {0.0, 0.6427876096865394, 0.984807753012208, 0.8660254037844386, 0.3420201433256687, -0.3420201433256687, -0.8660254037844386, -0.984807753012208, -0.6427876096865394};
+};
+template <> struct CTab<10> {
+    static constexpr double c[10] = {1.0, 0.8090169943749475, 0.30901699437494745, -0.30901699437494745, -0.8090169943749475, -1.0, -0.8090169943749475, -0.30901699437494745, 0.30901699437494745, 0.8090169943749475};
+    static constexpr double s[10] = {0.0, 0.5877852522924731, 0.9510565162951535, 0.9510565162951535, 0.5877852522924731, 0.0, -0.5877852522924731, -0.9510565162951535, -0.9510565162951535, -0.5877852522924731};
+};
+template <> struct CTab<12> {
+    static constexpr double c[12] = {1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5, 0.0, 0.5, 0.8660254037844386};
+    static constexpr double s[12] = {0.0, 0.5, 0.8660254037844386, 1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5};
+};
+template <> struct CTab<15> {
+    static constexpr double c[15] = {1.0, 0.9135454576426009, 0.6691306063588582, 0.30901699437494745, -0.10452846326765347, -0.5, -0.8090169943749475, -0.9781476007338057, -0.9781476007338057, -0.8090169943749475, -0.5, -0.10452846326765347, 0.30901699437494745, 0.6691306063588582, 0.9135454576426009};
+    static constexpr double s[15] = {0.0, 0.4067366430758002, 0.7431448254773942, 0.9510565162951535, 0.9945218953682733, 0.8660254037844386, 0.5877852522924731, 0.20791169081775934, -0.20791169081775934, -0.5877852522924731, -0.8660254037844386, -0.9945218953682733, -0.9510565162951535, -0.7431448254773942, -0.4067366430758002};
+};
+template <> struct CTab<16> {
+    static constexpr double c[16] = {1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867, -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898, 0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867};
+    static constexpr double s[16] = {0.0, 0.3826834323650898, 0.7071067811865476, 0.9238795325112867, 1.0, 0.9238795325112867, 0.7071067811865476, 0.3826834323650898, 0.0, -0.3826834323650898, -0.7071067811865476, -0.9238795325112867, -1.0, -0.9238795325112867, -0.7071067811865476, -0.3826834323650898};
+};
+
+/* v * W_R^e with W_R = exp(-2 pi i / R) (forward) or its conjugate (INV); e is a
+ * constant once the callers' loops are unrolled; 0, R/4, R/2, 3R/4 are exact */
+template <typename T2, int R, bool INV>
+__device__ __forceinline__ T2 tw_const(T2 v, int e) {
+    typedef decltype(v.x) T;
+    T2 o;
+    if (e == 0) return v;
+    if (2 * e == R) { o.x = -v.x; o.y = -v.y; return o; }
+    if (4 * e == R) {          /* forward -i, inverse +i */
+        if (!INV) { o.x = v.y; o.y = -v.x; } else { o.x = -v.y; o.y = v.x; }
+        return o;
+    }
+    if (4 * e == 3 * R) {      /* forward +i, inverse -i */
+        if (!INV) { o.x = -v.y; o.y = v.x; } else { o.x = v.y; o.y = -v.x; }
+        return o;
+    }
+    const T c = (T)CTab<R>::c[e], sn = INV ? (T)CTab<R>::s[e] : -(T)CTab<R>::s[e];
+    o.x = v.x * c - v.y * sn;
+    o.y = v.x * sn + v.y * c;
+    return o;
+}
+
+template <typename T2, int R>
+__device__ __forceinline__ void dft(T2 (&v)[R], bool inv);
+
+/* DFT of length R = R1 R2 in registers: input n = R2 n1 + n2, output
+ * k = k1 + R1 k2,  X_k = sum_n2 W_R2^(n2 k2) W_R^(n2 k1) sum_n1 x_n W_R1^(n1 k1).
+ * One radix-R pass replaces two passes (and an LDS round trip) of R1 and R2. */
+template <typename T2, int R1, int R2, bool INV>
+__device__ __forceinline__ void dft_pq(T2 (&v)[R1 * R2]) {
+    constexpr int R = R1 * R2;
+    T2 y[R2][R1];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) {
+        T2 a[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+        dft<T2, R1>(a, INV);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) y[n2][k1] = tw_const<T2, R, INV>(a[k1], (n2 * k1) % R);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+        T2 b[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2][k1];
+        dft<T2, R2>(b, INV);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+    }
+}
+
 template <typename T2, int R>
 __device__ __forceinline__ void dft(T2 (&v)[R], bool inv) {
     typedef decltype(v[0].x) T;
@@ -269,6 +348,18 @@ __device__ __forceinline__ void dft(T2 (&v)[R], bool inv) {
         dft4(b[0], b[1], b[2], b[3], inv);
 #pragma unroll
         for (int q = 0; q < 4; ++q) { v[2 * q] = a[q]; v[2 * q + 1] = b[q]; }
+    } else if constexpr (R == 6) {
+        if (inv) dft_pq<T2, 3, 2, true>(v); else dft_pq<T2, 3, 2, false>(v);
+    } else if constexpr (R == 9) {
+        if (inv) dft_pq<T2, 3, 3, true>(v); else dft_pq<T2, 3, 3, false>(v);
+    } else if constexpr (R == 10) {
+        if (inv) dft_pq<T2, 5, 2, true>(v); else dft_pq<T2, 5, 2, false>(v);
+    } else if constexpr (R == 12) {
+        if (inv) dft_pq<T2, 4, 3, true>(v); else dft_pq<T2, 4, 3, false>(v);
+    } else if constexpr (R == 15) {
+        if (inv) dft_pq<T2, 5, 3, true>(v); else dft_pq<T2, 5, 3, false>(v);
+    } else if constexpr (R == 16) {
+        if (inv) dft_pq<T2, 4, 4, true>(v); else dft_pq<T2, 4, 4, false>(v);
     } else {
         dft_odd<T2, R>(v, inv);
     }
@@ -473,6 +564,9 @@ __device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanD
         if constexpr (ENG == ENG_MFMA) {
             dft_mfma<T2>(S, nl, P, st, tb);
         } else {
+            /* the compound radices exist in the FP32 kernels only (the FP64
+             * plans never use them, and their registers would cost waves) */
+            constexpr bool CMP = MAXE >= 16;
             switch (st.R) {
                 case 2: fft_pass<T2, 2, MAXE>(S, nl, P, st, tb); break;
                 case 3: fft_pass<T2, 3, MAXE>(S, nl, P, st, tb); break;
@@ -482,7 +576,19 @@ __device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanD
                 case 8: fft_pass<T2, 8, MAXE>(S, nl, P, st, tb); break;
                 case 11: fft_pass<T2, 11, MAXE>(S, nl, P, st, tb); break;
                 case 13: fft_pass<T2, 13, MAXE>(S, nl, P, st, tb); break;
-                default: break;
+                default:
+                    if constexpr (CMP) {
+                        switch (st.R) {
+                            case 6: fft_pass<T2, 6, MAXE>(S, nl, P, st, tb); break;
+                            case 9: fft_pass<T2, 9, MAXE>(S, nl, P, st, tb); break;
+                            case 10: fft_pass<T2, 10, MAXE>(S, nl, P, st, tb); break;
+                            case 12: fft_pass<T2, 12, MAXE>(S, nl, P, st, tb); break;
+                            case 15: fft_pass<T2, 15, MAXE>(S, nl, P, st, tb); break;
+                            case 16: fft_pass<T2, 16, MAXE>(S, nl, P, st, tb); break;
+                            default: break;
+                        }
+                    }
+                    break;
             }
         }
     }

@@ -43,26 +43,45 @@ std::string& fame_err_ref() { return g_err; }
 /* ======================================================================== */
 /*  FFT plans                                                                */
 /* ======================================================================== */
-/* radices 8,4,2 then odd primes up to 13, largest first (the first pass of a
- * Stockham transform needs no twiddles, so it gets the largest butterfly) */
-static bool factor_gpu(int n, std::vector<int>* rad) {
+/* relative cost of one element through one pass of radix r (LDS round trip,
+ * twiddle and butterfly arithmetic); 6, 9, 10, 12, 15 and 16 are two-level
+ * in-register butterflies that save a pass */
+static double pass_cost(int r) {
+    switch (r) {
+        case 2: return 18; case 3: return 20; case 4: return 20; case 5: return 23; case 6: return 24;
+        case 7: return 26; case 8: return 23; case 9: return 26; case 10: return 26; case 11: return 30;
+        case 12: return 26; case 13: return 32; case 15: return 29; case 16: return 27;
+        default: return 1e9;
+    }
+}
+/* radices the FP64 kernels are compiled for (the single-level butterflies) */
+static bool is_prime_radix(int r) { return r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 11 || r == 13; }
+
+/* cheapest factorisation of n into the supported radices, largest radix first
+ * (the twiddle-free first pass gets the largest butterfly).  compound = false
+ * leaves out 6, 9, 10, 12, 15 and 16: the FP64 kernels hold 8 complex elements
+ * per thread, and the extra registers of the two-level butterflies cost them
+ * more waves than the saved pass gives back (measured on gfx950).           */
+static bool factor_gpu(int n, std::vector<int>* rad, bool compound = true) {
+    static const int RADS[] = {16, 15, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+    std::vector<double> cost(n + 1, 1e30);
+    std::vector<int> pick(n + 1, 0);
+    cost[1] = 0;
+    for (int m = 2; m <= n; ++m) {
+        if (n % m) continue;                   /* only divisors of n are needed */
+        for (int r : RADS)
+            if ((compound || is_prime_radix(r)) && m % r == 0 && cost[m / r] < 1e29 &&
+                cost[m / r] + pass_cost(r) < cost[m]) {
+                cost[m] = cost[m / r] + pass_cost(r);
+                pick[m] = r;
+            }
+    }
+    if (n < 1 || cost[n] > 1e29) return false;
     std::vector<int> r;
-    int m = n;
-    while (m % 8 == 0) { r.push_back(8); m /= 8; }
-    while (m % 4 == 0) { r.push_back(4); m /= 4; }
-    while (m % 2 == 0) { r.push_back(2); m /= 2; }
-    const int ps[] = {3, 5, 7, 11, 13};
-    for (int p : ps) while (m % p == 0) { r.push_back(p); m /= p; }
-    if (m != 1) return false;
+    for (int m = n; m > 1; m /= pick[m]) r.push_back(pick[m]);
     std::sort(r.begin(), r.end(), [](int x, int y) { return x > y; });
     if (rad) *rad = r;
     return true;
-}
-/* relative cost of one element through one pass of radix r (LDS round trip,
- * twiddle and butterfly arithmetic) */
-static double pass_cost(int r) {
-    switch (r) { case 2: return 18; case 3: return 20; case 4: return 20; case 5: return 23; case 7: return 26;
-                 case 8: return 23; case 11: return 30; case 13: return 32; default: return 1e9; }
 }
 static double plan_cost(const std::vector<int>& rad) { double c = 0; for (int r : rad) c += pass_cost(r); return c; }
 
@@ -178,8 +197,14 @@ fame_status AxisPlan::build(int n_, int engine) {
         *dst = d;
         return FAME_OK;
     };
+    /* FP64 lists: the same length M factored without the compound radices */
+    std::vector<int> rad64 = radices;
+    if (!mfma && !factor_gpu(M, &rad64, false)) FAIL(FAME_ERR_UNSUPPORTED, "no FP64 FFT plan for length " + std::to_string(M));
+    for (int r : rad64)
+        if (!is_prime_radix(r)) FAIL(FAME_ERR_UNSUPPORTED, "FP64 FFT plan with a compound radix");
     for (int pi = 0; pi < 2; ++pi) {
         const int prec = pi == 0 ? 8 : 4;
+        const std::vector<int>& rads = pi == 0 ? rad64 : radices;
         LinePlanDev base;
         std::memset(&base, 0, sizeof(base));
         base.n = n; base.M = M; base.blue = blue;
@@ -205,14 +230,14 @@ fame_status AxisPlan::build(int n_, int engine) {
             LinePlanDev& d = lists[pi][l];
             d = base;
             switch (l) {
-                case LIST_FWD: ok = ok && add_transform(d, radices, M, -1, blue, true); d.post = blue ? POST_CHIRP : POST_NONE; break;
-                case LIST_INV: ok = ok && add_transform(d, radices, M, +1, blue, true); d.post = blue ? POST_CHIRP_CONJ : POST_NONE; break;
+                case LIST_FWD: ok = ok && add_transform(d, rads, M, -1, blue, true); d.post = blue ? POST_CHIRP : POST_NONE; break;
+                case LIST_INV: ok = ok && add_transform(d, rads, M, +1, blue, true); d.post = blue ? POST_CHIRP_CONJ : POST_NONE; break;
                 case LIST_EPS:
-                    ok = ok && add_transform(d, radices, M, +1, blue, true) && add_mid(d) && add_transform(d, radices, M, -1, blue, false);
+                    ok = ok && add_transform(d, rads, M, +1, blue, true) && add_mid(d) && add_transform(d, rads, M, -1, blue, false);
                     d.post = blue ? POST_CHIRP : POST_NONE;
                     break;
                 default:
-                    ok = ok && add_transform(d, radices, M, -1, blue, true) && add_mid(d) && add_transform(d, radices, M, +1, blue, false);
+                    ok = ok && add_transform(d, rads, M, -1, blue, true) && add_mid(d) && add_transform(d, rads, M, +1, blue, false);
                     d.post = blue ? POST_CHIRP_CONJ : POST_NONE;
                     break;
             }
@@ -221,7 +246,7 @@ fame_status AxisPlan::build(int n_, int engine) {
         /* register budget per thread: 8 (FP64) / 16 (FP32) complex elements */
         const int maxe = prec == 8 ? MaxE<double>::v : MaxE<float>::v;
         int c = 1 << 12;
-        for (int r : radices) c = std::min(c, r * nbt_of(r, maxe));
+        for (int r : rads) c = std::min(c, r * nbt_of(r, maxe));
         cap[pi] = FFT_THREADS * c;     /* direct DFT stages hold no lines in registers */
     }
     return FAME_OK;
