@@ -121,7 +121,6 @@ struct LinePlanDev {
     int nst;            /* stages in st[]                                      */
     int post;           /* POST_*: multiplier the store applies                */
     int ntab;           /* LDS table entries: M (W) or 4M (W, chirp, bm, bp)   */
-    int wsz;            /* LDS workspace entries of the direct DFT stages (0: none) */
     int pad;
     unsigned mM, mN;    /* udiv magics for / M and / n                         */
     const void* W;      /* M entries exp(-2 pi i m / M) in the kernel precision */
@@ -132,11 +131,11 @@ struct LinePlanDev {
 };
 
 template <typename T2>
-struct Tabs { const T2* W; const T2* chirp; const T2* bm; const T2* bp; T2* ws; };
+struct Tabs { const T2* W; const T2* chirp; const T2* bm; const T2* bp; };
 
-/* LDS layout of a line-transform block: tables (ntab), direct-DFT workspace
- * (wsz), then the kernel's own entries and lines */
-__host__ __device__ inline int lds_head(const LinePlanDev& pl) { return pl.ntab + pl.wsz; }
+/* LDS layout of a line-transform block: tables (ntab), then the kernel's own
+ * entries and lines */
+__host__ __device__ inline int lds_head(const LinePlanDev& pl) { return pl.ntab; }
 
 /* copy a plan's tables into LDS at `base` (caller synchronises) */
 template <typename T2>
@@ -151,7 +150,6 @@ __device__ __forceinline__ Tabs<T2> load_tabs(const LinePlanDev& pl, T2* base) {
     Tabs<T2> tb;
     tb.W = base;
     tb.chirp = base + M; tb.bm = base + 2 * M; tb.bp = base + 3 * M;
-    tb.ws = base + pl.ntab;
     return tb;
 }
 
@@ -435,8 +433,10 @@ __device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& s
  *   A_q = sum_t cos(2 pi q t / n) s_t,   B_q = sum_t sin(2 pi q t / n) d_t.
  * A and B are two real matrix products (Q x m) x (m x 2 columns per line) run
  * as 16x16x4 MFMAs: wave w owns output rows q = 16w..16w+15 and reads its cos /
- * sin operands from the twiddle table; the s / d columns of 16 lines at a time
- * are staged in the LDS workspace.  About a quarter of the flops of the complex DFT-GEMM. */
+ * sin operands from the twiddle table.  For 16 lines at a time, s_t and d_t
+ * replace x_t and x_{n-t} in place in LDS (no workspace), and each lane reads
+ * its B operands from its line.  About a quarter of the flops of the complex
+ * DFT-GEMM. */
 template <typename T> struct Mfma;
 /* Operands of both 16x16x4 variants: lane l holds A[l % 16][l / 16] and
  * B[l / 16][l % 16].  Result row of accumulator element v in lane l (column
@@ -456,8 +456,6 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int row(int kq, int v) { return kq + 4 * v; }
 };
 
-/* workspace entries (in complex units) of a direct DFT with K rows */
-__host__ __device__ inline int dft_ws_entries(int K) { return 2 * K * 16 + 32; }
 
 template <typename T2>
 __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb) {
@@ -475,72 +473,64 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
     const bool qon = w < nqt && q < Q;
     const int dr = (4 * q) % n;
     const int r0 = (q * (kq + 1)) % n;
-    /* workspace: s_t and d_t of 16 lines as complex [k][line], then x_0 [16], h [16] */
-    T2* SP = tb.ws;
-    T2* SM = SP + K * 16;
-    T2* xh = SM + K * 16;
     const bool even = (n & 1) == 0;
     const int ngr = (nl + 15) >> 4;
     for (int gi = 0; gi < ngr; ++gi) {
-        for (int idx = threadIdx.x; idx < K * 16; idx += FFT_THREADS) {
+        /* s_t and d_t in place: s_t -> x_t, d_t -> x_{n-t} (t = 1..m); x_0 and,
+         * for even n, h = x_{n/2} are left where they are */
+        for (int idx = threadIdx.x; idx < m * 16; idx += FFT_THREADS) {
             const int k = idx >> 4, j = idx & 15, line = gi * 16 + j, t = k + 1;
-            T2 sp, sm;
-            sp.x = 0; sp.y = 0; sm.x = 0; sm.y = 0;
-            if (line < nl && t <= m) {
-                const T2 a = S[line * P + t], b = S[line * P + n - t];
-                sp = cadd(a, b); sm = csub(a, b);
-            }
-            SP[idx] = sp; SM[idx] = sm;
-        }
-        if (threadIdx.x < 16) {
-            const int line = gi * 16 + threadIdx.x;
-            T2 x0, h;
-            x0.x = 0; x0.y = 0; h.x = 0; h.y = 0;
             if (line < nl) {
-                x0 = S[line * P];
-                if (even) h = S[line * P + (n >> 1)];
+                T2* Sl = S + line * P;
+                const T2 a = Sl[t], b = Sl[n - t];
+                Sl[t] = cadd(a, b);
+                Sl[n - t] = csub(a, b);
             }
-            xh[threadIdx.x] = x0; xh[16 + threadIdx.x] = h;
         }
         __syncthreads();
+        /* lane's column (line) row; lines past nl read the last line, whose
+         * outputs in those columns are discarded */
+        const int line = gi * 16 + row;
+        const T2* Sb = S + min(line, nl - 1) * P;
+        acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+        T2 x0, h;
+        x0.x = 0; x0.y = 0; h.x = 0; h.y = 0;
         if (w < nqt) {
-            acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+            x0 = Sb[0];
+            if (even) h = Sb[n >> 1];
             int r = r0;
-            /* k step ks: A = cos / -sin (zero outside the q and t ranges), B = the
-             * s / d columns (lane holds row k = 4 ks + kq, column (line) row).
-             * The table is read unconditionally (r < n) and masked afterwards, so
-             * that no branch separates the LDS reads from the MFMAs.           */
             for (int ks = 0; ks < nks; ++ks) {
-                const T2 wv = tb.W[r];
-                const int o = (ks * 4 + kq) * 16 + row;
-                const T2 bs = SP[o], bd = SM[o];
-                const bool on = qon && (ks * 4 + kq + 1 <= m);
+                /* B: s / d of k = 4 ks + kq (t = k + 1; t > m rows meet a zero A) */
+                const int t = min(ks * 4 + kq + 1, max(m, 1));
+                const T2 bs = Sb[t], bd = Sb[n - t];
+                T a_c = 0, a_s = 0;
+                if (qon && ks * 4 + kq + 1 <= m) {
+                    const T2 wv = tb.W[r];
+                    a_c = wv.x; a_s = wv.y;
+                }
                 r += dr;
                 if (r >= n) r -= n;
-                T a_c = 0, a_s = 0;
-                if (on) { a_c = wv.x; a_s = -wv.y; }
                 cr = Mfma<T>::mac(a_c, bs.x, cr);
                 ci = Mfma<T>::mac(a_c, bs.y, ci);
                 sr = Mfma<T>::mac(a_s, bd.x, sr);
                 si = Mfma<T>::mac(a_s, bd.y, si);
             }
-            /* result: lane holds rows q = 16w + Mfma::row(kq, v), column (line) row */
-            const int line = gi * 16 + row;
-            if (line < nl) {
-                const T2 x0 = xh[row], h = xh[16 + row];
+        }
+        __syncthreads();   /* every wave has read its operands before S is overwritten */
+        /* result: lane holds rows q = 16w + Mfma::row(kq, v), column (line) row */
+        if (w < nqt && line < nl) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int qq = w * 16 + Mfma<T>::row(kq, v);
-                    if (qq < Q) {
-                        const T Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
-                        const T Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
-                        const T Bx = sr[v], By = si[v];
-                        T2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
-                        if (!inv) { lo.x = Ax + By; lo.y = Ay - Bx; hi.x = Ax - By; hi.y = Ay + Bx; }
-                        else      { lo.x = Ax - By; lo.y = Ay + Bx; hi.x = Ax + By; hi.y = Ay - Bx; }
-                        S[line * P + qq] = lo;
-                        if (qq != 0 && 2 * qq != n) S[line * P + n - qq] = hi;
-                    }
+            for (int v = 0; v < 4; ++v) {
+                const int qq = w * 16 + Mfma<T>::row(kq, v);
+                if (qq < Q) {
+                    const T Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
+                    const T Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
+                    const T Bx = -sr[v], By = -si[v];
+                    T2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
+                    if (!inv) { lo.x = Ax + By; lo.y = Ay - Bx; hi.x = Ax - By; hi.y = Ay + Bx; }
+                    else      { lo.x = Ax - By; lo.y = Ay + Bx; hi.x = Ax + By; hi.y = Ay - Bx; }
+                    S[line * P + qq] = lo;
+                    if (qq != 0 && 2 * qq != n) S[line * P + n - qq] = hi;
                 }
             }
         }

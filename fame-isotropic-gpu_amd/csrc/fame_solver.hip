@@ -209,7 +209,6 @@ fame_status AxisPlan::build(int n_, int engine) {
         std::memset(&base, 0, sizeof(base));
         base.n = n; base.M = M; base.blue = blue;
         base.ntab = blue ? 4 * M : M;
-        base.wsz = mfma ? dft_ws_entries(std::max(4, ((n - 1) / 2 + 3) / 4 * 4)) : 0;
         base.mM = fame_magic32(M); base.mN = fame_magic32(n);
         CHK(upload(W, prec, &base.W));
         CHK(upload(ch, prec, &base.chirp));
