@@ -102,56 +102,35 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
     active[p] = c.active;
 }
 
-__global__ void k_cg_alpha(CgScal* sc, const double* partials, int nblk, const int* active) {
+/* partials per inner block: <p,Ap> (2), <r,Ap> (2), <Ap,Ap>, <r,r> of the
+ * current residual (written by the expanding pass) */
+__global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int* active) {
     const int p = blockIdx.x;
     if (!active[p]) return;
-    double v[2] = {0.0, 0.0};
+    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-        v[0] += partials[((size_t)p * nblk + b) * 2];
-        v[1] += partials[((size_t)p * nblk + b) * 2 + 1];
+        const double* q = partials + ((size_t)p * nblk + b) * 6;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] += q[c];
     }
-    block_sum_all<2>(v);
+    block_sum_all<6>(v);
     if (threadIdx.x != 0) return;
-    const double re = v[0], im = v[1];
     CgScal& c = sc[p];
+    const double re = v[0], im = v[1];          /* <p, Ap> */
+    const double rre = v[2], rim = v[3];        /* <r, Ap> */
+    const double aa = v[4];                     /* <Ap, Ap> */
+    const double rr = v[5];                     /* <r, r>, summed from the vector */
     c.pap_re = re; c.pap_im = im;
     /* a = r1 / dot  (CG.cu:60-63) */
-    double temp = re * re + im * im;
-    c.alpha_re = c.rr * re / temp;
-    c.alpha_im = -c.rr * im / temp;
+    const double temp = re * re + im * im;
+    const double ar = rr * re / temp, ai = -rr * im / temp;
+    c.alpha_re = ar; c.alpha_im = ai;
     c.pend = 1;
-}
-
-/* r -= alpha Ap with the <r,r> partials.  The matching x += alpha p is deferred
- * to the next iteration's first QBQ pass, or to k_cg_finish_x.              */
-template <typename T>
-__global__ void __launch_bounds__(RB) k_cg_update_r(typename V2<T>::t* r, const typename V2<T>::t* Ap,
-                                                    const CgScal* sc, double* partials, size_t n, size_t stride,
-                                                    const int* active) {
-    typedef typename V2<T>::t T2;
-    const int p = blockIdx.y;
-    T acc = 0;
-    if (active[p]) {
-        const T ar = (T)sc[p].alpha_re, ai = (T)sc[p].alpha_im;
-        const size_t o = (size_t)p * stride;
-        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-            T2 ap = Ap[o + i], rr = r[o + i];
-            rr.x -= ar * ap.x - ai * ap.y; rr.y -= ar * ap.y + ai * ap.x;
-            r[o + i] = rr;
-            acc += rr.x * rr.x + rr.y * rr.y;
-        }
-    }
-    double a2[1] = {(double)acc};
-    block_partials<1>(a2, partials, p);
-}
-
-__global__ void k_cg_beta(CgScal* sc, const double* partials, int nblk, int* active) {
-    const int p = blockIdx.x;
-    if (!active[p]) return;
-    const double s = sum_partials(partials, p, nblk, 1, 0);
-    if (threadIdx.x != 0) return;
-    CgScal& c = sc[p];
-    c.beta = s / c.rr;
+    /* r1 of r - a Ap from one step of the recurrence off the summed <r,r>,
+     * accurate to rounding of that step; b = r1 / r0 and the stopping rule
+     * (CG.cu:66-78) use it, the next alpha the summed norm again */
+    const double s = fmax(0.0, rr - 2.0 * (ar * rre - ai * rim) + (ar * ar + ai * ai) * aa);
+    c.beta = s / rr;
     c.rr = s;
     c.iters += 1;
     c.active = (s > c.tol2) && (c.iters <= c.maxit);
@@ -437,22 +416,8 @@ hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, doubl
     hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
     return hipGetLastError();
 }
-hipError_t fame_cg_alpha(CgScal* sc, const double* partials, int nblk, int nprob, const int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
-    return hipGetLastError();
-}
-hipError_t fame_cg_update_r(void* r, const void* Ap, const CgScal* sc, double* partials, int nblk, size_t n,
-                            size_t stride, int nprob, int prec, const int* active, hipStream_t s) {
-    if (prec == 8)
-        hipLaunchKernelGGL(k_cg_update_r<double>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (double2*)r,
-                           (const double2*)Ap, sc, partials, n, stride, active);
-    else
-        hipLaunchKernelGGL(k_cg_update_r<float>, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)r,
-                           (const float2*)Ap, sc, partials, n, stride, active);
-    return hipGetLastError();
-}
-hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
+hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,

@@ -56,7 +56,7 @@ struct fame_ctx {
     void* cg32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  /* r, p, Ap, rhs32, sol32 */
     void* ref64[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double* partials = nullptr;   /* [problem][NBLK_VEC][<=4]   */
-    double* partials2 = nullptr;  /* [problem][inner blocks][2] */
+    double* partials2 = nullptr;  /* [problem][inner blocks][2, or 6 inside CG] */
     CgScal* cgsc = nullptr;
     int* active = nullptr;
     int* mask = nullptr;
@@ -84,7 +84,8 @@ struct fame_ctx {
     KAxis kmid(int prec, int p0, const int* act, int list) const;
     KAxis kout(int prec, int p0, const int* act, int list) const;
     fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
-                    void* cg_r = nullptr, void* cg_x = nullptr);
+                    void* cg_r = nullptr, void* cg_x = nullptr, const void* cg_ap = nullptr,
+                    const void* dotr = nullptr);
     fame_status qr_field(const void* x, int p0, int nprob, int pikind, const int* act);
     fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
     fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);

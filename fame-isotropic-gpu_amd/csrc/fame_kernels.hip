@@ -111,7 +111,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         T2* const Pw = const_cast<T2*>(X);
         const T2* const R = cgf ? reinterpret_cast<const T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
         T2* const Xs = cgf ? reinterpret_cast<T2*>(args.cg_x) + (size_t)p * args.xstride : nullptr;
-        struct XIn { T2 x1, x2, r1, r2, s1, s2; };
+        /* r -= alpha Ap of the previous iteration rides on this pass too */
+        const bool rup = cgf && pend && args.cg_ap != nullptr;
+        const T2* const AP = rup ? reinterpret_cast<const T2*>(args.cg_ap) + (size_t)p * args.xstride : nullptr;
+        struct XIn { T2 x1, x2, r1, r2, s1, s2, a1, a2; };
+        double rnorm = 0.0;   /* <r,r> of the updated residual, this thread's share */
         batched_loop<InflightInner<T>::v>(
             L * Wd,
             [&](int idx) -> XIn {
@@ -124,6 +128,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     if (cgf) {
                         d.r1 = R[f]; d.r2 = R[N + f];
                         if (pend) { d.s1 = Xs[f]; d.s2 = Xs[N + f]; }
+                        if (rup) { d.a1 = AP[f]; d.a2 = AP[N + f]; }
                     }
                 }
                 return d;
@@ -142,8 +147,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                             s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
                             Xs[f] = s1; Xs[N + f] = s2;
                         }
-                        xv1.x = d.r1.x + beta * xv1.x; xv1.y = d.r1.y + beta * xv1.y;
-                        xv2.x = d.r2.x + beta * xv2.x; xv2.y = d.r2.y + beta * xv2.y;
+                        T2 r1 = d.r1, r2 = d.r2;
+                        if (rup) {
+                            r1.x -= ar * d.a1.x - ai * d.a1.y; r1.y -= ar * d.a1.y + ai * d.a1.x;
+                            r2.x -= ar * d.a2.x - ai * d.a2.y; r2.y -= ar * d.a2.y + ai * d.a2.x;
+                            T2* Rw = const_cast<T2*>(R);
+                            Rw[f] = r1; Rw[N + f] = r2;
+                        }
+                        rnorm += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
+                        xv1.x = r1.x + beta * xv1.x; xv1.y = r1.y + beta * xv1.y;
+                        xv2.x = r2.x + beta * xv2.x; xv2.y = r2.y + beta * xv2.y;
                         Pw[f] = xv1; Pw[N + f] = xv2;
                     }
                     if (!(pr.gamma && f == 0)) {
@@ -161,7 +174,20 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
             });
+        /* inside CG: block sum of <r,r> into slot 5 of partials[p][tile] (the
+         * contracting pass fills slots 0..4) */
+        __shared__ double rred[FFT_THREADS / 64];
+        if (cgf) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) rnorm += __shfl_xor(rnorm, o);
+            if ((threadIdx.x & 63) == 0) rred[threadIdx.x >> 6] = rnorm;
+        }
         __syncthreads();
+        if (cgf && threadIdx.x == 0) {
+            double sum = 0;
+            for (int w = 0; w < FFT_THREADS / 64; ++w) sum += rred[w];
+            args.partials[((size_t)p * nblk + tile) * 6 + 5] = sum;
+        }
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
             const int line = udiv(idx, g.m_na), a = idx - line * na;
@@ -206,7 +232,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0;
-        struct DIn { T2 d1, d2; };
+        const T2* DR = (D && args.dotr) ? reinterpret_cast<const T2*>(args.dotr) + (size_t)p * args.xstride : nullptr;
+        double rre = 0.0, rim = 0.0, yy = 0.0;
+        struct DIn { T2 d1, d2, r1, r2; };
         batched_loop<InflightInner<T>::v>(
             L * na,
             [&](int idx) -> DIn {
@@ -216,6 +244,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 if (D && gl < nlines) {
                     const unsigned f = (unsigned)(gl * na + a);
                     d.d1 = D[f]; d.d2 = D[N + f];
+                    if (DR) { d.r1 = DR[f]; d.r2 = DR[N + f]; }
                 }
                 return d;
             },
@@ -249,21 +278,36 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     const double2 q1 = to_d(o1), q2 = to_d(o2);
                     dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
                     dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                    if (DR) {
+                        const double2 e1 = to_d(dd.r1), e2 = to_d(dd.r2);
+                        rre += e1.x * q1.x + e1.y * q1.y + e2.x * q2.x + e2.y * q2.y;
+                        rim += e1.x * q1.y - e1.y * q1.x + e2.x * q2.y - e2.y * q2.x;
+                        yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
+                    }
                 }
             });
         if (D) {
-            /* block sum of conj(d) . y into partials[p][tile]: wave
-             * shuffles, then the four wave sums in a fixed order */
-            __shared__ double red[2][FFT_THREADS / 64];
+            /* block sums of conj(d) . y (and, with dotr, conj(r) . y and y . y)
+             * into partials[p][tile]: wave shuffles, then the four wave sums in
+             * a fixed order */
+            __shared__ double red[5][FFT_THREADS / 64];
+            double v[5] = {dre, dim, rre, rim, yy};
+            const int nc = DR ? 5 : 2, stride = DR ? 6 : 2;
 #pragma unroll
-            for (int o = 32; o > 0; o >>= 1) { dre += __shfl_xor(dre, o); dim += __shfl_xor(dim, o); }
-            if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = dre; red[1][threadIdx.x >> 6] = dim; }
+            for (int c = 0; c < 5; ++c)
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+            if ((threadIdx.x & 63) == 0)
+#pragma unroll
+                for (int c = 0; c < 5; ++c) red[c][threadIdx.x >> 6] = v[c];
             __syncthreads();
             if (threadIdx.x == 0) {
-                double sr = 0, si = 0;
-                for (int w = 0; w < FFT_THREADS / 64; ++w) { sr += red[0][w]; si += red[1][w]; }
-                double* part = args.partials + ((size_t)p * nblk + tile) * 2;
-                part[0] = sr; part[1] = si;
+                double* part = args.partials + ((size_t)p * nblk + tile) * stride;
+                for (int c = 0; c < nc; ++c) {
+                    double sum = 0;
+                    for (int w = 0; w < FFT_THREADS / 64; ++w) sum += red[c][w];
+                    part[c] = sum;
+                }
             }
         }
     }

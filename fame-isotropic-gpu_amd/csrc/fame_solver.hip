@@ -377,7 +377,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     /* inner-kernel partials: room for one line per block (any tuned tile) */
     int nbmax = std::max(g.nb * g.nc, NBLK_VEC);
     A(c->partials, sizeof(double) * 4 * nbmax * B);
-    A(c->partials2, sizeof(double) * 2 * nbmax * B);
+    A(c->partials2, sizeof(double) * 6 * nbmax * B);
     A(c->cgsc, sizeof(CgScal) * B);
     A(c->active, sizeof(int) * B);
     A(c->mask, sizeof(int) * B);
@@ -530,12 +530,13 @@ KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
 /* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
  * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
 fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
-                          void* cg_r, void* cg_x) {
+                          void* cg_r, void* cg_x, const void* cg_ap, const void* dotr) {
     KInner ki = kin(prec, p0, act, LIST_INV);
     ki.x = x; ki.pikind = 0;
-    if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; }
+    if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; ki.cg_ap = cg_ap; ki.partials = partials2; }
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
-    ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr;
+    ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr; ki.cg_ap = nullptr;
+    ki.dotr = dotvec ? dotr : nullptr;
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
     HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
@@ -778,14 +779,17 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
 }
 
 /* one CG iteration (CG.cu:53-78) on the masked batch:
- *   first QBQ pass: x += alpha p (previous iteration's, when pending) and
- *                   p = r + beta p, then Ap = QBQ p with the fused <p,Ap> partials;
- *   alpha; r -= alpha Ap with the <r,r> partials; beta and the active flags.
- * The reference's order of updates is kept; the x and p updates of iteration k
- * are applied while iteration k+1 reads p anyway (k_cg_begin sets p = r and
- * beta = 0, so the first pass leaves p = r).  The last x update is applied by
- * k_cg_finish_x after the loop.  All scalars stay on the device, so the eight
- * launches are captured once as a graph.                                   */
+ *   first QBQ pass: x += alpha p and r -= alpha Ap (the previous iteration's,
+ *                   when pending), then p = r + beta p;
+ *   QBQ p -> Ap, the last pass with the fused <p,Ap>, <r,Ap> and <Ap,Ap> partials;
+ *   alpha from <r,r> summed by the first pass; the next <r,r> by one step of
+ *   |r - alpha Ap|^2 = |r|^2 - 2 Re(alpha <r,Ap>) + |alpha|^2 |Ap|^2 for beta
+ *   and the stopping rule (no pass over r); the active flags.
+ * The reference's alpha, beta and stopping rule are kept; the x, r and p updates
+ * of iteration k are applied while iteration k+1 reads p anyway (k_cg_begin sets
+ * p = r and beta = 0, so the first pass leaves p = r).  The last x update is
+ * applied by k_cg_finish_x after the loop.  All scalars stay on the device, so
+ * the six launches are captured once as a graph.                            */
 fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
@@ -796,13 +800,10 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
     auto launch = [&](hipStream_t s) -> fame_status {
         const hipStream_t keep = stream;
         stream = s;
-        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x);
+        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, r);
         stream = keep;
         CHK(q);
-        const int nbi = nblk_in[pix(prec)];
-        HIPCHK(fame_cg_alpha(cgsc, partials2, nbi, nprob, active, s));
-        HIPCHK(fame_cg_update_r(r, Ap, cgsc, partials, NBLK_VEC, n, st, nprob, prec, active, s));
-        HIPCHK(fame_cg_beta(cgsc, partials, NBLK_VEC, nprob, active, s));
+        HIPCHK(fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, s));
         return FAME_OK;
     };
     if (!use_graphs) return launch(stream);
