@@ -72,7 +72,7 @@ __device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulh
 #define FAME_MAX_STAGES 24
 #define FFT_THREADS 256
 
-/* A block-strided loop over idx = threadIdx.x + k * FFT_THREADS < total in two
+/* A block-strided loop over idx = threadIdx.x + k * blockDim.x < total in two
  * phases per group of U elements: ld(idx) issues the global loads of each
  * element and returns them, then use(idx, value) consumes them.  The U loads
  * of a thread are in flight together instead of one load-wait-store round
@@ -80,16 +80,17 @@ __device__ __forceinline__ int udiv(int n, unsigned m) { return m ? (int)__umulh
 template <int U, typename Ld, typename Use>
 __device__ __forceinline__ void batched_loop(int total, Ld ld, Use use) {
     typedef decltype(ld(0)) R;
-    for (int i0 = (int)threadIdx.x; i0 < total; i0 += U * FFT_THREADS) {
+    const int nt = (int)blockDim.x;
+    for (int i0 = (int)threadIdx.x; i0 < total; i0 += U * nt) {
         R r[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int idx = i0 + u * FFT_THREADS;
+            const int idx = i0 + u * nt;
             if (idx < total) r[u] = ld(idx);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int idx = i0 + u * FFT_THREADS;
+            const int idx = i0 + u * nt;
             if (idx < total) use(idx, r[u]);
         }
     }
@@ -478,7 +479,7 @@ __device__ __forceinline__ void dft_mfma(T2* S, int nl, int P, const StageDev& s
     for (int gi = 0; gi < ngr; ++gi) {
         /* s_t and d_t in place: s_t -> x_t, d_t -> x_{n-t} (t = 1..m); x_0 and,
          * for even n, h = x_{n/2} are left where they are */
-        for (int idx = threadIdx.x; idx < m * 16; idx += FFT_THREADS) {
+        for (int idx = threadIdx.x; idx < m * 16; idx += blockDim.x) {
             const int k = idx >> 4, j = idx & 15, line = gi * 16 + j, t = k + 1;
             if (line < nl) {
                 T2* Sl = S + line * P;

@@ -340,7 +340,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * nb * na;
     /* 32-bit element offsets from the uniform plane pointer */
     const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)nb;
-    for (int b = threadIdx.x; b < nb; b += FFT_THREADS) twb[b] = tw_expipi<T2>(phc * (double)b);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) twb[b] = tw_expipi<T2>(phc * (double)b);
     __syncthreads();
     const int Wd = pl.blue ? pl.M : nb;
     batched_loop<Inflight<T>::v>(
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_mid(KAxis args) {
         });
     __syncthreads();
     run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
-    for (int idx = threadIdx.x; idx < L * nb; idx += FFT_THREADS) {
+    for (int idx = threadIdx.x; idx < L * nb; idx += blockDim.x) {
         const int b = udiv(idx, args.mL), ta = idx - b * L, a = a0 + ta;
         if (a >= na) continue;
         T2 v = post_mul<T2, T>(pl, tb, b, S[ta * P + b]);
@@ -546,8 +546,17 @@ static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) 
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(xcd_round(ntile * a.g.nc), 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
-    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_mid<T, -1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
+    /* the matrix-core DFT keeps one wave per 16-row output tile busy: launch
+     * exactly those waves (at most four) */
+    int nthr = FFT_THREADS;
+    if (ENG == ENG_MFMA) {
+        int q = 1;
+        for (int i = 0; i < a.plan.nst; ++i)
+            if (a.plan.st[i].R == DFT_MFMA) q = a.plan.st[i].dqt > q ? a.plan.st[i].dqt : q;
+        nthr = 64 * (q < 4 ? q : 4);
+    }
+    if (dir > 0) hipLaunchKernelGGL((k_mid<T, 1, ENG>), grid, dim3(nthr), lds, s, a);
+    else hipLaunchKernelGGL((k_mid<T, -1, ENG>), grid, dim3(nthr), lds, s, a);
     return hipGetLastError();
 }
 
