@@ -153,10 +153,51 @@ __global__ void __launch_bounds__(RB) k_cg_finish_x(typename V2<T>::t* x, const 
     }
 }
 
+/* start of an FP32 solve from an FP64 right-hand side (the refinement's
+ * residual): x = 0, r = p = b rounded to FP32, and the partials of <r,r> */
+__global__ void __launch_bounds__(RB) k_cg_begin_from64(float2* x, float2* r, float2* pv, const double2* b,
+                                                        double* partials, size_t n, size_t stride, const int* mask) {
+    const int p = blockIdx.y;
+    double acc[1] = {0.0};
+    if (!mask || mask[p]) {
+        const size_t o = (size_t)p * stride;
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+            const double2 w = b[o + i];
+            const float2 v = make_float2((float)w.x, (float)w.y);
+            x[o + i] = make_float2(0.0f, 0.0f); r[o + i] = v; pv[o + i] = v;
+            const float s = v.x * v.x + v.y * v.y;
+            acc[0] += (double)s;
+        }
+    }
+    block_partials<1>(acc, partials, p);
+}
+
+/* end of an FP32 solve inside the refinement: x64 += x + alpha p (the update
+ * still pending) for the problems of the mask; the FP32 solution itself is
+ * not written back */
+__global__ void __launch_bounds__(RB) k_cg_finish_mixed(double2* x64, const float2* x, const float2* pv,
+                                                        const CgScal* sc, size_t n, size_t stride, const int* mask) {
+    const int p = blockIdx.y;
+    if (mask && !mask[p]) return;
+    const bool pend = sc[p].pend;
+    const float ar = (float)sc[p].alpha_re, ai = (float)sc[p].alpha_im;
+    const size_t o = (size_t)p * stride;
+    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+        float2 xx = x[o + i];
+        if (pend) {
+            const float2 pp = pv[o + i];
+            xx.x += ar * pp.x - ai * pp.y; xx.y += ar * pp.y + ai * pp.x;
+        }
+        double2 w = x64[o + i];
+        w.x += (double)xx.x; w.y += (double)xx.y;
+        x64[o + i] = w;
+    }
+}
+
 /* ------------------------------------------------------------ helpers ---- */
 template <typename T>
-__global__ void k_scale_real(typename V2<T>::t* y, const typename V2<T>::t* x, const double* s, size_t n,
-                             size_t xstride, size_t sstride, size_t smod, const int* mask) {
+__global__ void k_scale_real(typename V2<T>::t* y, size_t ystride, const typename V2<T>::t* x, size_t xstride,
+                             const double* s, size_t n, size_t sstride, size_t smod, const int* mask) {
     typedef typename V2<T>::t T2;
     const int p = blockIdx.y;
     if (mask && !mask[p]) return;
@@ -164,7 +205,7 @@ __global__ void k_scale_real(typename V2<T>::t* y, const typename V2<T>::t* x, c
         T2 v = x[(size_t)p * xstride + i];
         T f = (T)s[(size_t)p * sstride + (i % smod)];
         v.x *= f; v.y *= f;
-        y[(size_t)p * xstride + i] = v;
+        y[(size_t)p * ystride + i] = v;
     }
 }
 
@@ -202,6 +243,22 @@ __global__ void k_sub(double2* r, const double2* b, const double2* ax, size_t n,
     }
 }
 
+/* r = b - ax and the partials of <r,r> (problems outside the mask write zero partials) */
+__global__ void __launch_bounds__(RB) k_sub_norm2(double2* r, const double2* b, const double2* ax, double* partials,
+                                                  size_t n, size_t stride, const int* mask) {
+    const int p = blockIdx.y;
+    double acc[1] = {0.0};
+    if (!mask || mask[p]) {
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
+            const double2 u = b[(size_t)p * stride + i], v = ax[(size_t)p * stride + i];
+            const double2 d = make_double2(u.x - v.x, u.y - v.y);
+            r[(size_t)p * stride + i] = d;
+            acc[0] += d.x * d.x + d.y * d.y;
+        }
+    }
+    block_partials<1>(acc, partials, p);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(RB) k_norm2(const typename V2<T>::t* x, double* partials, size_t n, size_t stride) {
     const int p = blockIdx.y;
@@ -222,7 +279,10 @@ __global__ void __launch_bounds__(RB) k_reduce_partials(const double* partials, 
 
 /* ------------------------------------------------------------ Lanczos ---- */
 /* h[j] = sum_i conj(V[j][i]) w[i], j < m  (partials of 2m doubles per block).
- * Columns go four at a time so that w is read once per four columns.        */
+ * Columns go CB at a time, so that w is read once per CB columns and CB + 1
+ * loads are in flight per element; CB = 4 for m <= 4 (the three-term step,
+ * where a wider group only adds reduction work), else 8.                    */
+template <int CB>
 __global__ void __launch_bounds__(RB) k_gemv_h(const double2* V, const double2* w, double* partials, int m, size_t n,
                                                size_t cstride, size_t pstride, size_t wstride, const int* mask) {
     const int p = blockIdx.y;
@@ -233,25 +293,27 @@ __global__ void __launch_bounds__(RB) k_gemv_h(const double2* V, const double2* 
     }
     const double2* Vp = V + (size_t)p * pstride;
     const double2* wp = w + (size_t)p * wstride;
-    for (int j0 = 0; j0 < m; j0 += 4) {
-        const int nj = min(4, m - j0);
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j0 = 0; j0 < m; j0 += CB) {
+        const int nj = min(CB, m - j0);
+        double acc[2 * CB];
+#pragma unroll
+        for (int q = 0; q < 2 * CB; ++q) acc[q] = 0.0;
         for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
             const double2 x = wp[i];
+            double2 v[CB];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q < nj) {
-                    const double2 v = Vp[(size_t)(j0 + q) * cstride + i];
-                    acc[2 * q] += v.x * x.x + v.y * x.y;
-                    acc[2 * q + 1] += v.x * x.y - v.y * x.x;
-                }
+            for (int q = 0; q < CB; ++q) v[q] = q < nj ? Vp[(size_t)(j0 + q) * cstride + i] : make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < CB; ++q) {
+                acc[2 * q] += v[q].x * x.x + v[q].y * x.y;
+                acc[2 * q + 1] += v[q].x * x.y - v[q].y * x.x;
             }
         }
-        block_sum_all<8>(acc);
+        block_sum_all<2 * CB>(acc);
         if ((int)threadIdx.x < 2 * nj) {
             double r = acc[0];
 #pragma unroll
-            for (int t = 1; t < 8; ++t) if ((int)threadIdx.x == t) r = acc[t];
+            for (int t = 1; t < 2 * CB; ++t) if ((int)threadIdx.x == t) r = acc[t];
             out[2 * j0 + threadIdx.x] = r;
         }
         __syncthreads();
@@ -432,14 +494,33 @@ hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, 
     return hipGetLastError();
 }
 
-hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
-                               size_t smod, int nprob, int prec, hipStream_t s, const int* mask) {
+hipError_t fame_cg_begin_mixed(void* x32, void* r32, void* p32, const void* b64, double* partials, int nblk, size_t n,
+                               size_t stride, int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_begin_from64, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x32, (float2*)r32, (float2*)p32,
+                       (const double2*)b64, partials, n, stride, mask);
+    return hipGetLastError();
+}
+hipError_t fame_cg_finish_mixed(void* x64, const void* x32, const void* p32, const CgScal* sc, size_t n, size_t stride,
+                                int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_finish_mixed, dim3(1024, nprob), dim3(RB), 0, s, (double2*)x64, (const float2*)x32,
+                       (const float2*)p32, sc, n, stride, mask);
+    return hipGetLastError();
+}
+
+hipError_t fame_scale_real_mod2(void* y, size_t ystride, const void* x, size_t xstride, const double* sreal, size_t n,
+                                size_t sstride, size_t smod, int nprob, int prec, hipStream_t s, const int* mask) {
     dim3 g(1024, nprob);
     if (prec == 8)
-        hipLaunchKernelGGL(k_scale_real<double>, g, dim3(RB), 0, s, (double2*)y, (const double2*)x, sreal, n, xstride, sstride, smod, mask);
+        hipLaunchKernelGGL(k_scale_real<double>, g, dim3(RB), 0, s, (double2*)y, ystride, (const double2*)x, xstride, sreal,
+                           n, sstride, smod, mask);
     else
-        hipLaunchKernelGGL(k_scale_real<float>, g, dim3(RB), 0, s, (float2*)y, (const float2*)x, sreal, n, xstride, sstride, smod, mask);
+        hipLaunchKernelGGL(k_scale_real<float>, g, dim3(RB), 0, s, (float2*)y, ystride, (const float2*)x, xstride, sreal,
+                           n, sstride, smod, mask);
     return hipGetLastError();
+}
+hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
+                               size_t smod, int nprob, int prec, hipStream_t s, const int* mask) {
+    return fame_scale_real_mod2(y, xstride, x, xstride, sreal, n, sstride, smod, nprob, prec, s, mask);
 }
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
                            int nprob, int prec, hipStream_t s) {
@@ -473,6 +554,12 @@ hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t str
     hipLaunchKernelGGL(k_sub, g, dim3(RB), 0, s, (double2*)r, (const double2*)b, (const double2*)ax, n, stride, mask);
     return hipGetLastError();
 }
+hipError_t fame_sub_norm2(void* r, const void* b, const void* ax, double* partials, int nblk, size_t n, size_t stride,
+                          int nprob, const int* mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_sub_norm2, dim3(nblk, nprob), dim3(RB), 0, s, (double2*)r, (const double2*)b, (const double2*)ax,
+                       partials, n, stride, mask);
+    return hipGetLastError();
+}
 hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,
                                hipStream_t s) {
     if (prec == 8)
@@ -487,8 +574,12 @@ hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, dou
 }
 hipError_t fame_gemv_h(const void* V, const void* w, double* partials, int nblk, int m, size_t n, size_t cstride,
                        size_t pstride, size_t wstride, int nprob, const int* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_gemv_h, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, (const double2*)w, partials, m, n,
-                       cstride, pstride, wstride, mask);
+    if (m <= 4)
+        hipLaunchKernelGGL(k_gemv_h<4>, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, (const double2*)w, partials,
+                           m, n, cstride, pstride, wstride, mask);
+    else
+        hipLaunchKernelGGL(k_gemv_h<8>, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, (const double2*)w, partials,
+                           m, n, cstride, pstride, wstride, mask);
     return hipGetLastError();
 }
 hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n, size_t cstride, size_t pstride,

@@ -54,7 +54,7 @@ struct fame_ctx {
     double2* F64 = nullptr; float2* F32 = nullptr;        /* fields, 3N per problem        */
     void* cg64[4] = {nullptr, nullptr, nullptr, nullptr};   /* r, p, Ap, spare (2N/problem) */
     void* cg32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  /* r, p, Ap, rhs32, sol32 */
-    void* ref64[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* ref64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double* partials = nullptr;   /* [problem][NBLK_VEC][<=4]   */
     double* partials2 = nullptr;  /* [problem][inner blocks][2, or 6 inside CG] */
     CgScal* cgsc = nullptr;
@@ -90,9 +90,16 @@ struct fame_ctx {
     fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
     fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);
     fame_status field_from_ref(const void* fref, void* fint, int nprob);
+    /* b64: b is complex double while prec is 4 (rounded to FP32 at the start);
+     * x64_accum: with prec 4, the FP32 solution is added to this complex
+     * double vector (problems of mask_dev) instead of being finished in x */
     fame_status cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                   const int* mask_dev, int* iters);
-    fame_status invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls, int* iters);
+                   const int* mask_dev, int* iters, bool b64 = false, void* x64_accum = nullptr);
+    /* x: problem stride xstride (0: 2N); out (optional): the result is written
+     * there, problem stride ostride, for the problems of mask_dev only, and y
+     * serves as the contiguous solve buffer */
+    fame_status invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls, int* iters,
+                      size_t xstride = 0, void* out = nullptr, size_t ostride = 0);
     fame_status tune(int prec);
     const AxisPlan& plan(int axis, int prec) const { return ax[axis][tile[axis][prec == 8 ? 0 : 1].eng]; }
     ~fame_ctx();

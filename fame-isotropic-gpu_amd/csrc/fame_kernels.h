@@ -94,7 +94,7 @@ hipError_t fame_launch_pack(const void* in, void* out, int N, int Nd, int dir, i
 
 /* ---- vector / reduction kernels (fame_blas.hip) ------------------------- */
 
-/* y = alpha * x (+ y if accumulate), complex vectors of n per problem, real alpha per element table */
+/* x = 0, r = p = b and the partials of <r,r> (stride: problem stride in elements) */
 hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* partials, int nblk,
                          size_t n, size_t stride, int nprob, int prec, const int* mask, hipStream_t s);
 hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob,
@@ -106,6 +106,13 @@ hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, doubl
 hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
                             hipStream_t s);
+/* FP32 solves inside FP64 refinement: start from the FP64 right-hand side
+ * (x32 = 0, r32 = p32 = b64 rounded), and end by accumulating the FP32
+ * solution with its pending update into x64 (problems of the mask only) */
+hipError_t fame_cg_begin_mixed(void* x32, void* r32, void* p32, const void* b64, double* partials, int nblk, size_t n,
+                               size_t stride, int nprob, const int* mask, hipStream_t s);
+hipError_t fame_cg_finish_mixed(void* x64, const void* x32, const void* p32, const CgScal* sc, size_t n, size_t stride,
+                                int nprob, const int* mask, hipStream_t s);
 
 /* generic helpers (complex double unless prec says float) */
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
@@ -117,6 +124,9 @@ hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, si
                            int nprob, const int* mask, hipStream_t s);
 hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
                     hipStream_t s);
+/* r = b - ax with the partials of <r,r> (nblk per problem; zero outside the mask) */
+hipError_t fame_sub_norm2(void* r, const void* b, const void* ax, double* partials, int nblk, size_t n, size_t stride,
+                          int nprob, const int* mask, hipStream_t s);
 hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,
                                hipStream_t s);
 hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, double* out, int nprob, hipStream_t s);
@@ -146,5 +156,9 @@ hipError_t fame_lz_beta(const double* nrm2, double* T1d, int ld, int j, double* 
                         hipStream_t s);
 hipError_t fame_scale_real_mod(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,
                                size_t smod, int nprob, int prec, hipStream_t s, const int* mask = nullptr);
+/* y = s x with separate problem strides for y and x */
+hipError_t fame_scale_real_mod2(void* y, size_t ystride, const void* x, size_t xstride, const double* sreal, size_t n,
+                                size_t sstride, size_t smod, int nprob, int prec, hipStream_t s,
+                                const int* mask = nullptr);
 hipError_t fame_cg_init_scal2(CgScal* sc, const double* partials, int nblk, double tol, double reltol, int maxit,
                               int nprob, int* active, const int* mask, hipStream_t s);

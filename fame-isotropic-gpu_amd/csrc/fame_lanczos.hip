@@ -43,7 +43,11 @@ std::string& fame_err_ref();
 int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z);
 void fame_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s, double shift, int n);
 
+/* partial-sum blocks per problem: 256 for single-column products (the
+ * three-term step, norms), 1024 for the Gram-Schmidt products over many
+ * columns, which then keep more loads in flight per CU */
 #define GEMV_BLK 256
+#define GEMV_BLK_GS 1024
 /* Gram blocks per problem: enough to fill the GPU for small batches */
 static int gram_blk(int nk) { return std::max(32, std::min(512, 2048 / std::max(nk, 1))); }
 
@@ -61,7 +65,7 @@ struct LzBuffers {
     double* hh = nullptr;   /* nk x 2(nstep+1) */
     double* nrm = nullptr;  /* nk */
     double* invb = nullptr; /* nk */
-    double* gpart = nullptr;/* nk x GEMV_BLK x 2(nstep+1) ; also gram partials */
+    double* gpart = nullptr;/* nk x GEMV_BLK_GS x 2(nstep+1) ; also gram partials */
     double* G = nullptr;    /* nk x 2 m^2 */
     double* Q = nullptr;    /* nk x m x (nwant+1) */
     int* mask = nullptr;    /* nk */
@@ -88,7 +92,7 @@ fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
     CHK(al((void**)&b.hh, sizeof(double) * 2 * m * nk));
     CHK(al((void**)&b.nrm, sizeof(double) * nk));
     CHK(al((void**)&b.invb, sizeof(double) * nk));
-    size_t gp = std::max((size_t)GEMV_BLK * 2 * m, (size_t)gram_blk(nk) * 2 * m * m);
+    size_t gp = std::max((size_t)GEMV_BLK_GS * 2 * m, (size_t)gram_blk(nk) * 2 * m * m);
     CHK(al((void**)&b.gpart, sizeof(double) * gp * nk));
     CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
     CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
@@ -106,14 +110,12 @@ static inline double2* col(const LzBuffers& b, int j) { return b.U + (size_t)j *
 static fame_status lz_invar(fame_ctx* c, LzBuffers& b, int j, int mode, const fame_ls& ls, const int* mask_dev,
                             std::vector<int>& ls_iter, std::vector<int>& es_iter, const std::vector<int>& mask_h) {
     const size_t ps = b.n2 * (b.nstep + 1);
-    double2* xin = (double2*)c->ref64[5];
-    double2* yout = (double2*)c->ref64[0];
-    HIPCHK(fame_convert(xin, 8, col(b, j), 8, b.n2, b.n2, ps, b.nk, c->stream));
+    double2* ysolve = (double2*)c->ref64[0];
     std::vector<int> it(b.nk, 0);
-    CHK(c->invar(mode, xin, yout, 0, b.nk, mask_dev, ls, it.data()));
-    /* only the problems still iterating: a converged problem keeps its basis,
+    /* read straight from column j; the result goes to column j + 1 of the
+     * problems still iterating only: a converged problem keeps its basis,
      * whose columns its final Ritz vectors are built from */
-    HIPCHK(fame_convert(col(b, j + 1), 8, yout, 8, b.n2, ps, b.n2, b.nk, c->stream, mask_dev));
+    CHK(c->invar(mode, col(b, j), ysolve, 0, b.nk, mask_dev, ls, it.data(), ps, col(b, j + 1), ps));
     for (int p = 0; p < b.nk; ++p) if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; }
     return FAME_OK;
 }
@@ -144,8 +146,8 @@ static fame_status lz_reorth(fame_ctx* c, LzBuffers& b, int j, const int* mask_d
     const int m = j + 1;
     const int* mk = mask_dev;
     for (int rep = 0; rep < 2; ++rep) {
-        HIPCHK(fame_gemv_h(col(b, 0), col(b, j + 1), b.gpart, GEMV_BLK, m, b.n2, b.n2, ps, ps, b.nk, mk, c->stream));
-        HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 2 * m, b.hh, b.nk, c->stream));
+        HIPCHK(fame_gemv_h(col(b, 0), col(b, j + 1), b.gpart, GEMV_BLK_GS, m, b.n2, b.n2, ps, ps, b.nk, mk, c->stream));
+        HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK_GS, 2 * m, b.hh, b.nk, c->stream));
         HIPCHK(fame_gemv_n(col(b, 0), col(b, j + 1), b.hh, m, b.n2, b.n2, ps, ps, 2 * m, b.nk, mk, c->stream));
         if (rep == 0) {
             HIPCHK(fame_lz_second_pass_mask(b.hh, 2 * m, 1e-16, mask_dev, b.mask2, b.nk, c->stream));

@@ -373,7 +373,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     A(c->F32, 8 * 3 * N * B);
     for (int i = 0; i < 4; ++i) A(c->cg64[i], 16 * 2 * N * B);
     for (int i = 0; i < 5; ++i) A(c->cg32[i], 8 * 2 * N * B);
-    for (int i = 0; i < 6; ++i) A(c->ref64[i], 16 * 2 * N * B);
+    for (int i = 0; i < 5; ++i) A(c->ref64[i], 16 * 2 * N * B);
     /* inner-kernel partials: room for one line per block (any tuned tile) */
     int nbmax = std::max(g.nb * g.nc, NBLK_VEC);
     A(c->partials, sizeof(double) * 4 * nbmax * B);
@@ -748,13 +748,15 @@ extern "C" fame_status fame_autotune(fame_ctx* c, int prec, int* choice) {
  * Stops per problem when <r,r> <= max(tol^2, reltol^2 <b,b>) or after maxit
  * (reference counter semantics, CG.cu:36-78).  iters[] (host) receives k.    */
 fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                         const int* mask_dev, int* iters) {
+                         const int* mask_dev, int* iters, bool b64, void* x64_accum) {
+    if ((b64 || x64_accum) && prec != 4) FAIL(FAME_ERR_ARG, "cg: FP64 right-hand side or accumulation needs prec 4");
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     const size_t es = prec == 8 ? 16 : 8;
     char* r = (char*)V[0] + es * st * p0;
     char* p = (char*)V[1] + es * st * p0;
-    HIPCHK(fame_cg_begin(x, r, p, b, partials, NBLK_VEC, n, st, nprob, prec, mask_dev, stream));
+    if (b64) HIPCHK(fame_cg_begin_mixed(x, r, p, b, partials, NBLK_VEC, n, st, nprob, mask_dev, stream));
+    else HIPCHK(fame_cg_begin(x, r, p, b, partials, NBLK_VEC, n, st, nprob, prec, mask_dev, stream));
     HIPCHK(fame_cg_init_scal2(cgsc, partials, NBLK_VEC, tol, reltol, maxit, nprob, active, mask_dev, stream));
     int done_it = 0;
     int chunk = std::max(1, std::min(maxit, cg_predict[pix(prec)] - 1));
@@ -768,7 +770,8 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
         if (!any || done_it >= maxit) break;
         chunk = 2;
     }
-    HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
+    if (x64_accum) HIPCHK(fame_cg_finish_mixed(x64_accum, x, p, cgsc, n, st, nprob, mask_dev, stream));
+    else HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
     std::vector<CgScal> sc(nprob);
     HIPCHK(hipMemcpyAsync(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
@@ -851,21 +854,23 @@ extern "C" fame_status fame_cg(fame_ctx* c, int ik, const void* rhs, void* x, in
 /*  invAr (batched): y = Sigma^-1 QBQ^-1 Sigma^-1 x                          */
 /* ======================================================================== */
 fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls,
-                            int* iters) {
+                            int* iters, size_t xstride, void* out, size_t ostride) {
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     double2* b = (double2*)ref64[2] + st * p0;
-    HIPCHK(fame_scale_real_mod(b, x, sig_inv + (size_t)p0 * N, n, st, N, N, nprob, 8, stream));
+    HIPCHK(fame_scale_real_mod2(b, st, x, xstride ? xstride : st, sig_inv + (size_t)p0 * N, n, N, N, nprob, 8, stream));
     std::vector<int> it(nprob, 0);
     if (mode == FAME_PREC_F64) {
         CHK(cg(8, b, y, p0, nprob, ls.tol, 0.0, ls.maxit, mask_dev, it.data()));
     } else {
-        /* FP64 iterative refinement around FP32 CG solves */
+        /* FP64 iterative refinement around FP32 CG solves: each sweep starts
+         * the FP32 solve from the FP64 residual, adds its solution into the
+         * FP64 iterate, and forms the next residual with its norm in one pass */
         double2* xr = (double2*)y;
         double2* rr = (double2*)ref64[3] + st * p0;
-        char* r32 = (char*)cg32[3] + 8 * st * p0;
         float2* d32 = (float2*)cg32[4] + st * p0;
         HIPCHK(hipMemsetAsync(xr, 0, 16 * st * nprob, stream));
         HIPCHK(hipMemcpyAsync(rr, b, 16 * st * nprob, hipMemcpyDeviceToDevice, stream));
+        HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));
         std::vector<int> msk(nprob, 1);
         if (mask_dev) {
             HIPCHK(hipMemcpyAsync(msk.data(), mask_dev, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
@@ -873,7 +878,6 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
         }
         double tol2 = ls.tol * ls.tol;
         for (int sweep = 0; sweep < refine_max_sweeps; ++sweep) {
-            HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));
             HIPCHK(fame_reduce_partials(partials, NBLK_VEC, 1, scal, nprob, stream));
             std::vector<double> rn(nprob);
             HIPCHK(hipMemcpyAsync(rn.data(), scal, sizeof(double) * nprob, hipMemcpyDeviceToHost, stream));
@@ -882,19 +886,18 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
             for (int i = 0; i < nprob; ++i) { msk[i] = msk[i] && (rn[i] > tol2); any |= msk[i]; }
             if (!any) break;
             HIPCHK(hipMemcpyAsync(mask, msk.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, stream));
-            HIPCHK(fame_convert(r32, 4, rr, 8, n, st, st, nprob, stream));
             std::vector<int> it32(nprob, 0);
-            CHK(cg(4, r32, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data()));
+            CHK(cg(4, rr, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, xr));
             for (int i = 0; i < nprob; ++i) if (msk[i]) it[i] += it32[i] - 1;
-            HIPCHK(fame_axpy_mixed(xr, d32, 4, 1.0, n, st, st, nprob, mask, stream));
-            /* rr = b - QBQ64 xr */
+            /* rr = b - QBQ64 xr, and its norm for the next sweep */
             double2* t = (double2*)ref64[4] + st * p0;
             CHK(qbq(8, xr, t, p0, nprob, mask, nullptr));
-            HIPCHK(fame_sub(rr, b, t, n, st, nprob, mask, stream));
+            HIPCHK(fame_sub_norm2(rr, b, t, partials, NBLK_VEC, n, st, nprob, mask, stream));
         }
         for (int i = 0; i < nprob; ++i) it[i] += 1;
     }
-    HIPCHK(fame_scale_real_mod(y, y, sig_inv + (size_t)p0 * N, n, st, N, N, nprob, 8, stream));
+    if (out) HIPCHK(fame_scale_real_mod2(out, ostride, y, st, sig_inv + (size_t)p0 * N, n, N, N, nprob, 8, stream, mask_dev));
+    else HIPCHK(fame_scale_real_mod(y, y, sig_inv + (size_t)p0 * N, n, st, N, N, nprob, 8, stream));
     if (iters) for (int i = 0; i < nprob; ++i) iters[i] = it[i];
     return FAME_OK;
 }
