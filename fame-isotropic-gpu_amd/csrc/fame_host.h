@@ -10,6 +10,7 @@
 #include "fame_kernels.h"
 
 #define NBLK_VEC 1024   /* blocks per problem of the vector/reduction kernels */
+#define CG_GRAPH_ITERS 4   /* CG iterations per captured graph (fewer launch gaps between them) */
 
 enum { FFT_ENGINE_AUTO = 0, FFT_ENGINE_STOCKHAM = 1, FFT_ENGINE_MFMA = 2 };
 enum { AX_INNER = 0, AX_MID = 1, AX_OUTER = 2 };
@@ -61,7 +62,7 @@ struct fame_ctx {
     int* active = nullptr;
     int* mask = nullptr;
     double* scal = nullptr;       /* 64 doubles per problem of small device scalars */
-    int* active_h = nullptr;      /* pinned */
+    CgScal* sc_h = nullptr;       /* pinned host copy of cgsc */
     std::vector<void*> allocs;
     std::vector<void*> big;       /* Lanczos basis etc., reallocated per solve shape */
     /* tuning / state */
@@ -76,9 +77,9 @@ struct fame_ctx {
      * checks of the active flags.  FAME_GRAPH=0 launches eagerly instead.   */
     bool use_graphs = true;
     hipStream_t cap_stream = nullptr;
-    std::map<std::tuple<int, int, int, const void*>, hipGraphExec_t> cg_graphs;
+    std::map<std::tuple<int, int, int, const void*, int>, hipGraphExec_t> cg_graphs;
     void drop_graphs();
-    fame_status cg_iteration(int prec, void* x, int p0, int nprob);
+    fame_status cg_iteration(int prec, void* x, int p0, int nprob, int count = 1);
 
     KInner kin(int prec, int p0, const int* act, int list) const;
     KAxis kmid(int prec, int p0, const int* act, int list) const;
@@ -97,9 +98,10 @@ struct fame_ctx {
                    const int* mask_dev, int* iters, bool b64 = false, void* x64_accum = nullptr);
     /* x: problem stride xstride (0: 2N); out (optional): the result is written
      * there, problem stride ostride, for the problems of mask_dev only, and y
-     * serves as the contiguous solve buffer */
+     * serves as the contiguous solve buffer; mask_host: a host copy of
+     * mask_dev, which saves reading it back */
     fame_status invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls, int* iters,
-                      size_t xstride = 0, void* out = nullptr, size_t ostride = 0);
+                      size_t xstride = 0, void* out = nullptr, size_t ostride = 0, const int* mask_host = nullptr);
     fame_status tune(int prec);
     const AxisPlan& plan(int axis, int prec) const { return ax[axis][tile[axis][prec == 8 ? 0 : 1].eng]; }
     ~fame_ctx();

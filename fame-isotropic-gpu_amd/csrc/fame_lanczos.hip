@@ -115,7 +115,7 @@ static fame_status lz_invar(fame_ctx* c, LzBuffers& b, int j, int mode, const fa
     /* read straight from column j; the result goes to column j + 1 of the
      * problems still iterating only: a converged problem keeps its basis,
      * whose columns its final Ritz vectors are built from */
-    CHK(c->invar(mode, col(b, j), ysolve, 0, b.nk, mask_dev, ls, it.data(), ps, col(b, j + 1), ps));
+    CHK(c->invar(mode, col(b, j), ysolve, 0, b.nk, mask_dev, ls, it.data(), ps, col(b, j + 1), ps, mask_h.data()));
     for (int p = 0; p < b.nk; ++p) if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; }
     return FAME_OK;
 }

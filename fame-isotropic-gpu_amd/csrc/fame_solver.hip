@@ -383,7 +383,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     A(c->mask, sizeof(int) * B);
     A(c->scal, sizeof(double) * 64 * B);
 #undef A
-    HIPCHK(hipHostMalloc((void**)&c->active_h, sizeof(int) * B, 0));
+    HIPCHK(hipHostMalloc((void**)&c->sc_h, sizeof(CgScal) * B, 0));
     c->big.clear();
     *out = c;
     return FAME_OK;
@@ -391,7 +391,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
 
 extern "C" fame_status fame_destroy(fame_ctx* c) {
     if (!c) return FAME_OK;
-    if (c->active_h) (void)hipHostFree(c->active_h);
+    if (c->sc_h) (void)hipHostFree(c->sc_h);
     delete c;
     return FAME_OK;
 }
@@ -761,22 +761,25 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     int done_it = 0;
     int chunk = std::max(1, std::min(maxit, cg_predict[pix(prec)] - 1));
     while (true) {
-        for (int it = 0; it < chunk; ++it) CHK(cg_iteration(prec, x, p0, nprob));
+        for (int it = 0; it < chunk;) {
+            const int k = std::min(chunk - it, CG_GRAPH_ITERS);
+            CHK(cg_iteration(prec, x, p0, nprob, k));
+            it += k;
+        }
         done_it += chunk;
-        HIPCHK(hipMemcpyAsync(active_h, active, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
+        /* the whole scalar state comes back, so that the last check also
+         * holds the final iteration counts (no read-back after the loop) */
+        HIPCHK(hipMemcpyAsync(sc_h, cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
         int any = 0;
-        for (int i = 0; i < nprob; ++i) any |= active_h[i];
+        for (int i = 0; i < nprob; ++i) any |= sc_h[i].active;
         if (!any || done_it >= maxit) break;
         chunk = 2;
     }
     if (x64_accum) HIPCHK(fame_cg_finish_mixed(x64_accum, x, p, cgsc, n, st, nprob, mask_dev, stream));
     else HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
-    std::vector<CgScal> sc(nprob);
-    HIPCHK(hipMemcpyAsync(sc.data(), cgsc, sizeof(CgScal) * nprob, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
     int mx = 0;
-    for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc[i].iters; mx = std::max(mx, sc[i].iters); }
+    for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc_h[i].iters; mx = std::max(mx, sc_h[i].iters); }
     cg_predict[pix(prec)] = std::max(2, mx - 1);
     return FAME_OK;
 }
@@ -793,7 +796,7 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
  * p = r and beta = 0, so the first pass leaves p = r).  The last x update is
  * applied by k_cg_finish_x after the loop.  All scalars stay on the device, so
  * the six launches are captured once as a graph.                            */
-fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
+fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int count) {
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     const size_t es = prec == 8 ? 16 : 8;
@@ -803,14 +806,17 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
     auto launch = [&](hipStream_t s) -> fame_status {
         const hipStream_t keep = stream;
         stream = s;
-        fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, r);
+        for (int k = 0; k < count; ++k) {
+            fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, r);
+            if (q != FAME_OK) { stream = keep; return q; }
+            const hipError_t e = fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, s);
+            if (e != hipSuccess) { stream = keep; FAIL(FAME_ERR_HIP, std::string("cg_alpha_beta: ") + hipGetErrorString(e)); }
+        }
         stream = keep;
-        CHK(q);
-        HIPCHK(fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, s));
         return FAME_OK;
     };
     if (!use_graphs) return launch(stream);
-    const auto key = std::make_tuple(prec, p0, nprob, (const void*)x);
+    const auto key = std::make_tuple(prec, p0, nprob, (const void*)x, count);
     auto itg = cg_graphs.find(key);
     if (itg == cg_graphs.end()) {
         if (!cap_stream) HIPCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
@@ -829,7 +835,7 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob) {
         itg = cg_graphs.emplace(key, ex).first;
     }
     HIPCHK(hipGraphLaunch(itg->second, stream));
-    if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
+    if (prec == 8) cnt_qbq64 += (long long)nprob * count; else cnt_qbq32 += (long long)nprob * count;
     return FAME_OK;
 }
 
@@ -854,7 +860,7 @@ extern "C" fame_status fame_cg(fame_ctx* c, int ik, const void* rhs, void* x, in
 /*  invAr (batched): y = Sigma^-1 QBQ^-1 Sigma^-1 x                          */
 /* ======================================================================== */
 fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob, const int* mask_dev, fame_ls ls,
-                            int* iters, size_t xstride, void* out, size_t ostride) {
+                            int* iters, size_t xstride, void* out, size_t ostride, const int* mask_host) {
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     double2* b = (double2*)ref64[2] + st * p0;
     HIPCHK(fame_scale_real_mod2(b, st, x, xstride ? xstride : st, sig_inv + (size_t)p0 * N, n, N, N, nprob, 8, stream));
@@ -872,7 +878,9 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
         HIPCHK(hipMemcpyAsync(rr, b, 16 * st * nprob, hipMemcpyDeviceToDevice, stream));
         HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));
         std::vector<int> msk(nprob, 1);
-        if (mask_dev) {
+        if (mask_host) {
+            std::copy(mask_host, mask_host + nprob, msk.begin());
+        } else if (mask_dev) {
             HIPCHK(hipMemcpyAsync(msk.data(), mask_dev, sizeof(int) * nprob, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
         }
