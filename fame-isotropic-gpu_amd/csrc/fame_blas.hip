@@ -102,34 +102,36 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
     active[p] = c.active;
 }
 
-/* partials per inner block: <p,Ap> (2), <r,Ap> (2), <Ap,Ap>, <r,r> of the
- * current residual (written by the expanding pass) */
+/* partials per inner block (6 doubles): <p,Ap> (slots 0, 1) and <Ap,Ap>
+ * (slot 2) from the contracting pass, <r,r> of the current residual (slot 5)
+ * from the expanding pass; slots 3 and 4 are not used */
 __global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int* active) {
     const int p = blockIdx.x;
     if (!active[p]) return;
-    double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double* q = partials + ((size_t)p * nblk + b) * 6;
-#pragma unroll
-        for (int c = 0; c < 6; ++c) v[c] += q[c];
+        v[0] += q[0]; v[1] += q[1]; v[2] += q[2]; v[3] += q[5];
     }
-    block_sum_all<6>(v);
+    block_sum_all<4>(v);
     if (threadIdx.x != 0) return;
     CgScal& c = sc[p];
     const double re = v[0], im = v[1];          /* <p, Ap> */
-    const double rre = v[2], rim = v[3];        /* <r, Ap> */
-    const double aa = v[4];                     /* <Ap, Ap> */
-    const double rr = v[5];                     /* <r, r>, summed from the vector */
+    const double aa = v[2];                     /* <Ap, Ap> */
+    const double rr = v[3];                     /* <r, r>, summed from the vector */
     c.pap_re = re; c.pap_im = im;
     /* a = r1 / dot  (CG.cu:60-63) */
     const double temp = re * re + im * im;
     const double ar = rr * re / temp, ai = -rr * im / temp;
     c.alpha_re = ar; c.alpha_im = ai;
     c.pend = 1;
-    /* r1 of r - a Ap from one step of the recurrence off the summed <r,r>,
-     * accurate to rounding of that step; b = r1 / r0 and the stopping rule
-     * (CG.cu:66-78) use it, the next alpha the summed norm again */
-    const double s = fmax(0.0, rr - 2.0 * (ar * rre - ai * rim) + (ar * ar + ai * ai) * aa);
+    /* r1 of r - a Ap from one step of the recurrence off the summed <r,r>:
+     * |r - a Ap|^2 = rr - 2 Re(a <r,Ap>) + |a|^2 <Ap,Ap>, where <r,Ap> = <p,Ap>
+     * because p = r + beta p_old with p_old A-conjugate to p, and a <p,Ap> = rr,
+     * so r1 = |a|^2 <Ap,Ap> - rr, to rounding and the loss of conjugacy between
+     * consecutive directions.  b = r1 / r0 and the stopping rule (CG.cu:66-78)
+     * use it, the next alpha the summed norm again */
+    const double s = fmax(0.0, (ar * ar + ai * ai) * aa - rr);
     c.beta = s / rr;
     c.rr = s;
     c.iters += 1;

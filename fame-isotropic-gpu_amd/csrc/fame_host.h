@@ -86,7 +86,7 @@ struct fame_ctx {
     KAxis kout(int prec, int p0, const int* act, int list) const;
     fame_status qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
                     void* cg_r = nullptr, void* cg_x = nullptr, const void* cg_ap = nullptr,
-                    const void* dotr = nullptr);
+                    bool cgdots = false);
     fame_status qr_field(const void* x, int p0, int nprob, int pikind, const int* act);
     fame_status qrs_field(void* y, int p0, int nprob, int pikind, const int* act);
     fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);

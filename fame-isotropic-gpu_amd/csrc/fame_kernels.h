@@ -55,11 +55,10 @@ struct KInner {
     /* CG residual update on the same pass: with cg_ap set, r -= alpha Ap of
      * the previous iteration (when pend) is applied before p = r + beta p    */
     const void* cg_ap;
-    /* !EXPAND with dotvec: also <dotr, y> and <y, y> (slots 2..4 of 6 doubles
-     * per block instead of 2; the expanding pass inside CG puts <r, r> in slot
-     * 5), from which the CG takes <r, r> of the next residual without a pass
-     * over r                                                                  */
-    const void* dotr;
+    /* !EXPAND with dotvec inside CG: also <y, y> (slot 2 of 6 doubles per
+     * block instead of 2; the expanding pass puts <r, r> in slot 5), from
+     * which the CG takes <r, r> of the next residual without a pass over r  */
+    int cgdots;
 };
 
 struct KAxis {
@@ -100,9 +99,9 @@ hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* parti
 hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob,
                              int* active, hipStream_t s);
 /* alpha from the summed <r,r>, <r,r> of the next residual by one step of
- * |r - alpha Ap|^2 = |r|^2 - 2 Re(alpha <r,Ap>) + |alpha|^2 |Ap|^2, beta and the
- * active flags, from the 6-double partials of the expanding and contracting
- * passes */
+ * |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 - |r|^2 (conjugacy: <r,Ap> = <p,Ap>),
+ * beta and the active flags, from the 6-double partials of the expanding and
+ * contracting passes */
 hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
                             hipStream_t s);

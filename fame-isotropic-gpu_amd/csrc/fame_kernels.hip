@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
             });
         /* inside CG: block sum of <r,r> into slot 5 of partials[p][tile] (the
-         * contracting pass fills slots 0..4) */
+         * contracting pass fills slots 0..2) */
         __shared__ double rred[FFT_THREADS / 64];
         if (cgf) {
 #pragma unroll
@@ -231,10 +231,9 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
-        double dre = 0.0, dim = 0.0;
-        const T2* DR = (D && args.dotr) ? reinterpret_cast<const T2*>(args.dotr) + (size_t)p * args.xstride : nullptr;
-        double rre = 0.0, rim = 0.0, yy = 0.0;
-        struct DIn { T2 d1, d2, r1, r2; };
+        double dre = 0.0, dim = 0.0, yy = 0.0;
+        const bool CGD = D && args.cgdots;
+        struct DIn { T2 d1, d2; };
         batched_loop<InflightInner<T>::v>(
             L * na,
             [&](int idx) -> DIn {
@@ -244,7 +243,6 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 if (D && gl < nlines) {
                     const unsigned f = (unsigned)(gl * na + a);
                     d.d1 = D[f]; d.d2 = D[N + f];
-                    if (DR) { d.r1 = DR[f]; d.r2 = DR[N + f]; }
                 }
                 return d;
             },
@@ -278,28 +276,23 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     const double2 q1 = to_d(o1), q2 = to_d(o2);
                     dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
                     dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
-                    if (DR) {
-                        const double2 e1 = to_d(dd.r1), e2 = to_d(dd.r2);
-                        rre += e1.x * q1.x + e1.y * q1.y + e2.x * q2.x + e2.y * q2.y;
-                        rim += e1.x * q1.y - e1.y * q1.x + e2.x * q2.y - e2.y * q2.x;
-                        yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
-                    }
+                    if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
                 }
             });
         if (D) {
-            /* block sums of conj(d) . y (and, with dotr, conj(r) . y and y . y)
-             * into partials[p][tile]: wave shuffles, then the four wave sums in
-             * a fixed order */
-            __shared__ double red[5][FFT_THREADS / 64];
-            double v[5] = {dre, dim, rre, rim, yy};
-            const int nc = DR ? 5 : 2, stride = DR ? 6 : 2;
+            /* block sums of conj(d) . y (and, inside CG, y . y) into
+             * partials[p][tile]: wave shuffles, then the four wave sums in a
+             * fixed order */
+            __shared__ double red[3][FFT_THREADS / 64];
+            double v[3] = {dre, dim, yy};
+            const int nc = CGD ? 3 : 2, stride = CGD ? 6 : 2;
 #pragma unroll
-            for (int c = 0; c < 5; ++c)
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
             if ((threadIdx.x & 63) == 0)
 #pragma unroll
-                for (int c = 0; c < 5; ++c) red[c][threadIdx.x >> 6] = v[c];
+                for (int c = 0; c < 3; ++c) red[c][threadIdx.x >> 6] = v[c];
             __syncthreads();
             if (threadIdx.x == 0) {
                 double* part = args.partials + ((size_t)p * nblk + tile) * stride;

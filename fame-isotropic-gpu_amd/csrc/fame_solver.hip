@@ -530,13 +530,13 @@ KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
 /* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
  * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
 fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
-                          void* cg_r, void* cg_x, const void* cg_ap, const void* dotr) {
+                          void* cg_r, void* cg_x, const void* cg_ap, bool cgdots) {
     KInner ki = kin(prec, p0, act, LIST_INV);
     ki.x = x; ki.pikind = 0;
     if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; ki.cg_ap = cg_ap; ki.partials = partials2; }
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
     ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr; ki.cg_ap = nullptr;
-    ki.dotr = dotvec ? dotr : nullptr;
+    ki.cgdots = dotvec && cgdots;
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
     HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
@@ -787,10 +787,10 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
 /* one CG iteration (CG.cu:53-78) on the masked batch:
  *   first QBQ pass: x += alpha p and r -= alpha Ap (the previous iteration's,
  *                   when pending), then p = r + beta p;
- *   QBQ p -> Ap, the last pass with the fused <p,Ap>, <r,Ap> and <Ap,Ap> partials;
+ *   QBQ p -> Ap, the last pass with the fused <p,Ap> and <Ap,Ap> partials;
  *   alpha from <r,r> summed by the first pass; the next <r,r> by one step of
- *   |r - alpha Ap|^2 = |r|^2 - 2 Re(alpha <r,Ap>) + |alpha|^2 |Ap|^2 for beta
- *   and the stopping rule (no pass over r); the active flags.
+ *   |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 - |r|^2 (conjugacy: <r,Ap> = <p,Ap>)
+ *   for beta and the stopping rule (no pass over r); the active flags.
  * The reference's alpha, beta and stopping rule are kept; the x, r and p updates
  * of iteration k are applied while iteration k+1 reads p anyway (k_cg_begin sets
  * p = r and beta = 0, so the first pass leaves p = r).  The last x update is
@@ -807,7 +807,7 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int cou
         const hipStream_t keep = stream;
         stream = s;
         for (int k = 0; k < count; ++k) {
-            fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, r);
+            fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, true);
             if (q != FAME_OK) { stream = keep; return q; }
             const hipError_t e = fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, s);
             if (e != hipSuccess) { stream = keep; FAIL(FAME_ERR_HIP, std::string("cg_alpha_beta: ") + hipGetErrorString(e)); }
