@@ -104,15 +104,21 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
 
 /* partials per inner block (6 doubles): <p,Ap> (slots 0, 1) and <Ap,Ap>
  * (slot 2) from the contracting pass, <r,r> of the current residual (slot 5)
- * from the expanding pass; slots 3 and 4 are not used */
-__global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int* active) {
+ * from the expanding pass; slots 3 and 4 are not used.  With pap, <p,Ap> is
+ * instead the sum of the outer pass's 3 npap block sums of the problem (real:
+ * the quadratic form of the real-space field), and slots 0 and 1 are unused. */
+__global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap,
+                                int* active) {
     const int p = blockIdx.x;
     if (!active[p]) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double* q = partials + ((size_t)p * nblk + b) * 6;
-        v[0] += q[0]; v[1] += q[1]; v[2] += q[2]; v[3] += q[5];
+        if (!pap) { v[0] += q[0]; v[1] += q[1]; }
+        v[2] += q[2]; v[3] += q[5];
     }
+    if (pap)
+        for (int b = threadIdx.x; b < 3 * npap; b += blockDim.x) v[0] += pap[(size_t)p * 3 * npap + b];
     block_sum_all<4>(v);
     if (threadIdx.x != 0) return;
     CgScal& c = sc[p];
@@ -480,8 +486,9 @@ hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, doubl
     hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
     return hipGetLastError();
 }
-hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
+hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap, int nprob,
+                              int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, pap, npap, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,

@@ -58,6 +58,7 @@ struct fame_ctx {
     void* ref64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double* partials = nullptr;   /* [problem][NBLK_VEC][<=4]   */
     double* partials2 = nullptr;  /* [problem][inner blocks][2, or 6 inside CG] */
+    double* pap_part = nullptr;   /* [problem][3][outer blocks]: <p,QBQ p> block sums of the outer pass */
     CgScal* cgsc = nullptr;
     int* active = nullptr;
     int* mask = nullptr;
@@ -104,6 +105,12 @@ struct fame_ctx {
                       size_t xstride = 0, void* out = nullptr, size_t ostride = 0, const int* mask_host = nullptr);
     fame_status tune(int prec);
     const AxisPlan& plan(int axis, int prec) const { return ax[axis][tile[axis][prec == 8 ? 0 : 1].eng]; }
+    /* outer-pass blocks per (problem, component) of precision prec, when the
+     * outer pass can sum <p, QBQ p> from the real-space field: every current
+     * line transform is an unnormalised DFT (no Bluestein plan), so that the
+     * forward chain is the exact adjoint of the inverse one and the outer
+     * pass's 1/N is the whole normalisation; 0 otherwise */
+    int pap_blocks(int prec) const;
     ~fame_ctx();
 };
 

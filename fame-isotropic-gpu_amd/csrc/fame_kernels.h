@@ -55,9 +55,10 @@ struct KInner {
     /* CG residual update on the same pass: with cg_ap set, r -= alpha Ap of
      * the previous iteration (when pend) is applied before p = r + beta p    */
     const void* cg_ap;
-    /* !EXPAND with dotvec inside CG: also <y, y> (slot 2 of 6 doubles per
-     * block instead of 2; the expanding pass puts <r, r> in slot 5), from
-     * which the CG takes <r, r> of the next residual without a pass over r  */
+    /* !EXPAND inside CG: <y, y> into slot 2 of 6 doubles per block (the
+     * expanding pass puts <r, r> in slot 5; with dotvec, conj(dotvec).y goes
+     * to slots 0 and 1), from which the CG takes <r, r> of the next residual
+     * without a pass over r                                                  */
     int cgdots;
 };
 
@@ -68,6 +69,11 @@ struct KAxis {
     const int* active;
     void* f;
     const void* invB;            /* outer EPS mode: 3N reals, internal layout  */
+    /* outer EPS mode inside CG (optional): per block, the sum over its lines of
+     * conj(v) (scale eps^-1 v) of the real-space values v, i.e. the block's
+     * share of <p, QBQ p> when every line transform is an unnormalised DFT;
+     * layout [problem][component][gridDim.x], blocks past the last tile write 0 */
+    double* pap;
     size_t fstride;
     int L, P;                    /* lines per block, LDS line pitch            */
     unsigned mL;                 /* udiv magic for / L                         */
@@ -102,7 +108,10 @@ hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, doubl
  * |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 - |r|^2 (conjugacy: <r,Ap> = <p,Ap>),
  * beta and the active flags, from the 6-double partials of the expanding and
  * contracting passes */
-hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
+/* pap (optional): <p,Ap> from the outer pass's block sums (3 npap per problem)
+ * instead of slots 0 and 1 of the inner partials */
+hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap, int nprob,
+                              int* active, hipStream_t s);
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
                             hipStream_t s);
 /* FP32 solves inside FP64 refinement: start from the FP64 right-hand side

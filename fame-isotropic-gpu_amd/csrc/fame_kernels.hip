@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
         double dre = 0.0, dim = 0.0, yy = 0.0;
-        const bool CGD = D && args.cgdots;
+        const bool CGD = args.cgdots != 0;
         struct DIn { T2 d1, d2; };
         batched_loop<InflightInner<T>::v>(
             L * na,
@@ -271,15 +271,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
                 Y[f] = o1;
                 Y[N + f] = o2;
-                if (D) {
-                    const double2 d1 = to_d(dd.d1), d2 = to_d(dd.d2);
+                if (D || CGD) {
                     const double2 q1 = to_d(o1), q2 = to_d(o2);
-                    dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
-                    dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                    if (D) {
+                        const double2 d1 = to_d(dd.d1), d2 = to_d(dd.d2);
+                        dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
+                        dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                    }
                     if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
                 }
             });
-        if (D) {
+        if (D || CGD) {
             /* block sums of conj(d) . y (and, inside CG, y . y) into
              * partials[p][tile]: wave shuffles, then the four wave sums in a
              * fixed order */
@@ -296,7 +298,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             __syncthreads();
             if (threadIdx.x == 0) {
                 double* part = args.partials + ((size_t)p * nblk + tile) * stride;
-                for (int c = 0; c < nc; ++c) {
+                for (int c = D ? 0 : 2; c < nc; ++c) {
                     double sum = 0;
                     for (int w = 0; w < FFT_THREADS / 64; ++w) sum += red[c][w];
                     part[c] = sum;
@@ -383,7 +385,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     const int tile = xcd_tile();
-    if (tile * args.L >= args.g.na * args.g.nb) return;
+    double* pap = (MODE == OUTER_EPS && args.pap) ? args.pap + ((size_t)p * 3 + comp) * gridDim.x + tile : nullptr;
+    if (tile * args.L >= args.g.na * args.g.nb) {
+        if (pap && threadIdx.x == 0) *pap = 0.0;
+        return;
+    }
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const GridDev& g = args.g;
     const ProbDev pr = args.prob[p];
@@ -415,6 +421,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
         const T scale = (T)(1.0 / (double)g.N / (pl.blue ? (double)pl.M : 1.0));
         const unsigned mL = args.mL;
+        double vw = 0.0;
         auto eps_step = [&](T2* Sx) {
             batched_loop<InflightEps<T>::v>(
                 L * Wd,
@@ -424,11 +431,25 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
                 },
                 [&](int idx, T e) {
                     const int c = udiv(idx, mL), tq = idx - c * L;
-                    Sx[tq * P + c] = cscale(Sx[tq * P + c], e * scale);
+                    const T2 v = Sx[tq * P + c], w = cscale(v, e * scale);
+                    Sx[tq * P + c] = w;
+                    if (pap) vw += (double)v.x * (double)w.x + (double)v.y * (double)w.y;
                 });
             __syncthreads();
         };
         run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
+        if (pap) {
+            /* block sum in a fixed order: wave shuffles, then the wave sums */
+            __shared__ double red[FFT_THREADS / 64];
+            for (int o = 32; o > 0; o >>= 1) vw += __shfl_xor(vw, o);
+            if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vw;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double sum = 0.0;
+                for (int w = 0; w < FFT_THREADS / 64; ++w) sum += red[w];
+                *pap = sum;
+            }
+        }
     } else {
         run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
     }

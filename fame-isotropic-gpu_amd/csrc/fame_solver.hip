@@ -378,6 +378,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     int nbmax = std::max(g.nb * g.nc, NBLK_VEC);
     A(c->partials, sizeof(double) * 4 * nbmax * B);
     A(c->partials2, sizeof(double) * 6 * nbmax * B);
+    A(c->pap_part, sizeof(double) * 3 * xcd_round(g.na * g.nb) * B);   /* the most outer blocks: one line each */
     A(c->cgsc, sizeof(CgScal) * B);
     A(c->active, sizeof(int) * B);
     A(c->mask, sizeof(int) * B);
@@ -527,6 +528,13 @@ KAxis fame_ctx::kout(int prec, int p0, const int* act, int list) const {
     return a;
 }
 
+int fame_ctx::pap_blocks(int prec) const {
+    for (int axis = 0; axis < 3; ++axis)
+        if (plan(axis, prec).lists[pix(prec)][LIST_FWD].blue) return 0;
+    const int L = tile[AX_OUTER][pix(prec)].L;
+    return xcd_round((g.na * g.nb + L - 1) / L);
+}
+
 /* y = QBQ x on nprob problems starting at p0 (padded reduced layout, 2N each);
  * with dotvec, partial sums of conj(dotvec).y go to partials2 per inner block. */
 fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, const int* act, const void* dotvec,
@@ -536,12 +544,17 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
     if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; ki.cg_ap = cg_ap; ki.partials = partials2; }
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
     ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr; ki.cg_ap = nullptr;
-    ki.cgdots = dotvec && cgdots;
+    /* inside CG, <p, QBQ p> comes from the outer pass where it can (no read
+     * of p by the last pass), else from the last pass's conj(p).y */
+    ki.cgdots = cgdots;
+    const bool pap = cgdots && pap_blocks(prec) > 0;
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
-    HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
+    KAxis ko = kout(prec, p0, act, LIST_EPS);
+    if (pap) ko.pap = pap_part;
+    HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
     ki.plan = plan(AX_INNER, prec).lists[pix(prec)][LIST_FWD];
-    ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
+    ki.y = y; ki.dotvec = pap ? nullptr : dotvec; ki.partials = partials2;
     HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
     if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
     return FAME_OK;
@@ -809,7 +822,9 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int cou
         for (int k = 0; k < count; ++k) {
             fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, true);
             if (q != FAME_OK) { stream = keep; return q; }
-            const hipError_t e = fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, s);
+            const int npap = pap_blocks(prec);
+            const hipError_t e = fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], npap ? pap_part : nullptr,
+                                                    npap, nprob, active, s);
             if (e != hipSuccess) { stream = keep; FAIL(FAME_ERR_HIP, std::string("cg_alpha_beta: ") + hipGetErrorString(e)); }
         }
         stream = keep;
