@@ -104,21 +104,15 @@ __global__ void k_cg_init_scal(CgScal* sc, const double* partials, int nblk, dou
 
 /* partials per inner block (6 doubles): <p,Ap> (slots 0, 1) and <Ap,Ap>
  * (slot 2) from the contracting pass, <r,r> of the current residual (slot 5)
- * from the expanding pass; slots 3 and 4 are not used.  With pap, <p,Ap> is
- * instead the sum of the outer pass's 3 npap block sums of the problem (real:
- * the quadratic form of the real-space field), and slots 0 and 1 are unused. */
-__global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap,
-                                int* active) {
+ * from the expanding pass; slots 3 and 4 are not used */
+__global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int* active) {
     const int p = blockIdx.x;
     if (!active[p]) return;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
         const double* q = partials + ((size_t)p * nblk + b) * 6;
-        if (!pap) { v[0] += q[0]; v[1] += q[1]; }
-        v[2] += q[2]; v[3] += q[5];
+        v[0] += q[0]; v[1] += q[1]; v[2] += q[2]; v[3] += q[5];
     }
-    if (pap)
-        for (int b = threadIdx.x; b < 3 * npap; b += blockDim.x) v[0] += pap[(size_t)p * 3 * npap + b];
     block_sum_all<4>(v);
     if (threadIdx.x != 0) return;
     CgScal& c = sc[p];
@@ -139,6 +133,40 @@ __global__ void k_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, co
      * use it, the next alpha the summed norm again */
     const double s = fmax(0.0, (ar * ar + ai * ai) * aa - rr);
     c.beta = s / rr;
+    c.rr = s;
+    c.iters += 1;
+    c.active = (s > c.tol2) && (c.iters <= c.maxit);
+    active[p] = c.active;
+}
+
+/* alpha = <r,r> / <p,Ap> with <p,Ap> summed from the outer pass's block sums
+ * (3 npap per problem); the solution update x += alpha p becomes pending */
+__global__ void k_cg_alpha(CgScal* sc, const double* pap, int npap, const int* active) {
+    const int p = blockIdx.x;
+    if (!active[p]) return;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < 3 * npap; b += blockDim.x) v[0] += pap[(size_t)p * 3 * npap + b];
+    block_sum_all<1>(v);
+    if (threadIdx.x != 0) return;
+    CgScal& c = sc[p];
+    /* a = r1 / dot  (CG.cu:60-63); the quadratic form is real */
+    c.pap_re = v[0]; c.pap_im = 0.0;
+    c.alpha_re = c.rr / v[0]; c.alpha_im = 0.0;
+    c.pend = 1;
+}
+
+/* <r,r> of the residual the contracting pass updated (slot 3 of its 6-double
+ * block partials): b = r1 / r0 and the stopping rule (CG.cu:66-78) */
+__global__ void k_cg_beta(CgScal* sc, const double* partials, int nblk, int* active) {
+    const int p = blockIdx.x;
+    if (!active[p]) return;
+    double v[1] = {0.0};
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) v[0] += partials[((size_t)p * nblk + b) * 6 + 3];
+    block_sum_all<1>(v);
+    if (threadIdx.x != 0) return;
+    CgScal& c = sc[p];
+    const double s = v[0];
+    c.beta = s / c.rr;
     c.rr = s;
     c.iters += 1;
     c.active = (s > c.tol2) && (c.iters <= c.maxit);
@@ -486,9 +514,16 @@ hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, doubl
     hipLaunchKernelGGL(k_cg_init_scal, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, tol, maxit, active, (const int*)nullptr);
     return hipGetLastError();
 }
-hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap, int nprob,
-                              int* active, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_alpha_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, pap, npap, active);
+hipError_t fame_cg_alpha(CgScal* sc, const double* pap, int npap, int nprob, const int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha, dim3(nprob), dim3(RB), 0, s, sc, pap, npap, active);
+    return hipGetLastError();
+}
+hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
+    return hipGetLastError();
+}
+hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s) {
+    hipLaunchKernelGGL(k_cg_alpha_beta, dim3(nprob), dim3(RB), 0, s, sc, partials, nblk, active);
     return hipGetLastError();
 }
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,

@@ -81,6 +81,8 @@ struct fame_ctx {
     std::map<std::tuple<int, int, int, const void*, int>, hipGraphExec_t> cg_graphs;
     void drop_graphs();
     fame_status cg_iteration(int prec, void* x, int p0, int nprob, int count = 1);
+    fame_status cg_step_ap(int prec, void* p, void* r, void* x, void* Ap, int p0, int nprob);
+    fame_status cg_step_rc(int prec, void* p, void* r, void* x, int p0, int nprob, int npap);
 
     KInner kin(int prec, int p0, const int* act, int list) const;
     KAxis kmid(int prec, int p0, const int* act, int list) const;

@@ -53,7 +53,11 @@ struct KInner {
     void* cg_x;
     const CgScal* cgsc;
     /* CG residual update on the same pass: with cg_ap set, r -= alpha Ap of
-     * the previous iteration (when pend) is applied before p = r + beta p    */
+     * the previous iteration (when pend) is applied before p = r + beta p,
+     * and <r,r> goes to slot 5 of the partials.
+     * !EXPAND with cg_r set: the pass updates r -= alpha y in place instead of
+     * storing y (alpha of this iteration, already in cgsc), and puts <r,r> of
+     * the updated residual in slot 3 of 6 doubles per block                 */
     const void* cg_ap;
     /* !EXPAND inside CG: <y, y> into slot 2 of 6 doubles per block (the
      * expanding pass puts <r, r> in slot 5; with dotvec, conj(dotvec).y goes
@@ -104,14 +108,18 @@ hipError_t fame_cg_begin(void* x, void* r, void* p, const void* b, double* parti
                          size_t n, size_t stride, int nprob, int prec, const int* mask, hipStream_t s);
 hipError_t fame_cg_init_scal(CgScal* sc, const double* partials, int nblk, double tol, int maxit, int nprob,
                              int* active, hipStream_t s);
-/* alpha from the summed <r,r>, <r,r> of the next residual by one step of
- * |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 - |r|^2 (conjugacy: <r,Ap> = <p,Ap>),
- * beta and the active flags, from the 6-double partials of the expanding and
- * contracting passes */
-/* pap (optional): <p,Ap> from the outer pass's block sums (3 npap per problem)
- * instead of slots 0 and 1 of the inner partials */
-hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, const double* pap, int npap, int nprob,
-                              int* active, hipStream_t s);
+/* CG iteration with Ap stored (any plan): alpha from the summed <r,r>, <r,r>
+ * of the next residual by one step of |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 -
+ * |r|^2 (conjugacy: <r,Ap> = <p,Ap>), beta and the active flags, from the
+ * 6-double partials of the expanding and contracting passes */
+/* CG iteration with the residual updated by the contracting pass (when the
+ * outer pass sums <p,Ap>): alpha = <r,r> / <p,Ap> from the outer pass's 3 npap
+ * block sums per problem, launched before the contracting pass; then beta and
+ * the active flags from the exact <r,r> of the updated residual (slot 3 of the
+ * inner partials) */
+hipError_t fame_cg_alpha(CgScal* sc, const double* pap, int npap, int nprob, const int* active, hipStream_t s);
+hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
+hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
                             hipStream_t s);
 /* FP32 solves inside FP64 refinement: start from the FP64 right-hand side

@@ -174,16 +174,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
             });
-        /* inside CG: block sum of <r,r> into slot 5 of partials[p][tile] (the
-         * contracting pass fills slots 0..2) */
+        /* inside CG with the stored Ap: block sum of <r,r> into slot 5 of
+         * partials[p][tile] (the contracting pass fills slots 0..2); with the
+         * residual updated by the contracting pass, that pass sums <r,r> */
         __shared__ double rred[FFT_THREADS / 64];
-        if (cgf) {
+        const bool rsum = cgf && args.cg_ap != nullptr;
+        if (rsum) {
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) rnorm += __shfl_xor(rnorm, o);
             if ((threadIdx.x & 63) == 0) rred[threadIdx.x >> 6] = rnorm;
         }
         __syncthreads();
-        if (cgf && threadIdx.x == 0) {
+        if (rsum && threadIdx.x == 0) {
             double sum = 0;
             for (int w = 0; w < FFT_THREADS / 64; ++w) sum += rred[w];
             args.partials[((size_t)p * nblk + tile) * 6 + 5] = sum;
@@ -231,18 +233,25 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
-        double dre = 0.0, dim = 0.0, yy = 0.0;
+        double dre = 0.0, dim = 0.0, yy = 0.0, rn = 0.0;
         const bool CGD = args.cgdots != 0;
-        struct DIn { T2 d1, d2; };
+        /* inside CG with cg_r set: r -= alpha y (alpha of the problem's CgScal)
+         * is stored in place of y, with <r,r> of the updated residual */
+        const bool RU = args.cg_r != nullptr;
+        T2* const RW = RU ? reinterpret_cast<T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
+        T ar = 0, ai = 0;
+        if (RU) { ar = (T)args.cgsc[p].alpha_re; ai = (T)args.cgsc[p].alpha_im; }
+        struct DIn { T2 d1, d2, r1, r2; };
         batched_loop<InflightInner<T>::v>(
             L * na,
             [&](int idx) -> DIn {
                 const int line = udiv(idx, g.m_na), a = idx - line * na;
                 const int gl = l0 + line;
                 DIn d;
-                if (D && gl < nlines) {
+                if (gl < nlines) {
                     const unsigned f = (unsigned)(gl * na + a);
-                    d.d1 = D[f]; d.d2 = D[N + f];
+                    if (D) { d.d1 = D[f]; d.d2 = D[N + f]; }
+                    if (RU) { d.r1 = RW[f]; d.r2 = RW[N + f]; }
                 }
                 return d;
             },
@@ -269,8 +278,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     }
                 }
                 const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
-                Y[f] = o1;
-                Y[N + f] = o2;
+                if (RU) {
+                    T2 r1 = dd.r1, r2 = dd.r2;
+                    r1.x -= ar * o1.x - ai * o1.y; r1.y -= ar * o1.y + ai * o1.x;
+                    r2.x -= ar * o2.x - ai * o2.y; r2.y -= ar * o2.y + ai * o2.x;
+                    RW[f] = r1; RW[N + f] = r2;
+                    rn += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
+                } else {
+                    Y[f] = o1;
+                    Y[N + f] = o2;
+                }
                 if (D || CGD) {
                     const double2 q1 = to_d(o1), q2 = to_d(o2);
                     if (D) {
@@ -281,24 +298,25 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
                 }
             });
-        if (D || CGD) {
-            /* block sums of conj(d) . y (and, inside CG, y . y) into
-             * partials[p][tile]: wave shuffles, then the four wave sums in a
-             * fixed order */
-            __shared__ double red[3][FFT_THREADS / 64];
-            double v[3] = {dre, dim, yy};
-            const int nc = CGD ? 3 : 2, stride = CGD ? 6 : 2;
+        if (D || CGD || RU) {
+            /* block sums into partials[p][tile]: conj(d) . y (slots 0, 1), and
+             * inside CG y . y (slot 2) or <r,r> of the updated residual (slot
+             * 3); wave shuffles, then the four wave sums in a fixed order */
+            __shared__ double red[4][FFT_THREADS / 64];
+            double v[4] = {dre, dim, yy, rn};
+            const int stride = (CGD || RU) ? 6 : 2;
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
+            for (int c = 0; c < 4; ++c)
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
             if ((threadIdx.x & 63) == 0)
 #pragma unroll
-                for (int c = 0; c < 3; ++c) red[c][threadIdx.x >> 6] = v[c];
+                for (int c = 0; c < 4; ++c) red[c][threadIdx.x >> 6] = v[c];
             __syncthreads();
             if (threadIdx.x == 0) {
                 double* part = args.partials + ((size_t)p * nblk + tile) * stride;
-                for (int c = D ? 0 : 2; c < nc; ++c) {
+                for (int c = 0; c < 4; ++c) {
+                    if (!((c < 2 && D) || (c == 2 && CGD) || (c == 3 && RU))) continue;
                     double sum = 0;
                     for (int w = 0; w < FFT_THREADS / 64; ++w) sum += red[c][w];
                     part[c] = sum;

@@ -544,17 +544,12 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
     if (cg_r) { ki.cg_r = cg_r; ki.cg_x = cg_x; ki.cgsc = cgsc; ki.cg_ap = cg_ap; ki.partials = partials2; }
     HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
     ki.cg_r = nullptr; ki.cg_x = nullptr; ki.cgsc = nullptr; ki.cg_ap = nullptr;
-    /* inside CG, <p, QBQ p> comes from the outer pass where it can (no read
-     * of p by the last pass), else from the last pass's conj(p).y */
     ki.cgdots = cgdots;
-    const bool pap = cgdots && pap_blocks(prec) > 0;
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_INV), +1, prec, nprob, stream));
-    KAxis ko = kout(prec, p0, act, LIST_EPS);
-    if (pap) ko.pap = pap_part;
-    HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nprob, stream));
+    HIPCHK(fame_launch_outer(kout(prec, p0, act, LIST_EPS), OUTER_EPS, prec, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(prec, p0, act, LIST_FWD), -1, prec, nprob, stream));
     ki.plan = plan(AX_INNER, prec).lists[pix(prec)][LIST_FWD];
-    ki.y = y; ki.dotvec = pap ? nullptr : dotvec; ki.partials = partials2;
+    ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
     HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
     if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
     return FAME_OK;
@@ -797,18 +792,47 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     return FAME_OK;
 }
 
-/* one CG iteration (CG.cu:53-78) on the masked batch:
- *   first QBQ pass: x += alpha p and r -= alpha Ap (the previous iteration's,
- *                   when pending), then p = r + beta p;
- *   QBQ p -> Ap, the last pass with the fused <p,Ap> and <Ap,Ap> partials;
- *   alpha from <r,r> summed by the first pass; the next <r,r> by one step of
- *   |r - alpha Ap|^2 = |alpha|^2 |Ap|^2 - |r|^2 (conjugacy: <r,Ap> = <p,Ap>)
- *   for beta and the stopping rule (no pass over r); the active flags.
- * The reference's alpha, beta and stopping rule are kept; the x, r and p updates
+/* one CG iteration with Ap stored: QBQ p -> Ap whose first pass applies the
+ * previous iteration's x, r and p updates and whose last pass sums <p,Ap> and
+ * <Ap,Ap>, then alpha, beta and the flags (six kernels) */
+fame_status fame_ctx::cg_step_ap(int prec, void* p, void* r, void* x, void* Ap, int p0, int nprob) {
+    CHK(qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, true));
+    HIPCHK(fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, stream));
+    return FAME_OK;
+}
+
+/* one CG iteration with the residual updated by the last pass (seven kernels):
+ * the first pass applies x += alpha p of the previous iteration and forms
+ * p = r + beta p; the outer pass sums <p,Ap> from the real-space field; alpha;
+ * the last pass stores r - alpha Ap (Ap itself is never written) and sums its
+ * <r,r>; beta and the flags from that exact norm */
+fame_status fame_ctx::cg_step_rc(int prec, void* p, void* r, void* x, int p0, int nprob, int npap) {
+    KInner ki = kin(prec, p0, active, LIST_INV);
+    ki.x = p; ki.pikind = 0;
+    ki.cg_r = r; ki.cg_x = x; ki.cgsc = cgsc; ki.cg_ap = nullptr; ki.partials = partials2;
+    HIPCHK(fame_launch_inner(ki, 1, prec, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(prec, p0, active, LIST_INV), +1, prec, nprob, stream));
+    KAxis ko = kout(prec, p0, active, LIST_EPS);
+    ko.pap = pap_part;
+    HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nprob, stream));
+    HIPCHK(fame_cg_alpha(cgsc, pap_part, npap, nprob, active, stream));
+    HIPCHK(fame_launch_mid(kmid(prec, p0, active, LIST_FWD), -1, prec, nprob, stream));
+    KInner kc = kin(prec, p0, active, LIST_FWD);
+    kc.x = p; kc.y = r; kc.pikind = 0;
+    kc.cg_r = r; kc.cgsc = cgsc; kc.partials = partials2;
+    HIPCHK(fame_launch_inner(kc, 0, prec, nprob, stream));
+    HIPCHK(fame_cg_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, stream));
+    if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
+    return FAME_OK;
+}
+
+/* `count` CG iterations (CG.cu:53-78) on the masked batch, as cg_step_rc
+ * when the outer pass can sum <p,Ap> (no Bluestein plan), else cg_step_ap.
+ * The reference's alpha, beta and stopping rule are kept; the x and p updates
  * of iteration k are applied while iteration k+1 reads p anyway (k_cg_begin sets
  * p = r and beta = 0, so the first pass leaves p = r).  The last x update is
  * applied by k_cg_finish_x after the loop.  All scalars stay on the device, so
- * the six launches are captured once as a graph.                            */
+ * the launches are captured once per count as a graph.                      */
 fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int count) {
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
@@ -819,13 +843,10 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int cou
     auto launch = [&](hipStream_t s) -> fame_status {
         const hipStream_t keep = stream;
         stream = s;
+        const int npap = pap_blocks(prec);
         for (int k = 0; k < count; ++k) {
-            fame_status q = qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, true);
+            fame_status q = npap ? cg_step_rc(prec, p, r, x, p0, nprob, npap) : cg_step_ap(prec, p, r, x, Ap, p0, nprob);
             if (q != FAME_OK) { stream = keep; return q; }
-            const int npap = pap_blocks(prec);
-            const hipError_t e = fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], npap ? pap_part : nullptr,
-                                                    npap, nprob, active, s);
-            if (e != hipSuccess) { stream = keep; FAIL(FAME_ERR_HIP, std::string("cg_alpha_beta: ") + hipGetErrorString(e)); }
         }
         stream = keep;
         return FAME_OK;
