@@ -144,6 +144,46 @@ def test_operators_default_engine_medium(gpu, sname, nx):
     S.close()
 
 
+@pytest.mark.parametrize("fft", ["stockham", "auto"])
+def test_cg_and_eigs_prime_axis(gpu, fft):
+    """FCC 20 -> 20x17x16.  With the Stockham engine the prime 17 takes a
+    Bluestein plan, so the CG iteration keeps Ap and reads p in its last pass;
+    with the default engine (17 on the matrix cores) the last pass updates r
+    and the outer pass sums <p,Ap>.  Both against the oracle."""
+    import torch
+    lt, _, kw, mat, ein = __import__("cases").STRUCTS["fcc"]
+    lo = F.build_lattice(lt, 20, **kw)
+    fl = F.to_fame_lattice(lo)
+    assert list(fl.grid_nums) == [20, 17, 16]
+    eps = F.material_eps(lo, mat, [ein], 1.0)
+    ol = O.lattice_from(fl)
+    k = KPTS["k"]
+    old = os.environ.get("FAME_FFT")
+    os.environ["FAME_FFT"] = fft
+    try:
+        S = F.Solver(fl, max_batch=1)
+    finally:
+        if old is None:
+            del os.environ["FAME_FFT"]
+        else:
+            os.environ["FAME_FFT"] = old
+    S.set_eps(eps)
+    S.set_wave_vectors(k)
+    t = O.Tables(ol, k)
+    b = _rand(2 * t.Nd, 7) * 1e-2
+    xg, itg = S.cg(0, torch.tensor(b, device=gpu), maxit=1000, tol=1e-12)
+    xo, ito, _ = t.cg(1.0 / eps, b, 1000, 1e-12)
+    assert abs(itg - ito) <= 1
+    assert _rel(xg.cpu().numpy(), xo) < 1e-9
+    fo, _, _ = O.eigs(ol, eps, k, 6, 20)
+    lam_o = fo ** 2
+    for prec in (F.PREC_F64, F.PREC_F32_REFINE):
+        freq, _, _, prof = S.eigs(6, 20, precision=prec)
+        assert prof[0]["converged"] == 1
+        assert np.max(np.abs(freq[0] ** 2 - lam_o) / lam_o) < 1e-8, (fft, prec, freq[0] ** 2, lam_o)
+    S.close()
+
+
 def test_graph_replay_matches_eager(gpu):
     """The CG iteration replayed from a captured hipGraph gives bit-identical
     results to eager launches: several solves in a row (each a fresh replay
