@@ -257,44 +257,6 @@ __global__ void k_convert(typename V2<TD>::t* d, const typename V2<TS>::t* s, si
     }
 }
 
-template <typename TS>
-__global__ void k_axpy_mixed(double2* y, const typename V2<TS>::t* x, double alpha, size_t n, size_t ystride,
-                             size_t xstride, const int* mask) {
-    const int p = blockIdx.y;
-    if (mask && !mask[p]) return;
-    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-        double2 v = d2(x[(size_t)p * xstride + i]);
-        double2 w = y[(size_t)p * ystride + i];
-        w.x += alpha * v.x; w.y += alpha * v.y;
-        y[(size_t)p * ystride + i] = w;
-    }
-}
-
-__global__ void k_sub(double2* r, const double2* b, const double2* ax, size_t n, size_t stride, const int* mask) {
-    const int p = blockIdx.y;
-    if (mask && !mask[p]) return;
-    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-        double2 u = b[(size_t)p * stride + i], v = ax[(size_t)p * stride + i];
-        r[(size_t)p * stride + i] = make_double2(u.x - v.x, u.y - v.y);
-    }
-}
-
-/* r = b - ax and the partials of <r,r> (problems outside the mask write zero partials) */
-__global__ void __launch_bounds__(RB) k_sub_norm2(double2* r, const double2* b, const double2* ax, double* partials,
-                                                  size_t n, size_t stride, const int* mask) {
-    const int p = blockIdx.y;
-    double acc[1] = {0.0};
-    if (!mask || mask[p]) {
-        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-            const double2 u = b[(size_t)p * stride + i], v = ax[(size_t)p * stride + i];
-            const double2 d = make_double2(u.x - v.x, u.y - v.y);
-            r[(size_t)p * stride + i] = d;
-            acc[0] += d.x * d.x + d.y * d.y;
-        }
-    }
-    block_partials<1>(acc, partials, p);
-}
-
 template <typename T>
 __global__ void __launch_bounds__(RB) k_norm2(const typename V2<T>::t* x, double* partials, size_t n, size_t stride) {
     const int p = blockIdx.y;
@@ -581,27 +543,6 @@ hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t
         hipLaunchKernelGGL((k_convert<double, double>), g, dim3(RB), 0, s, (double2*)dst, (const double2*)src, n, dstride, sstride, mask);
     else
         hipLaunchKernelGGL((k_convert<float, float>), g, dim3(RB), 0, s, (float2*)dst, (const float2*)src, n, dstride, sstride, mask);
-    return hipGetLastError();
-}
-hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
-                           int nprob, const int* mask, hipStream_t s) {
-    dim3 g(1024, nprob);
-    if (xprec == 8)
-        hipLaunchKernelGGL(k_axpy_mixed<double>, g, dim3(RB), 0, s, (double2*)y64, (const double2*)x, alpha, n, ystride, xstride, mask);
-    else
-        hipLaunchKernelGGL(k_axpy_mixed<float>, g, dim3(RB), 0, s, (double2*)y64, (const float2*)x, alpha, n, ystride, xstride, mask);
-    return hipGetLastError();
-}
-hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
-                    hipStream_t s) {
-    dim3 g(1024, nprob);
-    hipLaunchKernelGGL(k_sub, g, dim3(RB), 0, s, (double2*)r, (const double2*)b, (const double2*)ax, n, stride, mask);
-    return hipGetLastError();
-}
-hipError_t fame_sub_norm2(void* r, const void* b, const void* ax, double* partials, int nblk, size_t n, size_t stride,
-                          int nprob, const int* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_sub_norm2, dim3(nblk, nprob), dim3(RB), 0, s, (double2*)r, (const double2*)b, (const double2*)ax,
-                       partials, n, stride, mask);
     return hipGetLastError();
 }
 hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,

@@ -55,7 +55,7 @@ struct fame_ctx {
     double2* F64 = nullptr; float2* F32 = nullptr;        /* fields, 3N per problem        */
     void* cg64[4] = {nullptr, nullptr, nullptr, nullptr};   /* r, p, Ap, spare (2N/problem) */
     void* cg32[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  /* r, p, Ap, rhs32, sol32 */
-    void* ref64[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    void* ref64[3] = {nullptr, nullptr, nullptr};
     double* partials = nullptr;   /* [problem][NBLK_VEC][<=4]   */
     double* partials2 = nullptr;  /* [problem][inner blocks][2, or 6 inside CG] */
     double* pap_part = nullptr;   /* [problem][3][outer blocks]: <p,QBQ p> block sums of the outer pass */
@@ -96,9 +96,15 @@ struct fame_ctx {
     fame_status field_from_ref(const void* fref, void* fint, int nprob);
     /* b64: b is complex double while prec is 4 (rounded to FP32 at the start);
      * x64_accum: with prec 4, the FP32 solution is added to this complex
-     * double vector (problems of mask_dev) instead of being finished in x */
+     * double vector (problems of mask_dev) instead of being finished in x;
+     * begun_nblk > 0: x, r, p were set up by qbq_residual32 (or the caller),
+     * with the <r,r> partials in partials, begun_nblk per problem */
     fame_status cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                   const int* mask_dev, int* iters, bool b64 = false, void* x64_accum = nullptr);
+                   const int* mask_dev, int* iters, bool b64 = false, void* x64_accum = nullptr, int begun_nblk = 0);
+    /* refinement residual: QBQ64 xr, whose last pass stores the next FP32
+     * solve's start r = p = FP32(b - QBQ64 xr), x = 0, and its norm partials
+     * (partials[problem][inner block]) */
+    fame_status qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act);
     /* x: problem stride xstride (0: 2N); out (optional): the result is written
      * there, problem stride ostride, for the problems of mask_dev only, and y
      * serves as the contiguous solve buffer; mask_host: a host copy of

@@ -59,6 +59,15 @@ struct KInner {
      * storing y (alpha of this iteration, already in cgsc), and puts <r,r> of
      * the updated residual in slot 3 of 6 doubles per block                 */
     const void* cg_ap;
+    /* !EXPAND, refinement residual (rb set, complex double like y): instead
+     * of y the pass stores the next FP32 solve's start, r32 = p32 = rb - y
+     * rounded to FP32 and x32 = 0 (FP32 complex vectors, problem stride
+     * xstride), and the block sums of |r32|^2 into rpart[problem][block]   */
+    const void* rb;
+    void* r32;
+    void* p32;
+    void* x32;
+    double* rpart;
     /* !EXPAND inside CG: <y, y> into slot 2 of 6 doubles per block (the
      * expanding pass puts <r, r> in slot 5; with dotvec, conj(dotvec).y goes
      * to slots 0 and 1), from which the CG takes <r, r> of the next residual
@@ -136,13 +145,6 @@ hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n
 /* mask (device, optional): problems with mask 0 are left untouched */
 hipError_t fame_convert(void* dst, int dprec, const void* src, int sprec, size_t n, size_t dstride, size_t sstride,
                         int nprob, hipStream_t s, const int* mask = nullptr);
-hipError_t fame_axpy_mixed(void* y64, const void* x, int xprec, double alpha, size_t n, size_t ystride, size_t xstride,
-                           int nprob, const int* mask, hipStream_t s);
-hipError_t fame_sub(void* r, const void* b, const void* ax, size_t n, size_t stride, int nprob, const int* mask,
-                    hipStream_t s);
-/* r = b - ax with the partials of <r,r> (nblk per problem; zero outside the mask) */
-hipError_t fame_sub_norm2(void* r, const void* b, const void* ax, double* partials, int nblk, size_t n, size_t stride,
-                          int nprob, const int* mask, hipStream_t s);
 hipError_t fame_norm2_partials(const void* x, int prec, double* partials, int nblk, size_t n, size_t stride, int nprob,
                                hipStream_t s);
 hipError_t fame_reduce_partials(const double* partials, int nblk, int ncomp, double* out, int nprob, hipStream_t s);

@@ -241,6 +241,12 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         T2* const RW = RU ? reinterpret_cast<T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
         T ar = 0, ai = 0;
         if (RU) { ar = (T)args.cgsc[p].alpha_re; ai = (T)args.cgsc[p].alpha_im; }
+        /* refinement residual: rb - y becomes the FP32 start r32 = p32, x32 = 0 */
+        const bool RB = args.rb != nullptr;
+        const T2* const RBv = RB ? reinterpret_cast<const T2*>(args.rb) + (size_t)p * args.xstride : nullptr;
+        float2* const R32 = RB ? reinterpret_cast<float2*>(args.r32) + (size_t)p * args.xstride : nullptr;
+        float2* const P32 = RB ? reinterpret_cast<float2*>(args.p32) + (size_t)p * args.xstride : nullptr;
+        float2* const X32 = RB ? reinterpret_cast<float2*>(args.x32) + (size_t)p * args.xstride : nullptr;
         struct DIn { T2 d1, d2, r1, r2; };
         batched_loop<InflightInner<T>::v>(
             L * na,
@@ -252,6 +258,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     const unsigned f = (unsigned)(gl * na + a);
                     if (D) { d.d1 = D[f]; d.d2 = D[N + f]; }
                     if (RU) { d.r1 = RW[f]; d.r2 = RW[N + f]; }
+                    if (RB) { d.r1 = RBv[f]; d.r2 = RBv[N + f]; }
                 }
                 return d;
             },
@@ -284,6 +291,15 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     r2.x -= ar * o2.x - ai * o2.y; r2.y -= ar * o2.y + ai * o2.x;
                     RW[f] = r1; RW[N + f] = r2;
                     rn += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
+                } else if (RB) {
+                    const double2 b1 = to_d(dd.r1), b2 = to_d(dd.r2), q1 = to_d(o1), q2 = to_d(o2);
+                    const float2 e1 = make_float2((float)(b1.x - q1.x), (float)(b1.y - q1.y));
+                    const float2 e2 = make_float2((float)(b2.x - q2.x), (float)(b2.y - q2.y));
+                    R32[f] = e1; R32[N + f] = e2;
+                    P32[f] = e1; P32[N + f] = e2;
+                    X32[f] = make_float2(0.0f, 0.0f); X32[N + f] = make_float2(0.0f, 0.0f);
+                    const float s1 = e1.x * e1.x + e1.y * e1.y, s2 = e2.x * e2.x + e2.y * e2.y;
+                    rn += (double)s1 + (double)s2;
                 } else {
                     Y[f] = o1;
                     Y[N + f] = o2;
@@ -298,7 +314,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
                 }
             });
-        if (D || CGD || RU) {
+        if (D || CGD || RU || RB) {
             /* block sums into partials[p][tile]: conj(d) . y (slots 0, 1), and
              * inside CG y . y (slot 2) or <r,r> of the updated residual (slot
              * 3); wave shuffles, then the four wave sums in a fixed order */
@@ -316,10 +332,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             if (threadIdx.x == 0) {
                 double* part = args.partials + ((size_t)p * nblk + tile) * stride;
                 for (int c = 0; c < 4; ++c) {
-                    if (!((c < 2 && D) || (c == 2 && CGD) || (c == 3 && RU))) continue;
+                    if (!((c < 2 && D) || (c == 2 && CGD) || (c == 3 && (RU || RB)))) continue;
                     double sum = 0;
                     for (int w = 0; w < FFT_THREADS / 64; ++w) sum += red[c][w];
-                    part[c] = sum;
+                    if (c == 3 && RB) args.rpart[(size_t)p * nblk + tile] = sum;
+                    else part[c] = sum;
                 }
             }
         }

@@ -373,7 +373,7 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     A(c->F32, 8 * 3 * N * B);
     for (int i = 0; i < 4; ++i) A(c->cg64[i], 16 * 2 * N * B);
     for (int i = 0; i < 5; ++i) A(c->cg32[i], 8 * 2 * N * B);
-    for (int i = 0; i < 5; ++i) A(c->ref64[i], 16 * 2 * N * B);
+    for (int i = 0; i < 3; ++i) A(c->ref64[i], 16 * 2 * N * B);
     /* inner-kernel partials: room for one line per block (any tuned tile) */
     int nbmax = std::max(g.nb * g.nc, NBLK_VEC);
     A(c->partials, sizeof(double) * 4 * nbmax * B);
@@ -552,6 +552,24 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
     ki.y = y; ki.dotvec = dotvec; ki.partials = partials2;
     HIPCHK(fame_launch_inner(ki, 0, prec, nprob, stream));
     if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
+    return FAME_OK;
+}
+
+fame_status fame_ctx::qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act) {
+    KInner ki = kin(8, p0, act, LIST_INV);
+    ki.x = xr; ki.pikind = 0;
+    HIPCHK(fame_launch_inner(ki, 1, 8, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_INV), +1, 8, nprob, stream));
+    HIPCHK(fame_launch_outer(kout(8, p0, act, LIST_EPS), OUTER_EPS, 8, nprob, stream));
+    HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_FWD), -1, 8, nprob, stream));
+    KInner kc = kin(8, p0, act, LIST_FWD);
+    const size_t es = 8 * 2 * (size_t)N;   /* FP32 complex vector bytes per problem */
+    kc.x = xr; kc.y = const_cast<void*>(b); kc.pikind = 0;
+    kc.rb = b; kc.x32 = x32;
+    kc.r32 = (char*)cg32[0] + es * p0; kc.p32 = (char*)cg32[1] + es * p0;
+    kc.rpart = partials;
+    HIPCHK(fame_launch_inner(kc, 0, 8, nprob, stream));
+    cnt_qbq64 += nprob;
     return FAME_OK;
 }
 
@@ -756,16 +774,18 @@ extern "C" fame_status fame_autotune(fame_ctx* c, int prec, int* choice) {
  * Stops per problem when <r,r> <= max(tol^2, reltol^2 <b,b>) or after maxit
  * (reference counter semantics, CG.cu:36-78).  iters[] (host) receives k.    */
 fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                         const int* mask_dev, int* iters, bool b64, void* x64_accum) {
+                         const int* mask_dev, int* iters, bool b64, void* x64_accum, int begun_nblk) {
     if ((b64 || x64_accum) && prec != 4) FAIL(FAME_ERR_ARG, "cg: FP64 right-hand side or accumulation needs prec 4");
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     const size_t es = prec == 8 ? 16 : 8;
     char* r = (char*)V[0] + es * st * p0;
     char* p = (char*)V[1] + es * st * p0;
-    if (b64) HIPCHK(fame_cg_begin_mixed(x, r, p, b, partials, NBLK_VEC, n, st, nprob, mask_dev, stream));
+    int nbp = NBLK_VEC;
+    if (begun_nblk > 0) nbp = begun_nblk;
+    else if (b64) HIPCHK(fame_cg_begin_mixed(x, r, p, b, partials, NBLK_VEC, n, st, nprob, mask_dev, stream));
     else HIPCHK(fame_cg_begin(x, r, p, b, partials, NBLK_VEC, n, st, nprob, prec, mask_dev, stream));
-    HIPCHK(fame_cg_init_scal2(cgsc, partials, NBLK_VEC, tol, reltol, maxit, nprob, active, mask_dev, stream));
+    HIPCHK(fame_cg_init_scal2(cgsc, partials, nbp, tol, reltol, maxit, nprob, active, mask_dev, stream));
     int done_it = 0;
     int chunk = std::max(1, std::min(maxit, cg_predict[pix(prec)] - 1));
     while (true) {
@@ -904,15 +924,17 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
     if (mode == FAME_PREC_F64) {
         CHK(cg(8, b, y, p0, nprob, ls.tol, 0.0, ls.maxit, mask_dev, it.data()));
     } else {
-        /* FP64 iterative refinement around FP32 CG solves: each sweep starts
-         * the FP32 solve from the FP64 residual, adds its solution into the
-         * FP64 iterate, and forms the next residual with its norm in one pass */
+        /* FP64 iterative refinement around FP32 CG solves: each sweep adds
+         * the FP32 solution into the FP64 iterate, and the FP64 residual pass
+         * writes the next FP32 solve's start (r = p = FP32(b - QBQ64 x), x = 0)
+         * with its norm, so no FP64 residual vector is kept */
         double2* xr = (double2*)y;
-        double2* rr = (double2*)ref64[3] + st * p0;
         float2* d32 = (float2*)cg32[4] + st * p0;
         HIPCHK(hipMemsetAsync(xr, 0, 16 * st * nprob, stream));
-        HIPCHK(hipMemcpyAsync(rr, b, 16 * st * nprob, hipMemcpyDeviceToDevice, stream));
-        HIPCHK(fame_norm2_partials(rr, 8, partials, NBLK_VEC, n, st, nprob, stream));
+        /* the first FP32 solve starts from b (x = 0, r = p = FP32(b)) */
+        HIPCHK(fame_cg_begin_mixed(d32, (char*)cg32[0] + 8 * st * p0, (char*)cg32[1] + 8 * st * p0, b, partials,
+                                   NBLK_VEC, n, st, nprob, nullptr, stream));
+        int nbp = NBLK_VEC;
         std::vector<int> msk(nprob, 1);
         if (mask_host) {
             std::copy(mask_host, mask_host + nprob, msk.begin());
@@ -922,7 +944,7 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
         }
         double tol2 = ls.tol * ls.tol;
         for (int sweep = 0; sweep < refine_max_sweeps; ++sweep) {
-            HIPCHK(fame_reduce_partials(partials, NBLK_VEC, 1, scal, nprob, stream));
+            HIPCHK(fame_reduce_partials(partials, nbp, 1, scal, nprob, stream));
             std::vector<double> rn(nprob);
             HIPCHK(hipMemcpyAsync(rn.data(), scal, sizeof(double) * nprob, hipMemcpyDeviceToHost, stream));
             HIPCHK(hipStreamSynchronize(stream));
@@ -931,12 +953,11 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
             if (!any) break;
             HIPCHK(hipMemcpyAsync(mask, msk.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, stream));
             std::vector<int> it32(nprob, 0);
-            CHK(cg(4, rr, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, xr));
+            CHK(cg(4, b, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, xr, nbp));
             for (int i = 0; i < nprob; ++i) if (msk[i]) it[i] += it32[i] - 1;
-            /* rr = b - QBQ64 xr, and its norm for the next sweep */
-            double2* t = (double2*)ref64[4] + st * p0;
-            CHK(qbq(8, xr, t, p0, nprob, mask, nullptr));
-            HIPCHK(fame_sub_norm2(rr, b, t, partials, NBLK_VEC, n, st, nprob, mask, stream));
+            /* the next sweep's start from r = b - QBQ64 xr, with its norm */
+            CHK(qbq_residual32(xr, b, d32, p0, nprob, mask));
+            nbp = nblk_in[pix(8)];
         }
         for (int i = 0; i < nprob; ++i) it[i] += 1;
     }
