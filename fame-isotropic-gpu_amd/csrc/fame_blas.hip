@@ -208,28 +208,6 @@ __global__ void __launch_bounds__(RB) k_cg_begin_from64(float2* x, float2* r, fl
     block_partials<1>(acc, partials, p);
 }
 
-/* end of an FP32 solve inside the refinement: x64 += x + alpha p (the update
- * still pending) for the problems of the mask; the FP32 solution itself is
- * not written back */
-__global__ void __launch_bounds__(RB) k_cg_finish_mixed(double2* x64, const float2* x, const float2* pv,
-                                                        const CgScal* sc, size_t n, size_t stride, const int* mask) {
-    const int p = blockIdx.y;
-    if (mask && !mask[p]) return;
-    const bool pend = sc[p].pend;
-    const float ar = (float)sc[p].alpha_re, ai = (float)sc[p].alpha_im;
-    const size_t o = (size_t)p * stride;
-    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-        float2 xx = x[o + i];
-        if (pend) {
-            const float2 pp = pv[o + i];
-            xx.x += ar * pp.x - ai * pp.y; xx.y += ar * pp.y + ai * pp.x;
-        }
-        double2 w = x64[o + i];
-        w.x += (double)xx.x; w.y += (double)xx.y;
-        x64[o + i] = w;
-    }
-}
-
 /* ------------------------------------------------------------ helpers ---- */
 template <typename T>
 __global__ void k_scale_real(typename V2<T>::t* y, size_t ystride, const typename V2<T>::t* x, size_t xstride,
@@ -504,12 +482,6 @@ hipError_t fame_cg_begin_mixed(void* x32, void* r32, void* p32, const void* b64,
                                size_t stride, int nprob, const int* mask, hipStream_t s) {
     hipLaunchKernelGGL(k_cg_begin_from64, vgrid(n, nprob, nblk), dim3(RB), 0, s, (float2*)x32, (float2*)r32, (float2*)p32,
                        (const double2*)b64, partials, n, stride, mask);
-    return hipGetLastError();
-}
-hipError_t fame_cg_finish_mixed(void* x64, const void* x32, const void* p32, const CgScal* sc, size_t n, size_t stride,
-                                int nprob, const int* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_cg_finish_mixed, dim3(1024, nprob), dim3(RB), 0, s, (double2*)x64, (const float2*)x32,
-                       (const float2*)p32, sc, n, stride, mask);
     return hipGetLastError();
 }
 

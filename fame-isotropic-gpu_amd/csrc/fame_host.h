@@ -95,15 +95,16 @@ struct fame_ctx {
     fame_status field_to_ref(const void* fint, void* fref, int nprob, size_t dst_stride);
     fame_status field_from_ref(const void* fref, void* fint, int nprob);
     /* b64: b is complex double while prec is 4 (rounded to FP32 at the start);
-     * x64_accum: with prec 4, the FP32 solution is added to this complex
-     * double vector (problems of mask_dev) instead of being finished in x;
+     * defer_finish: the last pending x += alpha p is left to the caller
+     * (qbq_residual32 applies it while adding x into the FP64 iterate);
      * begun_nblk > 0: x, r, p were set up by qbq_residual32 (or the caller),
      * with the <r,r> partials in partials, begun_nblk per problem */
     fame_status cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                   const int* mask_dev, int* iters, bool b64 = false, void* x64_accum = nullptr, int begun_nblk = 0);
-    /* refinement residual: QBQ64 xr, whose last pass stores the next FP32
-     * solve's start r = p = FP32(b - QBQ64 xr), x = 0, and its norm partials
-     * (partials[problem][inner block]) */
+                   const int* mask_dev, int* iters, bool b64 = false, bool defer_finish = false, int begun_nblk = 0);
+    /* refinement residual: xr += x32 + alpha p32 (the FP32 solve's result and
+     * its pending update) on the first pass, then QBQ64 xr, whose last pass
+     * stores the next FP32 solve's start r = p = FP32(b - QBQ64 xr), x = 0, and
+     * its norm partials (partials[problem][inner block]) */
     fame_status qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act);
     /* x: problem stride xstride (0: 2N); out (optional): the result is written
      * there, problem stride ostride, for the problems of mask_dev only, and y

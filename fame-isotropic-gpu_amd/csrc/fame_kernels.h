@@ -59,6 +59,12 @@ struct KInner {
      * storing y (alpha of this iteration, already in cgsc), and puts <r,r> of
      * the updated residual in slot 3 of 6 doubles per block                 */
     const void* cg_ap;
+    /* EXPAND, refinement (fx32 set): before it is expanded, x += fx32 + alpha
+     * fp32 (the FP32 solve's solution and its pending update, alpha and pend
+     * of cgsc, FP32 complex vectors with problem stride xstride) is formed in
+     * FP32, added in FP64 and stored back to x                               */
+    const void* fx32;
+    const void* fp32;
     /* !EXPAND, refinement residual (rb set, complex double like y): instead
      * of y the pass stores the next FP32 solve's start, r32 = p32 = rb - y
      * rounded to FP32 and x32 = 0 (FP32 complex vectors, problem stride
@@ -131,13 +137,10 @@ hipError_t fame_cg_beta(CgScal* sc, const double* partials, int nblk, int nprob,
 hipError_t fame_cg_alpha_beta(CgScal* sc, const double* partials, int nblk, int nprob, int* active, hipStream_t s);
 hipError_t fame_cg_finish_x(void* x, const void* p, const CgScal* sc, size_t n, size_t stride, int nprob, int prec,
                             hipStream_t s);
-/* FP32 solves inside FP64 refinement: start from the FP64 right-hand side
- * (x32 = 0, r32 = p32 = b64 rounded), and end by accumulating the FP32
- * solution with its pending update into x64 (problems of the mask only) */
+/* an FP32 solve inside FP64 refinement starting from the FP64 right-hand
+ * side: x32 = 0, r32 = p32 = b64 rounded, with the <r,r> partials */
 hipError_t fame_cg_begin_mixed(void* x32, void* r32, void* p32, const void* b64, double* partials, int nblk, size_t n,
                                size_t stride, int nprob, const int* mask, hipStream_t s);
-hipError_t fame_cg_finish_mixed(void* x64, const void* x32, const void* p32, const CgScal* sc, size_t n, size_t stride,
-                                int nprob, const int* mask, hipStream_t s);
 
 /* generic helpers (complex double unless prec says float) */
 hipError_t fame_scale_real(void* y, const void* x, const double* sreal, size_t n, size_t xstride, size_t sstride,

@@ -114,6 +114,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         /* r -= alpha Ap of the previous iteration rides on this pass too */
         const bool rup = cgf && pend && args.cg_ap != nullptr;
         const T2* const AP = rup ? reinterpret_cast<const T2*>(args.cg_ap) + (size_t)p * args.xstride : nullptr;
+        /* refinement: x += fx32 + alpha fp32 of the FP32 solve that just ended */
+        const bool FX = args.fx32 != nullptr;
+        const float2* const FX32 = FX ? reinterpret_cast<const float2*>(args.fx32) + (size_t)p * args.xstride : nullptr;
+        const float2* const FP32 = FX ? reinterpret_cast<const float2*>(args.fp32) + (size_t)p * args.xstride : nullptr;
+        float far = 0.0f, fai = 0.0f;
+        bool fpend = false;
+        if (FX) {
+            const CgScal& cs = args.cgsc[p];
+            far = (float)cs.alpha_re; fai = (float)cs.alpha_im; fpend = cs.pend != 0;
+        }
         struct XIn { T2 x1, x2, r1, r2, s1, s2, a1, a2; };
         double rnorm = 0.0;   /* <r,r> of the updated residual, this thread's share */
         batched_loop<InflightInner<T>::v>(
@@ -125,6 +135,14 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 if (gl < nlines && a < na) {
                     const unsigned f = (unsigned)(gl * na + a);
                     d.x1 = X[f]; d.x2 = X[N + f];
+                    if (FX) {
+                        const float2 u1 = FX32[f], u2 = FX32[N + f];
+                        d.s1.x = u1.x; d.s1.y = u1.y; d.s2.x = u2.x; d.s2.y = u2.y;
+                        if (fpend) {
+                            const float2 v1 = FP32[f], v2 = FP32[N + f];
+                            d.a1.x = v1.x; d.a1.y = v1.y; d.a2.x = v2.x; d.a2.y = v2.y;
+                        }
+                    }
                     if (cgf) {
                         d.r1 = R[f]; d.r2 = R[N + f];
                         if (pend) { d.s1 = Xs[f]; d.s2 = Xs[N + f]; }
@@ -140,6 +158,17 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 if (gl < nlines && a < na) {
                     const unsigned f = (unsigned)(gl * na + a);
                     T2 xv1 = d.x1, xv2 = d.x2;
+                    if (FX) {
+                        float2 u1 = make_float2((float)d.s1.x, (float)d.s1.y), u2 = make_float2((float)d.s2.x, (float)d.s2.y);
+                        if (fpend) {
+                            const float2 v1 = make_float2((float)d.a1.x, (float)d.a1.y);
+                            const float2 v2 = make_float2((float)d.a2.x, (float)d.a2.y);
+                            u1.x += far * v1.x - fai * v1.y; u1.y += far * v1.y + fai * v1.x;
+                            u2.x += far * v2.x - fai * v2.y; u2.y += far * v2.y + fai * v2.x;
+                        }
+                        xv1.x += (T)u1.x; xv1.y += (T)u1.y; xv2.x += (T)u2.x; xv2.y += (T)u2.y;
+                        Pw[f] = xv1; Pw[N + f] = xv2;
+                    }
                     if (cgf) {
                         if (pend) {
                             T2 s1 = d.s1, s2 = d.s2;

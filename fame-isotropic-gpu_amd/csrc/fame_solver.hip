@@ -556,14 +556,15 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
 }
 
 fame_status fame_ctx::qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act) {
+    const size_t es = 8 * 2 * (size_t)N;   /* FP32 complex vector bytes per problem */
     KInner ki = kin(8, p0, act, LIST_INV);
     ki.x = xr; ki.pikind = 0;
+    ki.fx32 = x32; ki.fp32 = (char*)cg32[1] + es * p0; ki.cgsc = cgsc;
     HIPCHK(fame_launch_inner(ki, 1, 8, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_INV), +1, 8, nprob, stream));
     HIPCHK(fame_launch_outer(kout(8, p0, act, LIST_EPS), OUTER_EPS, 8, nprob, stream));
     HIPCHK(fame_launch_mid(kmid(8, p0, act, LIST_FWD), -1, 8, nprob, stream));
     KInner kc = kin(8, p0, act, LIST_FWD);
-    const size_t es = 8 * 2 * (size_t)N;   /* FP32 complex vector bytes per problem */
     kc.x = xr; kc.y = const_cast<void*>(b); kc.pikind = 0;
     kc.rb = b; kc.x32 = x32;
     kc.r32 = (char*)cg32[0] + es * p0; kc.p32 = (char*)cg32[1] + es * p0;
@@ -774,8 +775,8 @@ extern "C" fame_status fame_autotune(fame_ctx* c, int prec, int* choice) {
  * Stops per problem when <r,r> <= max(tol^2, reltol^2 <b,b>) or after maxit
  * (reference counter semantics, CG.cu:36-78).  iters[] (host) receives k.    */
 fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, double tol, double reltol, int maxit,
-                         const int* mask_dev, int* iters, bool b64, void* x64_accum, int begun_nblk) {
-    if ((b64 || x64_accum) && prec != 4) FAIL(FAME_ERR_ARG, "cg: FP64 right-hand side or accumulation needs prec 4");
+                         const int* mask_dev, int* iters, bool b64, bool defer_finish, int begun_nblk) {
+    if (b64 && prec != 4) FAIL(FAME_ERR_ARG, "cg: an FP64 right-hand side needs prec 4");
     void** V = prec == 8 ? cg64 : cg32;
     const size_t n = 2 * (size_t)N, st = 2 * (size_t)N;
     const size_t es = prec == 8 ? 16 : 8;
@@ -804,8 +805,7 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
         if (!any || done_it >= maxit) break;
         chunk = 2;
     }
-    if (x64_accum) HIPCHK(fame_cg_finish_mixed(x64_accum, x, p, cgsc, n, st, nprob, mask_dev, stream));
-    else HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
+    if (!defer_finish) HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
     int mx = 0;
     for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc_h[i].iters; mx = std::max(mx, sc_h[i].iters); }
     cg_predict[pix(prec)] = std::max(2, mx - 1);
@@ -924,10 +924,10 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
     if (mode == FAME_PREC_F64) {
         CHK(cg(8, b, y, p0, nprob, ls.tol, 0.0, ls.maxit, mask_dev, it.data()));
     } else {
-        /* FP64 iterative refinement around FP32 CG solves: each sweep adds
-         * the FP32 solution into the FP64 iterate, and the FP64 residual pass
-         * writes the next FP32 solve's start (r = p = FP32(b - QBQ64 x), x = 0)
-         * with its norm, so no FP64 residual vector is kept */
+        /* FP64 iterative refinement around FP32 CG solves: the FP64 residual
+         * pass adds each FP32 solution into the FP64 iterate on its first pass
+         * and writes the next FP32 solve's start (r = p = FP32(b - QBQ64 x),
+         * x = 0) with its norm on its last, so no FP64 residual vector is kept */
         double2* xr = (double2*)y;
         float2* d32 = (float2*)cg32[4] + st * p0;
         HIPCHK(hipMemsetAsync(xr, 0, 16 * st * nprob, stream));
@@ -953,7 +953,7 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
             if (!any) break;
             HIPCHK(hipMemcpyAsync(mask, msk.data(), sizeof(int) * nprob, hipMemcpyHostToDevice, stream));
             std::vector<int> it32(nprob, 0);
-            CHK(cg(4, b, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, xr, nbp));
+            CHK(cg(4, b, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, true, nbp));
             for (int i = 0; i < nprob; ++i) if (msk[i]) it[i] += it32[i] - 1;
             /* the next sweep's start from r = b - QBQ64 xr, with its norm */
             CHK(qbq_residual32(xr, b, d32, p0, nprob, mask));
