@@ -10,12 +10,17 @@ Lanczos and CG tolerance 1e-12.  Synthetic input: the structure is generated,
 nothing is read from disk.
 
 One step = one batched FP32+FP64-refinement eigensolve of all 30 wave vectors
-(one fame_eigs call).  The plain FP64 solver (one wave vector at a time, FP64
-throughout: the reference algorithm) is timed beside it on `--plain-nk` of the
-wave vectors, and the per-band eigenvalue error of the batched solve against it
-is reported.  The roofline entry prices the dominant QBQ pass kernel with its
-hipEvent-timed average duration; cpu_baseline times the FP64 CPU restatement
-(oracle) on a bounded sample of QBQ applies at the same grid.
+(one fame_eigs call) with the restored E fields and the GPU relative-residual
+check of every band (fame_residuals), as the metric counts a k-point's
+eigensolve.  The plain FP64 solver (one wave vector at a time, FP64 throughout:
+the reference algorithm, also with restoration and residuals) is timed beside
+it on `--plain-nk` of the wave vectors, and the per-band eigenvalue error of the
+batched solve against it is reported.  The roofline entry prices the solve's
+dominant kernel -- one of the FP32 CG iteration kernels with their fused vector
+updates -- with its hipEvent-timed average duration and algorithmic bytes;
+cpu_baseline times the FP64 CPU restatement (oracle/) on complete invAr solves
+and Gram-Schmidt steps at the same grid, priced with the plain solve's invAr
+count per k-point.
 """
 import argparse
 import json
@@ -44,34 +49,85 @@ def build_workload(nx, nk):
     return F, lo, fl, eps, ks
 
 
-def cpu_baseline(fl, eps, k, qbq_per_k, seconds):
-    """FP64 oracle QBQ applies on all host cores for ~`seconds`; k-pts/s from the
-    measured QBQ count per k-point of the same algorithm."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def _host_cores():
     cores = os.cpu_count() or 1
     try:
         cores = len(os.sched_getaffinity(0))
     except Exception:
         pass
+    return cores
+
+
+def _median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def cpu_baseline(fl, eps, k, es_iter_per_k, nwant, nstep, reps=5):
+    """The FP64 CPU restatement of the reference algorithm (oracle/) on the
+    host cores, priced per wave vector of the same workload:
+
+      seconds per k = (invAr solves per k) x (t_invAr + t_reorth)
+
+    t_invAr: median of `reps` complete invAr solves (Sigma^-1 CG(QBQ, Sigma^-1 y)
+    to the workload's absolute tolerance, CG vector work included) from unit
+    random right-hand sides, on all host cores.  t_reorth: median of `reps`
+    classical Gram-Schmidt steps (h = U^H w, w -= U h, numpy/BLAS) against the
+    average number of basis columns a step sees, (nwant + nstep) / 2.  The invAr
+    count per k is the one the plain FP64 GPU solve measured (same algorithm).
+    The single-thread figure times `reps` QBQ applies on one thread and scales
+    t_invAr by the all-core QBQ-to-invAr ratio; its reorthogonalisation is timed
+    on one BLAS thread.  Lanczos restarts and the tridiagonal solves (O(nstep^2)
+    per restart) are left out."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from threadpoolctl import threadpool_limits
+    cores = _host_cores()
     O.set_threads(cores)
     t = O.Tables(O.lattice_from(fl), k)
     rng = np.random.default_rng(0)
-    x = rng.standard_normal(2 * t.Nd) + 1j * rng.standard_normal(2 * t.Nd)
     invB = 1.0 / eps
-    t.qbq(invB, x)                       # warm (plans, pages)
-    n, t0 = 0, time.perf_counter()
-    while True:
-        t.qbq(invB, x)
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= seconds or n >= 200:
-            break
-    per_qbq = el / n
-    return {"value": 1.0 / (per_qbq * qbq_per_k), "unit": "k-pts/s", "cores": cores, "kind": "port",
-            "sample": f"{n} FP64 QBQ applies of the CPU restatement at {tuple(fl.grid_nums)} on {cores} threads "
-                      f"({per_qbq:.3f} s each) x {qbq_per_k} QBQ applies per k-point measured by the plain FP64 "
-                      f"GPU solve; CPU CG/Lanczos vector work not included"}
+    nd2 = 2 * t.Nd
+
+    def unit():
+        y = rng.standard_normal(nd2) + 1j * rng.standard_normal(nd2)
+        return y / np.linalg.norm(y)
+
+    x = unit()
+    t.qbq(invB, x)                                        # warm (plans, pages)
+    t_qbq, _ = _median_time(lambda: t.qbq(invB, x), reps)
+    its = []
+    t_inv, _ = _median_time(lambda: its.append(t.invar(invB, unit(), maxit=1000, tol=1e-12)[1]), reps)
+    m = (nwant + nstep) // 2
+    U = rng.standard_normal((nd2, m)) + 1j * rng.standard_normal((nd2, m))
+    w = unit()
+
+    def cgs():
+        h = U.conj().T @ w
+        w[:] = w - U @ h
+
+    t_ro, _ = _median_time(cgs, reps)
+    O.set_threads(1)
+    t_qbq1, _ = _median_time(lambda: t.qbq(invB, x), reps)
+    with threadpool_limits(limits=1):
+        t_ro1, _ = _median_time(cgs, reps)
+    O.set_threads(cores)
+    t_inv1 = t_qbq1 * (t_inv / t_qbq)
+    per_k = es_iter_per_k * (t_inv + t_ro)
+    per_k1 = es_iter_per_k * (t_inv1 + t_ro1)
+    del U
+    return {"value": 1.0 / per_k, "unit": "k-pts/s", "cores": cores, "kind": "port",
+            "value_1thread": 1.0 / per_k1,
+            "sample": f"FP64 CPU restatement (oracle/) at {tuple(fl.grid_nums)}: median of {reps} complete invAr "
+                      f"solves on {cores} threads ({t_inv:.3f} s, {int(np.median(its))} CG iterations) and of {reps} "
+                      f"Gram-Schmidt steps against {m} columns ({t_ro:.3f} s), x {es_iter_per_k} invAr per k-point "
+                      f"(the plain FP64 GPU solve's count); 1 thread: {reps} QBQ applies ({t_qbq1:.3f} s vs "
+                      f"{t_qbq:.3f} s on {cores}) scaling the invAr time, Gram-Schmidt on one BLAS thread "
+                      f"({t_ro1:.3f} s); restarts and tridiagonal solves not included"}
 
 
 COPY_BYTES = 1 << 30
@@ -111,8 +167,12 @@ def passes_only(F, fl, eps, ks, args, torch, dev):
             S.set_tiles(4 if key == "fp32" else 8, tune[key])
     else:
         tune = {"fp32": S.autotune(4), "fp64": S.autotune(8)}
-    us32 = S.time_qbq_passes(4, args.reps)
-    us64 = S.time_qbq_passes(8, args.reps)
+    if args.cg_only:
+        cg32 = S.time_cg_passes(4, args.reps)
+        us32 = us64 = None
+    else:
+        us32 = S.time_qbq_passes(4, args.reps)
+        us64 = S.time_qbq_passes(8, args.reps)
     S.close()
     a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device=dev).fill_(1.0)
     b = torch.empty_like(a)
@@ -124,11 +184,17 @@ def passes_only(F, fl, eps, ks, args, torch, dev):
     e1.record()
     e1.synchronize()
     cp_us = e0.elapsed_time(e1) / args.reps * 1e3
-    names = ["inner_expand", "mid_inverse", "outer_eps", "mid_forward", "inner_contract", "qbq"]
-    print(json.dumps({"grid": list(fl.grid_nums), "nk": args.nk,
-                      "fp32_us": dict(zip(names, map(float, us32))), "fp64_us": dict(zip(names, map(float, us64))),
-                      "copy_bytes": COPY_BYTES, "copy_us": cp_us, "copy_gbs": 2 * COPY_BYTES / cp_us / 1e3,
-                      "tuned_tiles": tune}))
+    out = {"grid": list(fl.grid_nums), "nk": args.nk, "copy_bytes": COPY_BYTES, "copy_us": cp_us,
+           "copy_gbs": 2 * COPY_BYTES / cp_us / 1e3, "tuned_tiles": tune}
+    if args.cg_only:
+        names = ["cg_inner_expand", "cg_mid_inverse", "cg_outer_eps", "cg_alpha", "cg_mid_forward",
+                 "cg_inner_contract", "cg_beta", "cg_iteration"]
+        out["fp32_cg_us"] = dict(zip(names, map(float, cg32)))
+    else:
+        names = ["inner_expand", "mid_inverse", "outer_eps", "mid_forward", "inner_contract", "qbq"]
+        out["fp32_us"] = dict(zip(names, map(float, us32)))
+        out["fp64_us"] = dict(zip(names, map(float, us64)))
+    print(json.dumps(out))
 
 
 def main():
@@ -141,11 +207,14 @@ def main():
     ap.add_argument("--nwant", type=int, default=16)
     ap.add_argument("--nstep", type=int, default=48)
     ap.add_argument("--plain-nk", type=int, default=30, help="wave vectors timed with the plain FP64 solver")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-reps", type=int, default=5, help="repetitions (median) of each CPU baseline timing")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--passes-only", action="store_true",
                     help="only time the five QBQ passes on the batch (FP32 and FP64) and one plain copy of "
                          "known size (the reference kernel for HBM counter scaling); for profiler runs")
+    ap.add_argument("--cg-only", action="store_true",
+                    help="with --passes-only: time only the kernels of one fused FP32 CG iteration (for counter runs "
+                         "of those kernels alone)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--tiles", type=str, default=None,
                     help="with --passes-only: fixed tiles instead of tuning, 'fp32:e,L,e,L,e,L;fp64:e,L,e,L,e,L' "
@@ -171,33 +240,49 @@ def main():
     S = F.Solver(fl, max_batch=args.nk)
     S.set_eps(eps)
     S.set_wave_vectors(ks)
+    def solve(solver, prec):
+        # one k-point eigensolve as the metric counts it: Lanczos, restoration
+        # of the fields, and the relative residual check on the GPU
+        freq, _, E, prof = solver.eigs(args.nwant, args.nstep, tol=tol, maxit=maxit, precision=prec, want_fields=True)
+        res = solver.residuals(freq, E)
+        del E
+        return freq, res, prof
+
     for _ in range(args.warmup):
-        S.eigs(args.nwant, args.nstep, tol=tol, maxit=maxit, precision=F.PREC_F32_REFINE)
+        solve(S, F.PREC_F32_REFINE)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         S.set_wave_vectors(ks)
-        freq_b, _, _, prof_b = S.eigs(args.nwant, args.nstep, tol=tol, maxit=maxit, precision=F.PREC_F32_REFINE)
+        freq_b, res_b, prof_b = solve(S, F.PREC_F32_REFINE)
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
     kps_b = args.nk * args.steps / t_b
     log(f"batched FP32+refine: {t_b:.2f}s for {args.steps} step(s) -> {kps_b:.3f} k-pts/s")
 
-    # ---- roofline of the dominant QBQ pass (FP32 passes dominate the batched solve) ----
+    # ---- roofline of the solve's dominant kernel ----
+    # FP32 CG iterations are most of the batched solve (qbq32 vs qbq64 per k in
+    # the profile); its kernels are timed as the solver launches them, fused
+    # vector updates included, with all wave vectors active.
     tiles = {"fp32": S.autotune(4), "fp64": S.autotune(8)}    # the choices the solve used (same batch shape)
     us32 = S.time_qbq_passes(4, 10)
     us64 = S.time_qbq_passes(8, 10)
+    cg32 = S.time_cg_passes(4, 10)
     N = S.N
-    s32, s64, r32 = 8, 16, 4
-    bytes_pass = lambda s: [5 * N * s, 6 * N * s, 6 * N * s + 3 * N * (s // 2), 6 * N * s, 5 * N * s]
-    names = ["inner_expand", "mid_inverse", "outer_eps", "mid_forward", "inner_contract"]
-    tot32 = [us32[i] for i in range(5)]
-    dom = int(np.argmax(tot32))
-    alg_bytes = bytes_pass(s32)[dom] * args.nk
-    achieved = alg_bytes / (us32[dom] * 1e-6) / 1e9
-    traffic, traffic_note = measured_traffic(f"k_{names[dom]}_fp32", tiles, list(fl.grid_nums), args.nk)
+    s32, r32 = 8, 4
+    cg_names = ["cg_inner_expand", "cg_mid_inverse", "cg_outer_eps", "cg_alpha", "cg_mid_forward",
+                "cg_inner_contract", "cg_beta"]
+    # algorithmic bytes per wave vector: expand reads p, r, x (3 x 2N) and writes
+    # p, x (2 x 2N) and the field (3N); the outer pass adds 1/eps (3N reals); the
+    # contracting pass reads the field (3N) and reads and writes r (2 x 2N)
+    cg_bytes = [13 * N * s32, 6 * N * s32, 6 * N * s32 + 3 * N * r32, 0, 6 * N * s32, 7 * N * s32, 0]
+    dom = int(np.argmax(cg32[:7]))
+    alg_bytes = cg_bytes[dom] * args.nk
+    achieved = alg_bytes / (cg32[dom] * 1e-6) / 1e9
+    traffic, traffic_note = measured_traffic(f"k_{cg_names[dom]}_fp32", tiles, list(fl.grid_nums), args.nk)
     qbq_alg = (10 * N * s32 + 3 * N * r32) * args.nk
     qbq_gbs = qbq_alg / (us32[5] * 1e-6) / 1e9
+    cg_iter_gbs = sum(cg_bytes) * args.nk / (cg32[7] * 1e-6) / 1e9
     S.close()
 
     # ---- plain FP64, one wave vector at a time ----
@@ -209,20 +294,21 @@ def main():
         P = F.Solver(fl, max_batch=1)
         P.set_eps(eps)
         P.set_wave_vectors(ks[:1])
-        P.eigs(args.nwant, args.nstep, tol=tol, maxit=maxit, precision=F.PREC_F64)   # warm
+        solve(P, F.PREC_F64)                                                        # warm
         freq_p = np.zeros((plain_nk, args.nwant))
-        qbqs = []
+        res_p = np.zeros((plain_nk, args.nwant))
+        invar_per_k = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(plain_nk):
             P.set_wave_vectors(ks[i:i + 1])
-            f, _, _, pr = P.eigs(args.nwant, args.nstep, tol=tol, maxit=maxit, precision=F.PREC_F64)
-            freq_p[i] = f[0]
-            qbqs.append(pr[0]["qbq64"])
+            f, r, pr = solve(P, F.PREC_F64)
+            freq_p[i], res_p[i] = f[0], r[0]
+            invar_per_k.append(pr[0]["es_iter"])
         torch.cuda.synchronize()
         t_p = time.perf_counter() - t0
         kps_p = plain_nk / t_p
-        qbq_per_k = int(np.mean(qbqs))
+        qbq_per_k = int(np.mean(invar_per_k))
         P.close()
         log(f"plain FP64: {t_p:.2f}s for {plain_nk} wave vectors -> {kps_p:.3f} k-pts/s")
 
@@ -234,7 +320,11 @@ def main():
 
     cpu = None
     if not args.no_cpu and qbq_per_k:
-        cpu = cpu_baseline(fl, eps, ks[min(3, args.nk - 1)], qbq_per_k, args.cpu_seconds)
+        cpu = cpu_baseline(fl, eps, ks[min(3, args.nk - 1)], qbq_per_k, args.nwant, args.nstep, args.cpu_reps)
+
+    def max_rel_res(freq, res):
+        m = freq > 0
+        return float(np.max(res[m] / freq[m] ** 2)) if m.any() else None, float(np.max(res[m])) if m.any() else None
 
     out = {
         "metric": "k-pts/s @96^3,16 bands (batched FP32+refine vs plain FP64); HBM GB/s",
@@ -254,8 +344,12 @@ def main():
                                f"{args.nk} k on Gamma-X-W-L-Gamma, {args.nwant} bands, nstep {args.nstep}, tol 1e-12",
                    "global_batch": args.nk, "parallelism": "single GPU, all wave vectors in one batch"},
         "plain_fp64": {"value": kps_p, "unit": "k-pts/s", "nk_timed": plain_nk,
-                       "speedup_batched_vs_plain": (kps_b / kps_p) if kps_p else None},
+                       "speedup_batched_vs_plain": (kps_b / kps_p) if kps_p else None,
+                       "invar_per_k": qbq_per_k},
         "max_rel_eig_err_vs_fp64": err,
+        "max_rel_residual": {"batched": max_rel_res(freq_b, res_b)[1],
+                             "plain_fp64": max_rel_res(freq_p, res_p)[1] if freq_p is not None else None,
+                             "definition": "max over bands of ||C^H C E - omega^2 B E|| / ||E|| (GPU fame_residuals)"},
         "profile_batched": {"ls_iter_mean": float(np.mean([p["ls_iter"] for p in prof_b])),
                             "es_iter_mean": float(np.mean([p["es_iter"] for p in prof_b])),
                             "restarts_max": int(max(p["restarts"] for p in prof_b)),
@@ -263,11 +357,14 @@ def main():
                             "all_converged": bool(all(p["converged"] for p in prof_b))},
         "qbq_fp32_us_per_batch": us32[5], "qbq_fp64_us_per_batch": us64[5],
         "qbq_path_gbs": qbq_gbs,
+        "cg_iteration_us": float(cg32[7]), "cg_iteration_gbs": cg_iter_gbs,
+        "cg_kernels_us": dict(zip(cg_names, map(float, cg32[:7]))),
         "tuned_tiles": tiles,
-        "roofline": {"bound": "hbm", "kernel": f"k_{names[dom]} (FP32, {args.nk} problems per launch)",
+        "roofline": {"bound": "hbm", "kernel": f"{cg_names[dom]} (FP32 k_inner/k_mid/k_outer with the fused CG "
+                                               f"updates, {args.nk} wave vectors active per launch)",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
-                     "avg_launch_us": us32[dom], "algorithmic_bytes_per_launch": alg_bytes},
+                     "avg_launch_us": float(cg32[dom]), "algorithmic_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

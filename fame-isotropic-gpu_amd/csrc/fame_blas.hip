@@ -340,27 +340,29 @@ __global__ void k_gemv_n(const double2* V, double2* w, const double* h, int m, s
 }
 
 /* Gram matrix G[j][l] = sum_i conj(V[j][i]) V[l][i] over the upper triangle,
- * on the matrix cores (v_mfma_f64_16x16x4f64).  A block stages GK rows of all
- * m columns (m <= 64, padded to 16-column tiles) in LDS with an odd pitch, and
- * each wave accumulates a fixed set of upper-triangle 16x16 tile pairs:
+ * on the matrix cores (v_mfma_f64_16x16x4f64).  A block stages GKT rows of all
+ * m columns (m <= 16*MT, padded to 16-column tiles) in LDS with an odd pitch,
+ * and each of the 4 waves accumulates a fixed set of upper-triangle 16x16 tile
+ * pairs (pair w + 4q, q < QM; MT(MT+1)/2 <= 4 QM pairs):
  *   Re G += Vr^T Vr + Vi^T Vi,   Im G += Vr^T Vi - Vi^T Vr.
  * Operands: lane l holds A[l % 16][l / 16] = conj V_j(k) and B[l / 16][l % 16]
  * = V_l(k); result element v of lane l is row l / 16 + 4 v, column l % 16.
- * Partial per block: 2 m^2 doubles (lower triangle zero).                    */
-#define GK 32
+ * Instances: (GKT 32, MT 4, QM 3) for m <= 64 and (16, 8, 9) for m <= 128,
+ * both 34-35 KB of LDS.  Partial per block: 2 m^2 doubles (lower triangle zero). */
 typedef double gacc_t __attribute__((ext_vector_type(4)));
+template <int GKT, int MT, int QM>
 __global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials, int m, size_t n, size_t cstride,
                                              size_t pstride, const int* mask) {
     const int p = blockIdx.y;
     if (mask && !mask[p]) return;
-    __shared__ double2 tile[64 * (GK + 1)];
+    __shared__ double2 tile[16 * MT * (GKT + 1)];
     const int mt = (m + 15) >> 4, mp = mt * 16;
     const int npair = mt * (mt + 1) / 2;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row = lane & 15, kq = lane >> 4;
-    /* tile pairs of this wave: w, w + 4, w + 8 (at most 10 pairs for m <= 64) */
-    int pj[3], pl[3];
-    for (int q = 0; q < 3; ++q) {
+    int pj[QM], pl[QM];
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
         int t = w + 4 * q, j = 0;
         pj[q] = -1; pl[q] = -1;
         if (t < npair) {
@@ -368,25 +370,26 @@ __global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials,
             pj[q] = j; pl[q] = j + t;
         }
     }
-    gacc_t re[3], im[3];
-    for (int q = 0; q < 3; ++q) { re[q] = (gacc_t){0, 0, 0, 0}; im[q] = (gacc_t){0, 0, 0, 0}; }
+    gacc_t re[QM], im[QM];
+#pragma unroll
+    for (int q = 0; q < QM; ++q) { re[q] = (gacc_t){0, 0, 0, 0}; im[q] = (gacc_t){0, 0, 0, 0}; }
     const double2* Vp = V + (size_t)p * pstride;
-    const size_t chunk = (size_t)gridDim.x * GK;
-    for (size_t base = (size_t)blockIdx.x * GK; base < n; base += chunk) {
+    const size_t chunk = (size_t)gridDim.x * GKT;
+    for (size_t base = (size_t)blockIdx.x * GKT; base < n; base += chunk) {
         __syncthreads();
-        for (int t = threadIdx.x; t < mp * GK; t += RB) {
-            const int j = t / GK, r = t - j * GK;
+        for (int t = threadIdx.x; t < mp * GKT; t += RB) {
+            const int j = t / GKT, r = t - j * GKT;
             const size_t i = base + r;
-            tile[j * (GK + 1) + r] = (j < m && i < n) ? Vp[(size_t)j * cstride + i] : make_double2(0, 0);
+            tile[j * (GKT + 1) + r] = (j < m && i < n) ? Vp[(size_t)j * cstride + i] : make_double2(0, 0);
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
+        for (int q = 0; q < QM; ++q) {
             if (pj[q] < 0) continue;
-            const double2* A = tile + (pj[q] * 16 + row) * (GK + 1) + kq;
-            const double2* B = tile + (pl[q] * 16 + row) * (GK + 1) + kq;
+            const double2* A = tile + (pj[q] * 16 + row) * (GKT + 1) + kq;
+            const double2* B = tile + (pl[q] * 16 + row) * (GKT + 1) + kq;
 #pragma unroll
-            for (int k0 = 0; k0 < GK; k0 += 4) {
+            for (int k0 = 0; k0 < GKT; k0 += 4) {
                 const double2 a = A[k0], bb = B[k0];
                 re[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, bb.x, re[q], 0, 0, 0);
                 re[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bb.y, re[q], 0, 0, 0);
@@ -398,7 +401,8 @@ __global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials,
     double* o = partials + ((size_t)p * gridDim.x + blockIdx.x) * 2 * m * m;
     for (int t = threadIdx.x; t < 2 * m * m; t += RB) o[t] = 0;
     __syncthreads();
-    for (int q = 0; q < 3; ++q) {
+#pragma unroll
+    for (int q = 0; q < QM; ++q) {
         if (pj[q] < 0) continue;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -412,7 +416,8 @@ __global__ void __launch_bounds__(RB) k_gram(const double2* V, double* partials,
 }
 
 /* W[l][i] = sum_j V[j][i] Q[j][l], Q real m x q (column-major, leading dim m).
- * Output columns are produced 16 at a time; Q is staged in LDS.              */
+ * Output columns are produced 16 at a time; the 16 columns of Q they need
+ * (16 m doubles, at most 16 KB for m <= 128) are staged in LDS per group.    */
 __global__ void __launch_bounds__(RB) k_gemm_vq(const double2* V, const double* Q, double2* W, int m, int q, size_t n,
                                                 size_t cstride, size_t pstride, size_t wcstride, size_t wpstride,
                                                 size_t qstride, const int* mask) {
@@ -422,10 +427,12 @@ __global__ void __launch_bounds__(RB) k_gemm_vq(const double2* V, const double* 
     const double2* Vp = V + (size_t)p * pstride;
     double2* Wp = W + (size_t)p * wpstride;
     const double* Qp = Q + (size_t)p * qstride;
-    for (int t = threadIdx.x; t < m * q; t += RB) qs[t] = Qp[t];
-    __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
-        for (int l0 = 0; l0 < q; l0 += 16) {
+    for (int l0 = 0; l0 < q; l0 += 16) {
+        const int nl = min(16, q - l0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m * nl; t += RB) qs[t] = Qp[(size_t)l0 * m + t];
+        __syncthreads();
+        for (size_t i = (size_t)blockIdx.x * RB + threadIdx.x; i < n; i += (size_t)gridDim.x * RB) {
             double2 acc[16];
 #pragma unroll
             for (int l = 0; l < 16; ++l) acc[l] = make_double2(0, 0);
@@ -433,15 +440,15 @@ __global__ void __launch_bounds__(RB) k_gemm_vq(const double2* V, const double* 
                 double2 v = Vp[(size_t)j * cstride + i];
 #pragma unroll
                 for (int l = 0; l < 16; ++l) {
-                    if (l0 + l < q) {
-                        double qq = qs[(l0 + l) * m + j];
+                    if (l < nl) {
+                        double qq = qs[l * m + j];
                         acc[l].x += v.x * qq; acc[l].y += v.y * qq;
                     }
                 }
             }
 #pragma unroll
             for (int l = 0; l < 16; ++l)
-                if (l0 + l < q) Wp[(size_t)(l0 + l) * wcstride + i] = acc[l];
+                if (l < nl) Wp[(size_t)(l0 + l) * wcstride + i] = acc[l];
         }
     }
 }
@@ -557,13 +564,19 @@ hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n,
 }
 hipError_t fame_gram(const void* V, double* partials, int nblk, int m, size_t n, size_t cstride, size_t pstride, int nprob,
                      const int* mask, hipStream_t s) {
-    if (m > 64) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gram, dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, partials, m, n, cstride, pstride, mask);
+    if (m < 1 || m > FAME_LZ_MAXCOL) return hipErrorInvalidValue;
+    if (m <= 64)
+        hipLaunchKernelGGL((k_gram<32, 4, 3>), dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, partials, m, n, cstride,
+                           pstride, mask);
+    else
+        hipLaunchKernelGGL((k_gram<16, 8, 9>), dim3(nblk, nprob), dim3(RB), 0, s, (const double2*)V, partials, m, n, cstride,
+                           pstride, mask);
     return hipGetLastError();
 }
 hipError_t fame_gemm_vq(const void* V, const double* Q, void* W, int m, int q, size_t n, size_t cstride, size_t pstride,
                         size_t wcstride, size_t wpstride, size_t qstride, int nprob, const int* mask, hipStream_t s) {
-    hipLaunchKernelGGL(k_gemm_vq, dim3(1024, nprob), dim3(RB), sizeof(double) * m * q, s, (const double2*)V, Q, (double2*)W, m, q, n, cstride,
+    if (m < 1 || m > FAME_LZ_MAXCOL || q < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gemm_vq, dim3(1024, nprob), dim3(RB), sizeof(double) * m * 16, s, (const double2*)V, Q, (double2*)W, m, q, n, cstride,
                        pstride, wcstride, wpstride, qstride, mask);
     return hipGetLastError();
 }

@@ -637,6 +637,7 @@ struct GridDev {
     int na, nb, nc;
     int N;
     int general;
+    int pi_guard;             /* General with three equal mesh lengths: Lambda can be parallel to (1,1,1) */
     double ha, hb, hc;
     double iha, ihb, ihc;     /* 1/h */
     unsigned m_na, m_nb;      /* udiv magics for / na, / nb */
@@ -697,25 +698,37 @@ __device__ __forceinline__ double rsq(double x) {
     return rsqrt(x);
 }
 
-template <bool FAST = false>
+/* General gauge (FAME_Matrix_Lambdas.cu:441-461) with S = Lx + Ly + Lz:
+ *   p1 = (Lq - L_i conj(S)) / sqrt(Lq (3 Lq - |S|^2)),  p2 = conj(Lz - Ly, Lx - Lz, Ly - Lx) / sqrt(3 Lq - |S|^2).
+ * Its normalisers vanish (0/0) where L is parallel to (1, 1, 1), and lose
+ * accuracy near it.  Every symbol has the form (e^{i phi} - 1)/h_axis, so L can
+ * be parallel to (1, 1, 1) (at a nonzero mode) only when the three mesh lengths
+ * are equal.  Such grids (pi_guard) take the GUARD form: with d = (Lz - Ly,
+ * Lx - Lz, Ly - Lx), 3 Lq - |S|^2 = |d|^2 and the p1 numerator is conj(L) x d,
+ * both formed from d without cancellation, and a mode with |d|^2 <= 1e-30 Lq
+ * takes the Simple gauge (w = (3, 5, 1)), which is well defined there.  Every
+ * kernel evaluates the same form and branch from the same FP64 symbols, so the
+ * basis is consistent; other grids keep the reference's arithmetic, whose
+ * register footprint matters in the FP32 inner kernels.                     */
+template <bool FAST = false, bool GUARD = false>
 __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 ly, double2 lz,
                                          double2 p1[3], double2 p2[3]) {
     const double Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
-    if (!g.general) {
-        const double al = 3.0, be = 5.0;
-        double2 t1 = csub(cscale(lz, be), ly);
-        double2 t2 = csub(lx, cscale(lz, al));
-        double2 t3 = csub(cscale(ly, al), cscale(lx, be));
-        const double n2 = t1.x * t1.x + t1.y * t1.y + t2.x * t2.x + t2.y * t2.y + t3.x * t3.x + t3.y * t3.y;
-        const double i2 = rsq<FAST>(n2);
-        const double i1 = i2 * rsq<FAST>(Lq);
-        p1[0] = cscale(csub(cmulc(t3, ly), cmulc(t2, lz)), i1);
-        p1[1] = cscale(csub(cmulc(t1, lz), cmulc(t3, lx)), i1);
-        p1[2] = cscale(csub(cmulc(t2, lx), cmulc(t1, ly)), i1);
-        p2[0] = cscale(cconj(t1), i2);
-        p2[1] = cscale(cconj(t2), i2);
-        p2[2] = cscale(cconj(t3), i2);
-    } else {
+    if (GUARD) {
+        const double2 d1 = csub(lz, ly), d2 = csub(lx, lz), d3 = csub(ly, lx);
+        const double nd = d1.x * d1.x + d1.y * d1.y + d2.x * d2.x + d2.y * d2.y + d3.x * d3.x + d3.y * d3.y;
+        if (nd > 1e-30 * Lq) {
+            const double i2 = rsq<FAST>(nd);
+            const double i1 = i2 * rsq<FAST>(Lq);
+            p1[0] = cscale(csub(cmul(cconj(ly), d3), cmul(cconj(lz), d2)), i1);
+            p1[1] = cscale(csub(cmul(cconj(lz), d1), cmul(cconj(lx), d3)), i1);
+            p1[2] = cscale(csub(cmul(cconj(lx), d2), cmul(cconj(ly), d1)), i1);
+            p2[0] = cscale(cconj(d1), i2);
+            p2[1] = cscale(cconj(d2), i2);
+            p2[2] = cscale(cconj(d3), i2);
+            return;
+        }
+    } else if (g.general) {
         const double2 Ls = cadd(cadd(lx, ly), lz);
         const double Lp = Ls.x * Ls.x + Ls.y * Ls.y;
         const double i1 = rsq<FAST>(3.0 * Lq * Lq - Lq * Lp);
@@ -727,7 +740,21 @@ __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 l
         p2[0] = cscale(csub(cconj(lz), cconj(ly)), i2);
         p2[1] = cscale(csub(cconj(lx), cconj(lz)), i2);
         p2[2] = cscale(csub(cconj(ly), cconj(lx)), i2);
+        return;
     }
+    const double al = 3.0, be = 5.0;
+    double2 t1 = csub(cscale(lz, be), ly);
+    double2 t2 = csub(lx, cscale(lz, al));
+    double2 t3 = csub(cscale(ly, al), cscale(lx, be));
+    const double n2 = t1.x * t1.x + t1.y * t1.y + t2.x * t2.x + t2.y * t2.y + t3.x * t3.x + t3.y * t3.y;
+    const double i2 = rsq<FAST>(n2);
+    const double i1 = i2 * rsq<FAST>(Lq);
+    p1[0] = cscale(csub(cmulc(t3, ly), cmulc(t2, lz)), i1);
+    p1[1] = cscale(csub(cmulc(t1, lz), cmulc(t3, lx)), i1);
+    p1[2] = cscale(csub(cmulc(t2, lx), cmulc(t1, ly)), i1);
+    p2[0] = cscale(cconj(t1), i2);
+    p2[1] = cscale(cconj(t2), i2);
+    p2[2] = cscale(cconj(t3), i2);
 }
 
 }  // namespace fame

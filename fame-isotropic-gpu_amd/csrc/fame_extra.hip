@@ -150,6 +150,48 @@ __global__ void k_grad_acc(double* g, const double2* E, double coef, size_t n3) 
     }
 }
 
+/* Relative residuals of the bands of wave vectors p0..p0+np-1, through the
+ * FFT curl-curl pipeline on the context's field buffer, with Ecopy (np x 3N
+ * complex, kernel layout) holding the fields themselves for the residual sums. */
+static fame_status residuals_chunk(fame_ctx* c, int nwant, const double* freq, const void* E_dev, double* res,
+                                   double2* Ecopy, int p0, int np) {
+    const size_t N3 = 3 * (size_t)c->N;
+    const int NB = 256;
+    std::vector<double> w2(np), hp((size_t)np * NB * 2);
+    double* w2d = c->scal;
+    for (int i = 0; i < nwant; ++i) {
+        for (int q = 0; q < np; ++q) {
+            const int p = p0 + q;
+            const double2* src = (const double2*)E_dev + ((size_t)p * nwant + i) * N3;
+            CHK(c->field_from_ref(src, c->F64 + (size_t)p * N3, 1));
+            CHK(c->field_from_ref(src, Ecopy + (size_t)q * N3, 1));
+            w2[q] = freq[(size_t)p * nwant + i] * freq[(size_t)p * nwant + i];
+        }
+        KInner ki = c->kin(8, p0, nullptr, LIST_FI);
+        HIPCHK(hipMemcpyAsync(w2d, w2.data(), sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(fame_launch_outer(c->kout(8, p0, nullptr, LIST_FWD), OUTER_QRS, 8, np, c->stream));
+        HIPCHK(fame_launch_mid(c->kmid(8, p0, nullptr, LIST_FWD), -1, 8, np, c->stream));
+        dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, np);
+        const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
+        if (plan_engine(ki.plan) == ENG_MFMA) hipLaunchKernelGGL(k_curlcurl<ENG_MFMA>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        else hipLaunchKernelGGL(k_curlcurl<ENG_STOCKHAM>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        HIPCHK(hipGetLastError());
+        HIPCHK(fame_launch_mid(c->kmid(8, p0, nullptr, LIST_INV), +1, 8, np, c->stream));
+        HIPCHK(fame_launch_outer(c->kout(8, p0, nullptr, LIST_INV), OUTER_QR, 8, np, c->stream));
+        hipLaunchKernelGGL(k_resid, dim3(NB, np), dim3(256), 0, c->stream, c->F64 + (size_t)p0 * N3, Ecopy, c->eps_int,
+                           w2d, c->partials, N3, N3);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hp.data(), c->partials, sizeof(double) * np * NB * 2, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int q = 0; q < np; ++q) {
+            double a = 0, b = 0;
+            for (int t = 0; t < NB; ++t) { a += hp[((size_t)q * NB + t) * 2]; b += hp[((size_t)q * NB + t) * 2 + 1]; }
+            res[(size_t)(p0 + q) * nwant + i] = b > 0 ? std::sqrt(a / b) : 0.0;
+        }
+    }
+    return FAME_OK;
+}
+
 extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq, const void* E_dev, double* res) {
     if (!c || !freq || !E_dev || !res || nwant < 1) FAIL(FAME_ERR_ARG, "fame_residuals: bad arguments");
     if (!c->have_eps) FAIL(FAME_ERR_STATE, "fame_residuals: permittivity not set");
@@ -158,42 +200,8 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
     const size_t N3 = 3 * (size_t)c->N;
     double2* Ecopy = nullptr;
     HIPCHK(hipMalloc(&Ecopy, 16 * N3 * nk));
-    std::vector<double> w2(nk), part;
-    const int NB = 256;
-    std::vector<double> hp((size_t)nk * NB * 2);
-    double* w2d = c->scal;
-    fame_status st = FAME_OK;
-    for (int i = 0; i < nwant && st == FAME_OK; ++i) {
-        for (int p = 0; p < nk; ++p) {
-            const double2* src = (const double2*)E_dev + ((size_t)p * nwant + i) * N3;
-            if ((st = c->field_from_ref(src, c->F64 + (size_t)p * N3, 1)) != FAME_OK) break;
-            if ((st = c->field_from_ref(src, Ecopy + (size_t)p * N3, 1)) != FAME_OK) break;
-            w2[p] = freq[(size_t)p * nwant + i] * freq[(size_t)p * nwant + i];
-        }
-        if (st != FAME_OK) break;
-        KInner ki = c->kin(8, 0, nullptr, LIST_FI);
-        if (hipMemcpyAsync(w2d, w2.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-            fame_launch_outer(c->kout(8, 0, nullptr, LIST_FWD), OUTER_QRS, 8, nk, c->stream) != hipSuccess ||
-            fame_launch_mid(c->kmid(8, 0, nullptr, LIST_FWD), -1, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
-        dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, nk);
-        const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
-        if (plan_engine(ki.plan) == ENG_MFMA) hipLaunchKernelGGL(k_curlcurl<ENG_MFMA>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
-        else hipLaunchKernelGGL(k_curlcurl<ENG_STOCKHAM>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
-        if (hipGetLastError() != hipSuccess ||
-            fame_launch_mid(c->kmid(8, 0, nullptr, LIST_INV), +1, 8, nk, c->stream) != hipSuccess ||
-            fame_launch_outer(c->kout(8, 0, nullptr, LIST_INV), OUTER_QR, 8, nk, c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
-        hipLaunchKernelGGL(k_resid, dim3(NB, nk), dim3(256), 0, c->stream, c->F64, Ecopy, c->eps_int, w2d, c->partials,
-                           N3, N3);
-        if (hipMemcpyAsync(hp.data(), c->partials, sizeof(double) * nk * NB * 2, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess) { st = FAME_ERR_HIP; break; }
-        for (int p = 0; p < nk; ++p) {
-            double a = 0, b = 0;
-            for (int q = 0; q < NB; ++q) { a += hp[((size_t)p * NB + q) * 2]; b += hp[((size_t)p * NB + q) * 2 + 1]; }
-            res[(size_t)p * nwant + i] = b > 0 ? std::sqrt(a / b) : 0.0;
-        }
-    }
+    const fame_status st = residuals_chunk(c, nwant, freq, E_dev, res, Ecopy, 0, nk);
     (void)hipFree(Ecopy);
-    if (st == FAME_ERR_HIP) FAIL(FAME_ERR_HIP, std::string("fame_residuals: HIP error: ") + hipGetErrorString(hipGetLastError()));
     return st;
 }
 
@@ -234,6 +242,80 @@ extern "C" fame_status fame_time_qbq_passes(fame_ctx* c, int prec, int reps, dou
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
         us6[which] = 1000.0 * ms / reps;
     }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return FAME_OK;
+}
+
+/* Average durations (microseconds, hipEvent timing on the context stream) of
+ * the kernels of one fused CG iteration as the solver launches them
+ * (fame_ctx::cg_step_rc), with every problem of the context active:
+ *   us8 = {inner expand + x, p updates, middle inverse, outer eps + <p,Ap> sums,
+ *          alpha, middle forward, inner contract + r update and <r,r> sums,
+ *          beta and flags, whole iteration}.
+ * The CG state starts from x = 0, r = p = b for a constant nonzero b, with a
+ * zero tolerance, so that no problem stops during the timing, and one untimed
+ * iteration leaves the solution update pending as inside a solve.  Returns
+ * FAME_ERR_UNSUPPORTED when an axis plan is Bluestein (the iteration then
+ * stores Ap; time it with fame_time_qbq_passes).  Measurement helper for the
+ * benchmark, no reference counterpart.                                       */
+extern "C" fame_status fame_time_cg_passes(fame_ctx* c, int prec, int reps, double* us8) {
+    if (!c || !us8 || reps < 1 || (prec != 8 && prec != 4)) FAIL(FAME_ERR_ARG, "fame_time_cg_passes: bad arguments");
+    if (!c->have_eps || c->nk < 1) FAIL(FAME_ERR_STATE, "fame_time_cg_passes: permittivity and wave vectors must be set");
+    const int npap = c->pap_blocks(prec);
+    if (!npap) FAIL(FAME_ERR_UNSUPPORTED, "fame_time_cg_passes: an axis plan is Bluestein");
+    HIPCHK(hipSetDevice(c->device));
+    const int nk = c->nk;
+    const size_t st = 2 * (size_t)c->N, es = prec == 8 ? 16 : 8;
+    void** V = prec == 8 ? c->cg64 : c->cg32;
+    void* r = V[0];
+    void* pv = V[1];
+    void* b = V[3];
+    void* x = prec == 8 ? c->ref64[1] : c->cg32[4];
+    /* b: every byte 0x3c (1.15e-2 in FP32, 1.6e-19 in FP64), nonzero and finite */
+    HIPCHK(hipMemsetAsync(b, 0x3c, es * st * nk, c->stream));
+    KInner ki = c->kin(prec, 0, c->active, LIST_INV);
+    ki.x = pv; ki.pikind = 0;
+    ki.cg_r = r; ki.cg_x = x; ki.cgsc = c->cgsc; ki.cg_ap = nullptr; ki.partials = c->partials2;
+    const KAxis kmi = c->kmid(prec, 0, c->active, LIST_INV), kmf = c->kmid(prec, 0, c->active, LIST_FWD);
+    KAxis ko = c->kout(prec, 0, c->active, LIST_EPS);
+    ko.pap = c->pap_part;
+    KInner kc = c->kin(prec, 0, c->active, LIST_FWD);
+    kc.x = pv; kc.y = r; kc.pikind = 0;
+    kc.cg_r = r; kc.cgsc = c->cgsc; kc.partials = c->partials2;
+    const int nbi = c->nblk_in[prec == 8 ? 0 : 1];
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const long long c64 = c->cnt_qbq64, c32 = c->cnt_qbq32;
+    for (int which = 0; which < 8; ++which) {
+        /* a fresh state for every kernel (repeating one kernel alone drifts the
+         * vectors, and a non-finite <r,r> would clear the active flags), and
+         * one iteration: alpha leaves the x update pending, so that the first
+         * pass reads and writes x as it does inside a solve */
+        HIPCHK(fame_cg_begin(x, r, pv, b, c->partials, NBLK_VEC, st, st, nk, prec, nullptr, c->stream));
+        HIPCHK(fame_cg_init_scal2(c->cgsc, c->partials, NBLK_VEC, 0.0, 0.0, 1 << 30, nk, c->active, nullptr, c->stream));
+        CHK(c->cg_step_rc(prec, pv, r, x, 0, nk, npap));
+        for (int rep = -1; rep < reps; ++rep) {
+            if (rep == 0) HIPCHK(hipEventRecord(e0, c->stream));
+            switch (which) {
+                case 0: HIPCHK(fame_launch_inner(ki, 1, prec, nk, c->stream)); break;
+                case 1: HIPCHK(fame_launch_mid(kmi, +1, prec, nk, c->stream)); break;
+                case 2: HIPCHK(fame_launch_outer(ko, OUTER_EPS, prec, nk, c->stream)); break;
+                case 3: HIPCHK(fame_cg_alpha(c->cgsc, c->pap_part, npap, nk, c->active, c->stream)); break;
+                case 4: HIPCHK(fame_launch_mid(kmf, -1, prec, nk, c->stream)); break;
+                case 5: HIPCHK(fame_launch_inner(kc, 0, prec, nk, c->stream)); break;
+                case 6: HIPCHK(fame_cg_beta(c->cgsc, c->partials2, nbi, nk, c->active, c->stream)); break;
+                default: CHK(c->cg_step_rc(prec, pv, r, x, 0, nk, npap)); break;
+            }
+        }
+        HIPCHK(hipEventRecord(e1, c->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        us8[which] = 1000.0 * ms / reps;
+    }
+    c->cnt_qbq64 = c64; c->cnt_qbq32 = c32;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return FAME_OK;

@@ -68,6 +68,12 @@ struct fame_ctx {
     std::vector<void*> big;       /* Lanczos basis etc., reallocated per solve shape */
     /* tuning / state */
     int cg_predict[2] = {40, 40};
+    /* inverse-iteration sweeps applied by fame_eigs to the converged Ritz pairs
+     * (fame_set_refinement; 0 = the reference's algorithm), their absolute CG
+     * tolerance and iteration limit */
+    int ritz_sweeps = 0;
+    double ritz_tol = 1e-14;
+    int ritz_maxit = 2000;
     int refine_max_sweeps = 8;
     double refine_reltol = 1e-6;
     fame_ls ls_last = {0, 0.0};

@@ -105,6 +105,13 @@ __host__ __device__ inline int inner_tab_offset(int entries, int prec) {
     return prec == 8 ? entries : ((entries + 1) & ~1);
 }
 __host__ __device__ inline int inner_tab_extra(int na, int prec) { return prec == 8 ? na : 2 * na + 1; }
+/* behind it, per line, the inner twiddle exp(2 i pi phi a / na) split as
+ * a = 16 ah + al: 16 entries for al and ceil(na / 16) for ah, in the kernel's
+ * precision.  Entries (kernel precision) of both tables for L lines:         */
+__host__ __device__ inline int inner_tw_per_line(int na) { return 16 + (na + 15) / 16; }
+__host__ __device__ inline int inner_lds_extra(int na, int prec, int L) {
+    return inner_tab_extra(na, prec) + 1 + L * inner_tw_per_line(na);
+}
 
 /* LDS bytes of a line-transform launch: tables + nl lines + extra entries */
 size_t fame_line_lds(const fame::LinePlanDev& pl, int prec, int nl, int P, int extra);
@@ -158,6 +165,8 @@ hipError_t fame_gemv_h(const void* V, const void* w, double* partials, int nblk,
                        size_t pstride, size_t wstride, int nprob, const int* mask, hipStream_t s);
 hipError_t fame_gemv_n(const void* V, void* w, const double* h, int m, size_t n, size_t cstride, size_t pstride,
                        size_t wstride, size_t hstride, int nprob, const int* mask, hipStream_t s);
+/* most basis columns (nstep + 1) the Gram and U*Q kernels take */
+#define FAME_LZ_MAXCOL 128
 hipError_t fame_gram(const void* V, double* partials, int nblk, int m, size_t n, size_t cstride, size_t pstride,
                      int nprob, const int* mask, hipStream_t s);
 hipError_t fame_gemm_vq(const void* V, const double* Q, void* W, int m, int q, size_t n, size_t cstride,

@@ -56,7 +56,7 @@ template <> __device__ __forceinline__ float2 tw_expipi<float2>(double x) {
  * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
  *              (FAME_Matrix_Lambdas.cu:217-243)
  * A block owns L lines (b, c) and their three components: 3L LDS lines.      */
-template <typename T, int EXPAND, int ENG>
+template <typename T, int EXPAND, int ENG, bool GUARD>
 __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y;
@@ -88,6 +88,18 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         }
     }
     for (int a = threadIdx.x; a < na; a += FFT_THREADS) ea[a] = expipi((double)a / (double)na);
+    __syncthreads();
+    /* inner twiddle exp(2 i pi phi a / na) = th[line][a >> 4] * tl[line][a & 15]:
+     * two table reads and one complex product per element instead of a sincos */
+    const int nh = (na + 15) >> 4, tpl = 16 + nh;
+    T2* twt = reinterpret_cast<T2*>(ea + na);
+    for (int t = threadIdx.x; t < L * tpl; t += FFT_THREADS) {
+        const int line = t / tpl, q = t - line * tpl;
+        if (l0 + line < nlines) {
+            const double ph = 2.0 * lsym[line].phi / (double)na;
+            twt[t] = tw_expipi<T2>(ph * (double)(q < 16 ? q : 16 * (q - 16)));
+        }
+    }
     __syncthreads();
     /* per-problem base pointers (uniform) with 32-bit element offsets below
      * (3N < 2^31 is checked at context creation) */
@@ -191,7 +203,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                     if (!(pr.gamma && f == 0)) {
                         double2 lx, ly, lz, p1[3], p2[3];
                         lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                        pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
+                        pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
                         const double2 x1 = to_d(xv1), x2 = to_d(xv2);
 #pragma unroll
                         for (int c = 0; c < 3; ++c) {
@@ -224,7 +236,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             const int line = udiv(idx, g.m_na), a = idx - line * na;
             const int gl = l0 + line;
             if (gl >= nlines) continue;
-            const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
+            const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
             const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
             for (int c = 0; c < 3; ++c)
@@ -251,7 +263,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 const int line = udiv(idx, mW), a = idx - line * Wd;
                 const int gl = l0 + line;
                 if (gl < nlines && a < na) {
-                    const T2 w = tw_expipi<T2>(2.0 * lsym[line].phi * (double)a * inv_na);
+                    const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) d.v[c] = cmulc(d.v[c], w);
                 }
@@ -300,7 +312,7 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
                 if (!(pr.gamma && f == 0)) {
                     double2 lx, ly, lz, p1[3], p2[3];
                     lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                    pi_basis<sizeof(T) == 4>(g, lx, ly, lz, p1, p2);
+                    pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
@@ -619,9 +631,14 @@ template <typename T, int ENG>
 static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream_t s) {
     const int nlines = a.g.nb * a.g.nc;
     dim3 grid(xcd_round((nlines + a.L - 1) / a.L), nprob);
-    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, inner_tab_extra(a.g.na, (int)sizeof(T)));
-    if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
-    else hipLaunchKernelGGL((k_inner<T, 0, ENG>), grid, dim3(FFT_THREADS), lds, s, a);
+    const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, inner_lds_extra(a.g.na, (int)sizeof(T), a.L));
+    if (a.g.pi_guard) {
+        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, true>), grid, dim3(FFT_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_inner<T, 0, ENG, true>), grid, dim3(FFT_THREADS), lds, s, a);
+    } else {
+        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, false>), grid, dim3(FFT_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_inner<T, 0, ENG, false>), grid, dim3(FFT_THREADS), lds, s, a);
+    }
     return hipGetLastError();
 }
 

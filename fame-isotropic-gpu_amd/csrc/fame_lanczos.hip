@@ -15,7 +15,8 @@
  *    the normalisations, and applied to the basis as one U*Q product instead
  *    of sum(nstep-j) column rotations;
  *  - the tridiagonal eigensolver (LAPACKE_zpteqr in the reference) is an own
- *    implicit-QL routine (fame_tridiag.cpp).
+ *    cyclic-Jacobi routine, the shifted QR step an own banded explicit step
+ *    (fame_tridiag.cpp).
  */
 #include "fame_hip.h"
 #include "fame_host.h"
@@ -40,8 +41,6 @@ std::string& fame_err_ref();
     return FAME_ERR_HIP; } } while (0)
 #define CHK(x) do { fame_status s_ = (x); if (s_ != FAME_OK) return s_; } while (0)
 
-int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z);
-void fame_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s, double shift, int n);
 
 /* partial-sum blocks per problem: 256 for single-column products (the
  * three-term step, norms), 1024 for the Gram-Schmidt products over many
@@ -70,12 +69,14 @@ struct LzBuffers {
     double* Q = nullptr;    /* nk x m x (nwant+1) */
     int* mask = nullptr;    /* nk */
     int* mask2 = nullptr;   /* nk: problems that take the second Gram-Schmidt pass */
+    std::vector<double> ls_time;   /* nk: seconds inside invAr, each problem's share */
 };
 
 fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
     for (void* p : c->big) if (p) (void)hipFree(p);
     c->big.clear();
     b.n2 = 2 * (size_t)c->N; b.nk = nk; b.nstep = nstep; b.nwant = nwant;
+    b.ls_time.assign(nk, 0.0);
     const size_t m = nstep + 1;
     auto al = [&](void** p, size_t bytes) -> fame_status {
         hipError_t e = hipMalloc(p, bytes ? bytes : 16);
@@ -114,9 +115,15 @@ static fame_status lz_invar(fame_ctx* c, LzBuffers& b, int j, int mode, const fa
     std::vector<int> it(b.nk, 0);
     /* read straight from column j; the result goes to column j + 1 of the
      * problems still iterating only: a converged problem keeps its basis,
-     * whose columns its final Ritz vectors are built from */
+     * whose columns its final Ritz vectors are built from.  The solve ends
+     * with a host read of the CG state, so its host time is its wall time. */
+    const auto t0 = std::chrono::steady_clock::now();
     CHK(c->invar(mode, col(b, j), ysolve, 0, b.nk, mask_dev, ls, it.data(), ps, col(b, j + 1), ps, mask_h.data()));
-    for (int p = 0; p < b.nk; ++p) if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; }
+    const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    int active = 0;
+    for (int p = 0; p < b.nk; ++p) active += mask_h[p] ? 1 : 0;
+    for (int p = 0; p < b.nk; ++p)
+        if (mask_h[p]) { ls_iter[p] += it[p]; es_iter[p] += 1; b.ls_time[p] += dt / active; }
     return FAME_OK;
 }
 
@@ -186,6 +193,60 @@ static fame_status lz_decomp(fame_ctx* c, LzBuffers& b, bool init, int mode, con
 extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                                       void* evr_dev, void* E_dev, fame_profile* prof);
 
+/* Inverse iteration on the converged Ritz pairs, in lambda space: for each
+ * band y (unit norm, all problems of the batch at once)
+ *   w = A_r^{-1} y,  lambda = Re <w, y> / <w, w>,  y <- w / ||w||.
+ * Since A_r w = y, <w, y> = w^H A_r w, so lambda is w's Rayleigh quotient and
+ * needs no extra matrix apply.  A component of y along an eigenvector of
+ * eigenvalue lambda_i shrinks by lambda / lambda_i, so the high-frequency
+ * components that the theta-space stopping rule leaves (they dominate the
+ * field residual) are removed first; inside a cluster the ratio is ~1, so the
+ * bands keep their order and identity over a few sweeps.  The solves use an
+ * absolute CG tolerance of ritz_tol (well below es.tol, since the solve's
+ * error enters the new vector's lambda-space residual amplified by |Lambda|).
+ * Column 0 of each problem's Lanczos basis is free at this point and holds w. */
+static fame_status refine_ritz(fame_ctx* c, LzBuffers& b, int precision, double* freq, std::vector<int>& ls_iter,
+                               std::vector<int>& es_iter) {
+    const int nk = b.nk, nwant = b.nwant;
+    const size_t n2 = b.n2, ps = n2 * (b.nstep + 1), wps = n2 * (nwant + 1);
+    const fame_ls ls = {c->ritz_maxit, c->ritz_tol};
+    double2* ysolve = (double2*)c->ref64[0];
+    std::vector<int> it(nk, 0), all(nk, 1);
+    std::vector<double> hv(2 * (size_t)nk), nv(nk), sc(nk);
+    for (int sweep = 0; sweep < c->ritz_sweeps; ++sweep) {
+        for (int i = 0; i < nwant; ++i) {
+            double2* y = b.W + (size_t)i * n2;
+            std::fill(it.begin(), it.end(), 0);
+            const auto t0 = std::chrono::steady_clock::now();
+            CHK(c->invar(precision, y, ysolve, 0, nk, nullptr, ls, it.data(), wps, b.U, ps, all.data()));
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            for (int p = 0; p < nk; ++p) { ls_iter[p] += it[p]; es_iter[p] += 1; b.ls_time[p] += dt / nk; }
+            HIPCHK(fame_gemv_h(b.U, y, b.gpart, GEMV_BLK, 1, n2, n2, ps, wps, nk, nullptr, c->stream));
+            HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 2, b.hdot, nk, c->stream));
+            HIPCHK(fame_norm2_partials(b.U, 8, b.gpart, GEMV_BLK, n2, ps, nk, c->stream));
+            HIPCHK(fame_reduce_partials(b.gpart, GEMV_BLK, 1, b.nrm, nk, c->stream));
+            HIPCHK(hipMemcpyAsync(hv.data(), b.hdot, sizeof(double) * 2 * nk, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipMemcpyAsync(nv.data(), b.nrm, sizeof(double) * nk, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            for (int p = 0; p < nk; ++p) {
+                if (!(nv[p] > 0)) FAIL(FAME_ERR_STATE, "fame_eigs: Ritz refinement produced a zero vector");
+                freq[(size_t)p * nwant + i] = std::sqrt(std::max(hv[2 * p] / nv[p], 0.0));
+                sc[p] = 1.0 / std::sqrt(nv[p]);
+            }
+            HIPCHK(hipMemcpyAsync(b.nrm, sc.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(fame_scale_real_mod2(y, wps, b.U, ps, b.nrm, n2, 1, 1, nk, 8, c->stream));
+        }
+    }
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_set_refinement(fame_ctx* c, int sweeps, double tol, int maxit) {
+    if (!c || sweeps < 0 || sweeps > 16 || !(tol > 0) || maxit < 1)
+        FAIL(FAME_ERR_ARG, "fame_set_refinement: need 0 <= sweeps <= 16, tol > 0, maxit >= 1");
+    c->ritz_sweeps = sweeps; c->ritz_tol = tol; c->ritz_maxit = maxit;
+    return FAME_OK;
+}
+
 extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                                  void* evr_dev, void* E_dev, fame_profile* prof) {
     return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
@@ -238,8 +299,9 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
                                       void* evr_dev, void* E_dev, fame_profile* prof) {
     if (!c || !es || !ls || !freq) FAIL(FAME_ERR_ARG, "fame_eigs: null argument");
     if (!c->have_eps || c->nk < 1) FAIL(FAME_ERR_STATE, "fame_eigs: permittivity and wave vectors must be set first");
-    if (es->nwant < 1 || es->nstep <= es->nwant || es->nstep > 62 || es->maxit < 1 || ls->maxit < 1)
-        FAIL(FAME_ERR_ARG, "fame_eigs: need 1 <= nwant < nstep <= 62 and positive maxit");
+    if (es->nwant < 1 || es->nstep <= es->nwant || es->nstep + 1 > FAME_LZ_MAXCOL || es->maxit < 1 || ls->maxit < 1)
+        FAIL(FAME_ERR_ARG, "fame_eigs: need 1 <= nwant < nstep <= " + std::to_string(FAME_LZ_MAXCOL - 1) +
+                               " and positive maxit");
     if (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE) FAIL(FAME_ERR_ARG, "fame_eigs: unknown precision");
     HIPCHK(hipSetDevice(c->device));
     /* first solve of a context: tune the pass kernels on this batch shape */
@@ -316,7 +378,7 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
                 return std::sqrt(std::max(s, 0.0));
             };
             for (int j = 0; j < nstep - nwant; ++j) {
-                fame_gvqrrq(t0, t1, T2.data(), T3.data(), cc.data(), ss.data(), lt0[nwant + j], nstep - j);
+                fame_tridiag_qr_step(t0, t1, T2.data(), T3.data(), cc.data(), ss.data(), lt0[nwant + j], nstep - j);
                 int jj;
                 for (jj = 0; jj < nstep - 1 - j; ++jj) rot(jj + 1, jj, cc[jj], ss[jj]);
                 ss[jj - 1] *= t1[jj];
@@ -351,6 +413,7 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
     const size_t ps = b.n2 * m, wps = b.n2 * (nwant + 1);
     HIPCHK(fame_gemm_vq(b.U, b.Q, b.W, nstep, nwant, b.n2, b.n2, ps, b.n2, wps, (size_t)nstep * nwant, nk, nullptr,
                         c->stream));
+    if (c->ritz_sweeps > 0) CHK(refine_ritz(c, b, precision, freq, ls_iter, es_iter));
     if (evr_dev) {
         /* reduced eigenvectors, reference layout (2Nd) with a 2N stride per band */
         for (int p = 0; p < nk; ++p)
@@ -373,7 +436,7 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
         for (int p = 0; p < nk; ++p) {
             fame_profile& q = prof[p];
             q.ls_iter = ls_iter[p]; q.es_iter = es_iter[p]; q.restarts = restarts[p]; q.nd = c->nd[p];
-            q.converged = conv[p]; q.pad = 0; q.es_time = secs;
+            q.converged = conv[p]; q.pad = 0; q.es_time = secs; q.ls_time = b.ls_time[p];
             q.qbq64 = (int)((c->cnt_qbq64 - q64_0) / nk);
             q.qbq32 = (int)((c->cnt_qbq32 - q32_0) / nk);
         }

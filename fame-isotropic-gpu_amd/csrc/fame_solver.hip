@@ -253,8 +253,8 @@ fame_status AxisPlan::build(int n_, int engine) {
 
 /* LDS entries of one block beyond the lines: the middle-axis kernels keep one
  * twiddle per line position, the inner-axis kernels a phase per mode */
-static int tile_extra(int axis, const GridDev& g, int prec) {
-    return axis == AX_MID ? g.nb : (axis == AX_INNER ? inner_tab_extra(g.na, prec) : 0);
+static int tile_extra(int axis, const GridDev& g, int prec, int L) {
+    return axis == AX_MID ? g.nb : (axis == AX_INNER ? inner_lds_extra(g.na, prec, L) : 0);
 }
 static int tile_ncomp(int axis) { return axis == AX_INNER ? 3 : 1; }
 
@@ -265,7 +265,7 @@ static bool tile_ok(const AxisPlan& p, int axis, int prec, int L, const GridDev&
     const int pi = prec == 8 ? 0 : 1, nc = tile_ncomp(axis);
     if (L < 1 || (axis == AX_INNER && L > MAX_LINES_INNER)) return false;
     if (!p.mfma && (long long)nc * L * p.M > p.cap[pi]) return false;
-    return fame_line_lds(p.lists[pi][0], prec, nc * L, p.M | 1, tile_extra(axis, g, prec)) <= 60 * 1024;
+    return fame_line_lds(p.lists[pi][0], prec, nc * L, p.M | 1, tile_extra(axis, g, prec, L)) <= 60 * 1024;
 }
 
 /* default lines per block before tuning: as many as feasible up to maxL, rounded
@@ -319,6 +319,11 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
     const int Nx = lat->grid_nums[0], Ny = lat->grid_nums[1], Nz = lat->grid_nums[2];
     GridDev& g = c->g;
     g.general = lat->general ? 1 : 0;
+    {
+        const double* h = lat->mesh_len;
+        auto close = [](double u, double v) { return std::fabs(u - v) <= 1e-12 * std::fabs(u); };
+        g.pi_guard = g.general && close(h[0], h[1]) && close(h[1], h[2]) ? 1 : 0;
+    }
     if (g.general) { g.na = Nz; g.nb = Ny; g.nc = Nx; g.ha = lat->mesh_len[2]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[0]; }
     else { g.na = Nx; g.nb = Ny; g.nc = Nz; g.ha = lat->mesh_len[0]; g.hb = lat->mesh_len[1]; g.hc = lat->mesh_len[2]; }
     if ((long long)Nx * Ny * Nz * 3 >= (1LL << 31)) { delete c; FAIL(FAME_ERR_ARG, "grid too large: 3N must stay below 2^31"); }
@@ -437,10 +442,13 @@ extern "C" fame_status fame_set_eps(fame_ctx* c, const double* eps) {
     /* the context's stream may still be running kernels that read the old values */
     HIPCHK(hipSetDevice(c->device));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(c->eps_int, e.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->eps_ref, eps, sizeof(double) * 3 * N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->invB64, ib.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->invB32, ib32.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+    /* ordered on the context's stream (contexts of fame_eigs_many run on their
+     * own streams); the host staging vectors live until the stream is drained */
+    HIPCHK(hipMemcpyAsync(c->eps_int, e.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->eps_ref, eps, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->invB64, ib.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->invB32, ib32.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->have_eps = true;
     return FAME_OK;
 }
@@ -487,7 +495,7 @@ extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* 
         if (nd_out) nd_out[i] = c->nd[i];
     }
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(c->prob_d, c->prob_h.data(), sizeof(ProbDev) * nk, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(c->prob_d, c->prob_h.data(), sizeof(ProbDev) * nk, hipMemcpyHostToDevice, c->stream));
     HIPCHK(fame_launch_sigma(c->g, c->prob_d, c->sig, c->sig_inv, nk, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return FAME_OK;

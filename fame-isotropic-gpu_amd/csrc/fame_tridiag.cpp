@@ -1,22 +1,30 @@
 /*
- * fame_tridiag.cpp -- small dense pieces of the restart, on the host.
+ * fame_tridiag.cpp -- small dense pieces of the Lanczos restart, on the host.
  *
  * fame_tridiag_eig replaces LAPACKE_zpteqr (Lanczos_Isotropic.cu:53): all
  * eigenpairs of the symmetric tridiagonal T, eigenvalues in DESCENDING order,
  * eigenvectors as columns of z (n x n, column-major).  Method: cyclic Jacobi
- * rotations on the dense n x n matrix (n <= 63 here), which is unconditionally
+ * rotations on the dense n x n matrix (n <= 128 here), which is unconditionally
  * convergent and accurate to a few ulps of ||T||.
  *
- * fame_gvqrrq restates GVqrrq_g / given_rotation (Lanczos_LockPurge_gpu.cu:95-266):
- * one exact-shift QR step on T with the reference's Givens sign convention and
- * the (T1 + T3)/2 symmetrisation.
+ * fame_tridiag_qr_step is the exact-shift restart step of
+ * Lanczos_LockPurge_gpu.cu:7-48 (one explicit shifted QR step T - mu I = QR,
+ * T' = RQ + mu I), written from the textbook step (Golub & Van Loan, Matrix
+ * Computations, 4th ed., Sec. 8.3.3, explicit form).  The behaviour the restart
+ * depends on is the reference's: the shift is exact (an unwanted Ritz value),
+ * the rotation that annihilates T(i+1,i) has c = a/r, s = -b/r with
+ * r = hypot(a, b) >= 0 (so R has a non-negative diagonal), and the
+ * off-diagonal of T' is the mean of its computed sub- and super-diagonal.
  */
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
 
-int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z) {
+#include "fame_hip.h"
+
+extern "C" int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z) {
+    if (n <= 0 || !d || !w || !z || (n > 1 && !e)) return FAME_ERR_ARG;
     std::vector<double> A((size_t)n * n, 0.0), V((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i) {
         A[(size_t)i * n + i] = d[i];
@@ -64,47 +72,63 @@ int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double*
         w[j] = A[(size_t)idx[j] * n + idx[j]];
         for (int i = 0; i < n; ++i) z[(size_t)j * n + i] = V[(size_t)i * n + idx[j]];
     }
-    return 0;
+    return FAME_OK;
 }
 
-static void given_rotation(double* c, double* s, double* r, double a, double b) {
-    double t, u;
-    if (std::fabs(a) > std::fabs(b)) {
-        t = b / a; u = std::fabs(a) / a * std::sqrt(1 + t * t);
-        *c = 1.0 / u; *s = *c * t; *r = a * u; *s = -*s;
-    } else {
-        t = a / b; u = std::fabs(b) / b * std::sqrt(1 + t * t);
-        *s = 1.0 / u; *c = *s * t; *r = b * u; *s = -*s;
+/*
+ * One shifted QR step on the symmetric tridiagonal (diag[0..n-1], off[0..n-2]).
+ *
+ * QR sweep.  G_i acts on rows (i, i+1) as [[c, -s], [s, c]] and zeroes the
+ * sub-diagonal entry b = (i+1, i) against the current pivot a = (i, i).  R is
+ * upper triangular with three bands, kept as rd (diagonal, in diag), ru (first
+ * super-diagonal) and rw (second super-diagonal).
+ *
+ * RQ sweep.  R G_0^T ... G_{n-2}^T: G_i^T mixes columns (i, i+1).  Before it is
+ * applied, column i holds (hi, hd) at rows (i-1, i) from the previous rotation,
+ * column i+1 holds R's (rw[i-1], ru[i], rd[i+1]).  After it, column i is final:
+ *   T'(i-1, i) = c hi - s rw[i-1],  T'(i, i) = c hd - s ru[i],  T'(i+1, i) = -s rd[i+1],
+ * and column i+1 carries s hd + c ru[i] at row i and c rd[i+1] at row i+1 into
+ * the next rotation (its entry at row i-1 lies outside the band of the symmetric
+ * result and is not formed).
+ *
+ * ru and rw are caller scratch of n doubles; c and s receive the n-1 rotations.
+ */
+extern "C" int fame_tridiag_qr_step(double* diag, double* off, double* ru, double* rw, double* c, double* s,
+                                    double mu, int n) {
+    if (n < 1 || !diag || (n > 1 && (!off || !ru || !rw || !c || !s))) return FAME_ERR_ARG;
+    if (n == 1) return FAME_OK;
+    for (int i = 0; i < n; ++i) diag[i] -= mu;
+    for (int i = 0; i + 1 < n; ++i) ru[i] = off[i];
+    rw[n - 1] = rw[n - 2] = 0.0;
+    for (int i = 0; i + 1 < n; ++i) {
+        const double a = diag[i], b = off[i];
+        const double r = std::hypot(a, b);
+        const double ci = r > 0.0 ? a / r : 1.0, si = r > 0.0 ? -b / r : 0.0;
+        c[i] = ci; s[i] = si;
+        diag[i] = r;
+        const double up = ru[i], dn = diag[i + 1];
+        ru[i] = ci * up - si * dn;
+        diag[i + 1] = si * up + ci * dn;
+        if (i + 2 < n) {
+            const double u1 = ru[i + 1];
+            rw[i] = -si * u1;
+            ru[i + 1] = ci * u1;
+        }
     }
-}
-
-void fame_gvqrrq(double* T0, double* T1, double* T2, double* T3, double* c, double* s, double shift, int n) {
-    double a, b, r, R[6];
-    std::memcpy(T3, T1, (n - 1) * sizeof(double));
-    for (int i = 0; i < n; ++i) T0[i] -= shift;
-    for (int i = 0; i < n - 1; ++i) {            /* Q' T */
-        a = T0[i]; b = T1[i];
-        given_rotation(&c[i], &s[i], &r, a, b);
-        T0[i] = r; T1[i] = 0.0;
-        a = T3[i]; b = T0[i + 1];
-        T3[i] = c[i] * a - s[i] * b;
-        T0[i + 1] = s[i] * a + c[i] * b;
-        if (i < n - 2) { T2[i] = -s[i] * T3[i + 1]; T3[i + 1] = c[i] * T3[i + 1]; }
+    /* off now receives the sub-diagonal of RQ, ru its super-diagonal */
+    double hi = 0.0, hd = diag[0];
+    for (int i = 0; i + 1 < n; ++i) {
+        const double ci = c[i], si = s[i];
+        const double rdn = diag[i + 1], rui = ru[i];
+        if (i > 0) ru[i - 1] = ci * hi - si * rw[i - 1];
+        diag[i] = ci * hd - si * rui;
+        off[i] = -si * rdn;
+        hi = si * hd + ci * rui;
+        hd = ci * rdn;
     }
-    R[0] = T0[0]; R[1] = T3[0]; R[2] = T1[0]; R[3] = T0[1];    /* T Q */
-    T0[0] = c[0] * R[0] - s[0] * R[1];
-    T3[0] = s[0] * R[0] + c[0] * R[1];
-    T1[0] = c[0] * R[2] - s[0] * R[3];
-    T0[1] = s[0] * R[2] + c[0] * R[3];
-    for (int i = 1; i < n - 1; ++i) {
-        R[0] = T3[i - 1]; R[1] = T2[i - 1]; R[2] = T0[i]; R[3] = T3[i]; R[4] = T1[i]; R[5] = T0[i + 1];
-        T3[i - 1] = c[i] * R[0] - s[i] * R[1];
-        T2[i - 1] = s[i] * R[0] + c[i] * R[1];
-        T0[i] = c[i] * R[2] - s[i] * R[3];
-        T3[i] = s[i] * R[2] + c[i] * R[3];
-        T1[i] = c[i] * R[4] - s[i] * R[5];
-        T0[i + 1] = s[i] * R[4] + c[i] * R[5];
-    }
-    for (int i = 0; i < n - 1; ++i) T1[i] = (T1[i] + T3[i]) / 2.0;
-    for (int i = 0; i < n; ++i) T0[i] += shift;
+    diag[n - 1] = hd;
+    ru[n - 2] = hi;
+    for (int i = 0; i + 1 < n; ++i) off[i] = 0.5 * (off[i] + ru[i]);
+    for (int i = 0; i < n; ++i) diag[i] += mu;
+    return FAME_OK;
 }

@@ -393,6 +393,228 @@ extern "C" int fame_material_eps(int N, int material_num, const int* idx3N, cons
 /* ---------------------------------------------------------------------------- */
 /*  Brillouin zone  (FAME_Parameter_Brillouin_Zone_Point.cpp / _Path.cpp)        */
 /* ---------------------------------------------------------------------------- */
+namespace {
+
+/* High-symmetry points of the Brillouin zones of the 14 Bravais lattices, in
+ * units of the reciprocal basis vectors, after Setyawan & Curtarolo, Comput.
+ * Mater. Sci. 49 (2010) 299-312, as the reference tabulates them
+ * (FAME_Parameter_Brillouin_Zone_Point.cpp:12-345, including its variant
+ * conditions).  Each coordinate is a constant plus or minus at most one of the
+ * variant's parameters, written as a short expression ("0.5", "eta",
+ * "1-eta", "0.5+zeta", "-mu"); evaluated as c0 + s * p, one rounding, which is
+ * the reference's arithmetic.                                                */
+struct BzParams { double eta, zeta, delta, phi, mu, nu, psi, rho, omega; };
+struct BzPoint { char name; const char* x; const char* y; const char* z; };
+struct BzSet { const BzPoint* pts; int n; };
+
+double bz_param(const BzParams& P, const char* nm, int len) {
+    auto eq = [&](const char* w) { return (int)std::strlen(w) == len && std::strncmp(nm, w, len) == 0; };
+    if (eq("eta")) return P.eta;
+    if (eq("zeta")) return P.zeta;
+    if (eq("delta")) return P.delta;
+    if (eq("phi")) return P.phi;
+    if (eq("mu")) return P.mu;
+    if (eq("nu")) return P.nu;
+    if (eq("psi")) return P.psi;
+    if (eq("rho")) return P.rho;
+    if (eq("omega")) return P.omega;
+    return NAN;
+}
+
+/* "[c0][+|-][param]" */
+double bz_coord(const BzParams& P, const char* e) {
+    const char* q = e;
+    double c0 = 0.0;
+    bool have_c = false;
+    if (*q == '-' && (q[1] >= '0' && q[1] <= '9')) { c0 = std::strtod(q, (char**)&q); have_c = true; }
+    else if (*q >= '0' && *q <= '9') { c0 = std::strtod(q, (char**)&q); have_c = true; }
+    if (!*q) return c0;
+    double sg = 1.0;
+    if (*q == '+') ++q;
+    else if (*q == '-') { sg = -1.0; ++q; }
+    const double p = bz_param(P, q, (int)std::strlen(q));
+    if (!have_c) return sg * p;
+    return sg > 0 ? c0 + p : c0 - p;
+}
+
+#define BZN(a) (int)(sizeof(a) / sizeof(a[0]))
+const BzPoint BZ_CUB[] = {{'G', "0", "0", "0"}, {'M', "0.5", "0.5", "0"}, {'R', "0.5", "0.5", "0.5"}, {'X', "0", "0.5", "0"}};
+const BzPoint BZ_FCC[] = {{'G', "0", "0", "0"}, {'K', "0.375", "0.375", "0.75"}, {'L', "0.5", "0.5", "0.5"},
+                          {'U', "0.625", "0.25", "0.625"}, {'W', "0.5", "0.25", "0.75"}, {'X', "0.5", "0", "0.5"}};
+const BzPoint BZ_BCC[] = {{'G', "0", "0", "0"}, {'N', "0", "0", "0.5"}, {'H', "0.5", "-0.5", "0.5"},
+                          {'P', "0.25", "0.25", "0.25"}};
+const BzPoint BZ_TET[] = {{'G', "0", "0", "0"}, {'A', "0.5", "0.5", "0.5"}, {'M', "0.5", "0.5", "0"},
+                          {'R', "0", "0.5", "0.5"}, {'X', "0", "0.5", "0"}, {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_BCT1[] = {{'G', "0", "0", "0"}, {'X', "0", "0", "0.5"}, {'M', "-0.5", "0.5", "0.5"},
+                           {'N', "0", "0.5", "0"}, {'Z', "eta", "eta", "-eta"}, {'B', "-eta", "1-eta", "eta"},
+                           {'P', "0.25", "0.25", "0.25"}};
+const BzPoint BZ_BCT2[] = {{'G', "0", "0", "0"}, {'X', "0", "0", "0.5"}, {'Y', "-zeta", "zeta", "0.5"},
+                           {'A', "0.5", "0.5", "-zeta"}, {'s', "-eta", "eta", "eta"}, {'t', "eta", "1-eta", "-eta"},
+                           {'N', "0", "0.5", "0"}, {'P', "0.25", "0.25", "0.25"}, {'Z', "0.5", "0.5", "-0.5"}};
+const BzPoint BZ_ORC[] = {{'G', "0", "0", "0"}, {'R', "0.5", "0.5", "0.5"}, {'S', "0.5", "0.5", "0"},
+                          {'T', "0", "0.5", "0.5"}, {'U', "0.5", "0", "0.5"}, {'X', "0.5", "0", "0"},
+                          {'Y', "0", "0.5", "0"}, {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_ORCF1[] = {{'G', "0", "0", "0"}, {'L', "0.5", "0.5", "0.5"}, {'A', "0.5", "0.5+zeta", "zeta"},
+                            {'B', "0.5", "0.5-zeta", "1-zeta"}, {'T', "1", "0.5", "0.5"}, {'Y', "0.5", "0", "0.5"},
+                            {'X', "0", "eta", "eta"}, {'J', "1", "1-eta", "1-eta"}, {'Z', "0.5", "0.5", "0"}};
+const BzPoint BZ_ORCF2[] = {{'G', "0", "0", "0"}, {'L', "0.5", "0.5", "0.5"}, {'C', "0.5", "0.5-eta", "1-eta"},
+                            {'E', "0.5", "0.5+eta", "eta"}, {'D', "0.5-delta", "0.5", "1-delta"},
+                            {'F', "0.5+delta", "0.5", "delta"}, {'H', "1-phi", "0.5-phi", "0.5"},
+                            {'I', "phi", "0.5+phi", "0.5"}, {'X', "0", "0.5", "0.5"}, {'Y', "0.5", "0", "0.5"},
+                            {'Z', "0.5", "0.5", "0"}};
+const BzPoint BZ_ORCI[] = {{'G', "0", "0", "0"}, {'L', "-mu", "mu", "0.5-delta"}, {'I', "mu", "-mu", "0.5+delta"},
+                           {'J', "0.5-delta", "0.5+delta", "-mu"}, {'R', "0", "0.5", "0"}, {'S', "0.5", "0", "0"},
+                           {'T', "0", "0", "0.5"}, {'W', "0.25", "0.25", "0.25"}, {'X', "-zeta", "zeta", "zeta"},
+                           {'A', "zeta", "1-zeta", "-zeta"}, {'Y', "eta", "-eta", "eta"}, {'B', "1-eta", "eta", "-eta"},
+                           {'Z', "0.5", "0.5", "-0.5"}};
+const BzPoint BZ_ORCC[] = {{'G', "0", "0", "0"}, {'R', "0", "0.5", "0.5"}, {'S', "0", "0.5", "0"},
+                           {'T', "-0.5", "0.5", "0.5"}, {'A', "zeta", "zeta", "0.5"}, {'B', "-zeta", "1-zeta", "0.5"},
+                           {'X', "zeta", "zeta", "0"}, {'C', "-zeta", "1-zeta", "0"}, {'Y', "-0.5", "0.5", "0"},
+                           {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_RHL1[] = {{'G', "0", "0", "0"}, {'F', "0.5", "0.5", "0"}, {'B', "eta", "0.5", "1-eta"},
+                           {'C', "0.5", "1-eta", "-1+eta"}, {'L', "0.5", "0", "0"}, {'I', "0", "0", "-0.5"},
+                           {'P', "eta", "nu", "nu"}, {'R', "1-nu", "1-nu", "1-eta"}, {'S', "nu", "nu", "-1+eta"},
+                           {'Q', "1-nu", "nu", "0"}, {'X', "nu", "0", "-nu"}, {'Z', "0.5", "0.5", "0.5"}};
+const BzPoint BZ_RHL2[] = {{'G', "0", "0", "0"}, {'F', "0.5", "-0.5", "0"}, {'L', "0.5", "0", "0"},
+                           {'Z', "0.5", "-0.5", "0.5"}, {'P', "1-nu", "-nu", "1-nu"}, {'R', "nu", "-1+nu", "-1+nu"},
+                           {'Q', "eta", "eta", "eta"}, {'S', "1-eta", "-eta", "-eta"}};
+const BzPoint BZ_MCL[] = {{'G', "0", "0", "0"}, {'A', "0.5", "0.5", "0"}, {'C', "0", "0.5", "0.5"},
+                          {'D', "0.5", "0", "0.5"}, {'F', "0.5", "0", "-0.5"}, {'E', "0.5", "0.5", "0.5"},
+                          {'H', "0", "eta", "1-nu"}, {'I', "0", "1-eta", "nu"}, {'J', "0", "eta", "-nu"},
+                          {'M', "0.5", "eta", "1-nu"}, {'N', "0.5", "1-eta", "nu"}, {'O', "0.5", "eta", "-nu"},
+                          {'X', "0", "0.5", "0"}, {'Y', "0", "0", "0.5"}, {'B', "0", "0", "-0.5"}, {'Z', "0.5", "0", "0"}};
+const BzPoint BZ_MCLC1[] = {{'G', "0", "0", "0"}, {'N', "0.5", "0", "0"}, {'S', "0", "-0.5", "0"},
+                            {'F', "1-zeta", "1-zeta", "1-eta"}, {'J', "zeta", "zeta", "eta"},
+                            {'K', "-zeta", "-zeta", "1-eta"}, {'O', "1-zeta", "-zeta", "1-eta"},
+                            {'I', "phi", "1-phi", "0.5"}, {'R', "1-phi", "-1+phi", "0.5"}, {'L', "0.5", "0.5", "0.5"},
+                            {'M', "0.5", "0", "0.5"}, {'X', "1-psi", "-1+psi", "0"}, {'T', "psi", "1-psi", "0"},
+                            {'U', "-1+psi", "-psi", "0"}, {'Y', "0.5", "0.5", "0"}, {'V', "-0.5", "-0.5", "0"},
+                            {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_MCLC3[] = {{'G', "0", "0", "0"}, {'F', "1-phi", "1-phi", "1-psi"}, {'J', "phi", "-1+phi", "psi"},
+                            {'K', "1-phi", "-phi", "1-psi"}, {'H', "zeta", "zeta", "eta"},
+                            {'P', "1-zeta", "-zeta", "1-eta"}, {'Q', "-zeta", "-zeta", "1-eta"},
+                            {'I', "0.5", "-0.5", "0.5"}, {'M', "0.5", "0", "0.5"}, {'N', "0.5", "0", "0"},
+                            {'S', "0", "-0.5", "0"}, {'X', "0.5", "-0.5", "0"}, {'Y', "mu", "mu", "delta"},
+                            {'V', "1-mu", "-mu", "-delta"}, {'W', "-mu", "-mu", "-delta"}, {'A', "mu", "-1+mu", "delta"},
+                            {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_MCLC5[] = {{'G', "0", "0", "0"}, {'F', "nu", "nu", "omega"}, {'J', "1-nu", "1-nu", "1-omega"},
+                            {'K', "nu", "-1+nu", "omega"}, {'H', "zeta", "zeta", "eta"},
+                            {'P', "1-zeta", "-zeta", "1-eta"}, {'Q', "-zeta", "-zeta", "1-eta"},
+                            {'I', "rho", "1-rho", "0.5"}, {'R', "1-rho", "-1+rho", "0.5"}, {'L', "0.5", "0.5", "0.5"},
+                            {'M', "0.5", "0", "0.5"}, {'N', "0.5", "0", "0"}, {'S', "0", "-0.5", "0"},
+                            {'X', "0.5", "-0.5", "0"}, {'Y', "mu", "mu", "delta"}, {'V', "1-mu", "-mu", "-delta"},
+                            {'W', "-mu", "-mu", "-delta"}, {'A', "mu", "-1+mu", "delta"}, {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_TRI1[] = {{'G', "0", "0", "0"}, {'L', "0.5", "0.5", "0"}, {'M', "0", "0.5", "0.5"},
+                           {'N', "0.5", "0", "0.5"}, {'R', "0.5", "0.5", "0.5"}, {'X', "0.5", "0", "0"},
+                           {'Y', "0", "0.5", "0"}, {'Z', "0", "0", "0.5"}};
+const BzPoint BZ_TRI2[] = {{'G', "0", "0", "0"}, {'L', "0.5", "-0.5", "0"}, {'M', "0", "0", "0.5"},
+                           {'N', "-0.5", "-0.5", "0.5"}, {'R', "0", "-0.5", "0.5"}, {'X', "0", "-0.5", "0"},
+                           {'Y', "0.5", "0", "0"}, {'Z', "-0.5", "0", "0.5"}};
+
+/* the point set and parameters of a lattice (variant conditions as in the
+ * reference); {nullptr, 0} where the reference defines none (a = c for BCT,
+ * theta_1 = pi/2 for the rhombohedral lattice, b2.b3 = 0 for triclinic) */
+BzSet bz_set(const fame_lattice_out* L, BzParams& P) {
+    std::memset(&P, 0, sizeof(P));
+    const char* t = L->lattice_type;
+    const double a = L->a, b = L->b, c = L->c, al = L->alpha;
+    const double* B = L->recip_b;
+    if (is(t, "simple_cubic")) return {BZ_CUB, BZN(BZ_CUB)};
+    if (is(t, "face_centered_cubic")) return {BZ_FCC, BZN(BZ_FCC)};
+    if (is(t, "body_centered_cubic")) return {BZ_BCC, BZN(BZ_BCC)};
+    if (is(t, "primitive_tetragonal")) return {BZ_TET, BZN(BZ_TET)};
+    if (is(t, "body_centered_tetragonal")) {
+        if (c < a) { P.eta = (1.0 + std::pow(c / a, 2)) / 4.0; return {BZ_BCT1, BZN(BZ_BCT1)}; }
+        if (c > a) {
+            P.eta = (1.0 + std::pow(a / c, 2)) / 4.0;
+            P.zeta = 0.5 * std::pow(a, 2) / std::pow(c, 2);
+            return {BZ_BCT2, BZN(BZ_BCT2)};
+        }
+        return {nullptr, 0};
+    }
+    if (is(t, "primitive_orthorhombic")) return {BZ_ORC, BZN(BZ_ORC)};
+    if (is(t, "face_centered_orthorhombic")) {
+        if ((1.0 / std::pow(a, 2)) >= (1.0 / std::pow(b, 2) + 1.0 / std::pow(c, 2))) {
+            P.eta = (1.0 + std::pow(a / b, 2) + std::pow(a / c, 2)) / 4.0;
+            P.zeta = (1.0 + std::pow(a / b, 2) - std::pow(a / c, 2)) / 4.0;
+            return {BZ_ORCF1, BZN(BZ_ORCF1)};
+        }
+        P.eta = (1.0 + std::pow(a / b, 2) - std::pow(a / c, 2)) / 4.0;
+        P.delta = (1.0 + std::pow(b / a, 2) - std::pow(b / c, 2)) / 4.0;
+        P.phi = (1.0 + std::pow(c / b, 2) - std::pow(c / a, 2)) / 4.0;
+        return {BZ_ORCF2, BZN(BZ_ORCF2)};
+    }
+    if (is(t, "body_centered_orthorhombic")) {
+        P.zeta = (1.0 + std::pow(a / c, 2)) / 4.0;
+        P.eta = (1.0 + std::pow(b / c, 2)) / 4.0;
+        P.delta = (std::pow(b, 2) - std::pow(a, 2)) / (4.0 * std::pow(c, 2));
+        P.mu = (std::pow(a, 2) + std::pow(b, 2)) / (4.0 * std::pow(c, 2));
+        return {BZ_ORCI, BZN(BZ_ORCI)};
+    }
+    if (is(t, "c_base_centered_orthorhombic")) {
+        P.zeta = (1.0 + std::pow(a / b, 2)) / 4.0;
+        return {BZ_ORCC, BZN(BZ_ORCC)};
+    }
+    if (is(t, "hexagonal")) return {nullptr, -1};      /* handled by the caller (k point) */
+    if (is(t, "rhombohedral")) {
+        const double th = L->theta[0];
+        if (th < PI / 2) {
+            P.eta = (1.0 + 4.0 * std::cos(th)) / (2.0 + 4.0 * std::cos(th));
+            P.nu = 3.0 / 4.0 - P.eta / 2.0;
+            return {BZ_RHL1, BZN(BZ_RHL1)};
+        }
+        if (th > PI / 2) {
+            P.eta = 1.0 / (2.0 * std::pow(std::tan(th / 2), 2));
+            P.nu = 3.0 / 4.0 - P.eta / 2.0;
+            return {BZ_RHL2, BZN(BZ_RHL2)};
+        }
+        return {nullptr, 0};
+    }
+    if (is(t, "primitive_monoclinic")) {
+        P.eta = (1.0 - b * (std::cos(al) / c)) / (2.0 * std::pow(std::sin(al), 2));
+        P.nu = 1.0 / 2.0 - P.eta * c * (std::cos(al) / b);
+        return {BZ_MCL, BZN(BZ_MCL)};
+    }
+    if (is(t, "base_centered_monoclinic")) {
+        const double b12 = B[0] * B[3] + B[1] * B[4] + B[2] * B[5];
+        if (b12 <= 0) {
+            P.zeta = (2.0 - b * std::cos(al) / c) / (4.0 * std::pow(std::sin(al), 2));
+            P.eta = 1.0 / 2.0 + 2.0 * P.zeta * c * std::cos(al) / b;
+            P.psi = 3.0 / 4.0 - std::pow(a, 2) / (4.0 * std::pow(b, 2) * std::pow(std::sin(al), 2));
+            P.phi = P.psi + (3.0 / 4.0 - P.psi) * b * std::cos(al) / c;
+            return {BZ_MCLC1, BZN(BZ_MCLC1)};
+        }
+        const double q = b * std::cos(al) / c + std::pow(b, 2) * std::pow(std::sin(al), 2) / std::pow(a, 2);
+        if (q <= 1.0) {
+            P.mu = (1.0 + std::pow(b / a, 2)) / 4.0;
+            P.delta = b * c * std::cos(al) / (2.0 * std::pow(a, 2));
+            P.zeta = P.mu - 1.0 / 4.0 + (1.0 - b * std::cos(al) / c) / (4.0 * std::pow(std::sin(al), 2));
+            P.eta = 1.0 / 2.0 + 2.0 * P.zeta * c * std::cos(al) / b;
+            P.phi = 1.0 + P.zeta - 2.0 * P.mu;
+            P.psi = P.eta - 2.0 * P.delta;
+            return {BZ_MCLC3, BZN(BZ_MCLC3)};
+        }
+        P.zeta = (std::pow(b / a, 2) + (1.0 - b * std::cos(al) / c) / std::pow(std::sin(al), 2)) / 4.0;
+        P.eta = 1.0 / 2.0 + 2.0 * P.zeta * c * std::cos(al) / b;
+        P.mu = P.eta / 2.0 + std::pow(b, 2) / (4.0 * std::pow(a, 2)) - b * c * std::cos(al) / (2.0 * std::pow(a, 2));
+        P.nu = 2.0 * P.mu - P.zeta;
+        P.omega = ((4.0 * P.nu - 1.0 - std::pow(b, 2) * std::pow(std::sin(al), 2) / std::pow(a, 2)) * c) /
+                  (2.0 * b * std::cos(al));
+        P.delta = P.zeta * c * std::cos(al) / b + P.omega / 2.0 - 1.0 / 4.0;
+        P.rho = 1.0 - P.zeta * std::pow(a / b, 2);
+        return {BZ_MCLC5, BZN(BZ_MCLC5)};
+    }
+    if (is(t, "triclinic")) {
+        const double b23 = B[3] * B[6] + B[4] * B[7] + B[5] * B[8];
+        if (b23 < 0) return {BZ_TRI1, BZN(BZ_TRI1)};
+        if (b23 > 0) return {BZ_TRI2, BZN(BZ_TRI2)};
+        return {nullptr, 0};
+    }
+    return {nullptr, 0};
+}
+
+}  // namespace
+
 extern "C" int fame_bz_vertex(const fame_lattice_out* lat, char name, double v[3]) {
     const double* B = lat->recip_b;
     auto MV = [&](double V1, double V2, double V3) {
@@ -401,52 +623,8 @@ extern "C" int fame_bz_vertex(const fame_lattice_out* lat, char name, double v[3
         v[2] = B[2] * V1 + B[5] * V2 + B[8] * V3;
         return 0;
     };
-    const char* t = lat->lattice_type;
-    /* FAME_Parameter_Brillouin_Zone_Point.cpp:22-53, 85-94, 163-172 */
-    if (is(t, "simple_cubic")) {
-        switch (name) {
-            case 'G': return MV(0.0, 0.0, 0.0);
-            case 'M': return MV(0.5, 0.5, 0.0);
-            case 'R': return MV(0.5, 0.5, 0.5);
-            case 'X': return MV(0.0, 0.5, 0.0);
-        }
-    } else if (is(t, "face_centered_cubic")) {
-        switch (name) {
-            case 'G': return MV(0.000, 0.000, 0.000);
-            case 'K': return MV(0.375, 0.375, 0.750);
-            case 'L': return MV(0.500, 0.500, 0.500);
-            case 'U': return MV(0.625, 0.250, 0.625);
-            case 'W': return MV(0.500, 0.250, 0.750);
-            case 'X': return MV(0.500, 0.000, 0.500);
-        }
-    } else if (is(t, "body_centered_cubic")) {
-        switch (name) {
-            case 'G': return MV(0.00, 0.00, 0.00);
-            case 'N': return MV(0.00, 0.00, 0.50);
-            case 'H': return MV(0.50, -0.50, 0.50);
-            case 'P': return MV(0.25, 0.25, 0.25);
-        }
-    } else if (is(t, "primitive_tetragonal")) {
-        switch (name) {
-            case 'G': return MV(0.0, 0.0, 0.0);
-            case 'A': return MV(0.5, 0.5, 0.5);
-            case 'M': return MV(0.5, 0.5, 0.0);
-            case 'R': return MV(0.0, 0.5, 0.5);
-            case 'X': return MV(0.0, 0.5, 0.0);
-            case 'Z': return MV(0.0, 0.0, 0.5);
-        }
-    } else if (is(t, "primitive_orthorhombic")) {
-        switch (name) {
-            case 'G': return MV(0.0, 0.0, 0.0);
-            case 'R': return MV(0.5, 0.5, 0.5);
-            case 'S': return MV(0.5, 0.5, 0.0);
-            case 'T': return MV(0.0, 0.5, 0.5);
-            case 'U': return MV(0.5, 0.0, 0.5);
-            case 'X': return MV(0.5, 0.0, 0.0);
-            case 'Y': return MV(0.0, 0.5, 0.0);
-            case 'Z': return MV(0.0, 0.0, 0.5);
-        }
-    } else if (is(t, "hexagonal")) {
+    if (is(lat->lattice_type, "hexagonal")) {
+        /* :163-172; the auxiliary point k has an irrational coordinate */
         switch (name) {
             case 'G': return MV(0.0, 0.0, 0.0);
             case 'A': return MV(0.0, 0.0, 0.5);
@@ -456,7 +634,13 @@ extern "C" int fame_bz_vertex(const fame_lattice_out* lat, char name, double v[3
             case 'M': return MV(0.5, 0.0, 0.0);
             case 'k': return MV((0.5 * std::sqrt(2) - 1 / 3.), -1 / 3., 0.0);
         }
+        return -1;
     }
+    BzParams P;
+    const BzSet set = bz_set(lat, P);
+    for (int i = 0; i < set.n; ++i)
+        if (set.pts[i].name == name)
+            return MV(bz_coord(P, set.pts[i].x), bz_coord(P, set.pts[i].y), bz_coord(P, set.pts[i].z));
     return -1;
 }
 
@@ -467,14 +651,53 @@ extern "C" void fame_wave_vec_rotate(const fame_lattice_out* lat, const double V
     k[2] = Om[2] * V[0] + Om[5] * V[1] + Om[8] * V[2];
 }
 
+/* The reference's default path per lattice type and variant
+ * (FAME_Parameter_Brillouin_Zone_Path.cpp:166-267), with its own variant tests:
+ * the rhombohedral angle from a and c, the face-centred orthorhombic test with
+ * strict inequalities, and the base-centred monoclinic test in single
+ * precision (cosf, sinf) against the angle between b1 and b2.  nullptr where
+ * the reference sets no path. */
 extern "C" const char* fame_bz_default_path(const fame_lattice_out* lat) {
-    const char* t = lat->lattice_type;   /* BZ_Path.cpp:166-187 */
+    const char* t = lat->lattice_type;
+    const double a = lat->a, b = lat->b, c = lat->c, al = lat->alpha;
     if (is(t, "simple_cubic")) return "GXMGRX|MR";
     if (is(t, "face_centered_cubic")) return "GXWKGLUWLK|UX";
     if (is(t, "body_centered_cubic")) return "GHNGPH|PN";
     if (is(t, "hexagonal")) return "GMKGALHA|LM|KH";
+    if (is(t, "rhombohedral")) {
+        const double th = 2 * std::asin(3 * a / (2 * std::sqrt(std::pow(c, 2) + 3 * std::pow(a, 2))));
+        return th <= PI / 2 ? "GLC|BZGX|QFRZ|LP" : "GPZQGFRSLZ";
+    }
     if (is(t, "primitive_tetragonal")) return "GXMGZRAZ|XR|MA";
+    if (is(t, "body_centered_tetragonal")) {
+        if (c < a) return "GXMGZPNBM|XP";
+        if (c > a) return "GXYsGZtNPAZ|XP";
+        return nullptr;
+    }
     if (is(t, "primitive_orthorhombic")) return "GXSYGZURTZ|YT|UX|SR";
+    if (is(t, "c_base_centered_orthorhombic")) return "GXSRAZGYCBTY|ZT";
+    if (is(t, "face_centered_orthorhombic")) {
+        const double q1 = 1 / std::pow(a, 2), q2 = 1 / std::pow(b, 2) + 1 / std::pow(c, 2);
+        if (q1 > q2) return "GYTZGXBY|TJ|XAZ|LG";
+        if (q1 < q2) return "GYCDXGZFHC|EZ|XI|HY|LG";
+        return "GYTZGXBY|XAZ|LG";
+    }
+    if (is(t, "body_centered_orthorhombic")) return "GXLTWRAZGYSW|IY|BZ";
+    if (is(t, "primitive_monoclinic")) return "GYHCENAXI|MDZ|YD";
+    if (is(t, "base_centered_monoclinic")) {
+        const double q = b * cosf(al) / c + std::pow(b, 2) * std::pow(sinf(al), 2) / std::pow(a, 2);
+        const double* B = lat->recip_b;
+        double dot = 0, n1 = 0, n2 = 0;
+        for (int i = 0; i < 3; i++) { dot += B[i] * B[3 + i]; n1 += std::pow(B[i], 2); n2 += std::pow(B[i + 3], 2); }
+        n1 = std::sqrt(n1); n2 = std::sqrt(n2);
+        const double kg = std::acos(dot / (n1 * n2));
+        if (kg > PI / 2) return "GYFLI|RZJ|YT|XGN|MG";
+        if (kg == PI / 2) return "GYFLI|RZJ|NGM";
+        if (q < 1) return "GYFHZIJ|PVXGN|MG";
+        if (q == 1) return "GYFHZI|PVXGN|MG";
+        return "GYFLI|RZHJ|PVXGN|MG";
+    }
+    if (is(t, "triclinic")) return "XGY|LGZ|NGM|RG";
     return nullptr;
 }
 

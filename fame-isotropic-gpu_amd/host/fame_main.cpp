@@ -108,7 +108,13 @@ extern "C" int Lanczos_Isotropic(double* Freq_array, void* ev_dev, fame_ctx* ctx
     return mx;
 }
 
-extern "C" int FAME_Main_Code(PAR Par, PROFILE* Profile) {
+/* C++-linkage entry, as the reference declares it (include/FAME_Main_Code.cuh:4);
+ * the C-linkage symbol of the same name lives in fame_main_c.c */
+int FAME_Main_Code(PAR Par, PROFILE* Profile) { return fame_main_code_run(&Par, Profile); }
+
+extern "C" int fame_main_code_run(const PAR* ParP, PROFILE* Profile) {
+    if (!ParP) return FAME_ERR_ARG;
+    const PAR& Par = *ParP;
     const int Nx = Par.mesh.grid_nums[0], Ny = Par.mesh.grid_nums[1], Nz = Par.mesh.grid_nums[2];
     const int N = Nx * Ny * Nz;
     const size_t N3 = 3 * (size_t)N;
@@ -187,7 +193,7 @@ extern "C" int FAME_Main_Code(PAR Par, PROFILE* Profile) {
                 Profile->ls_iter[k0 + p] = prof[p].ls_iter;
                 Profile->es_iter[k0 + p] = prof[p].es_iter;
                 Profile->es_time[k0 + p] = secs / nb;
-                Profile->ls_time[k0 + p] = secs / nb;
+                Profile->ls_time[k0 + p] = prof[p].ls_time;
                 print_profile(*Profile);
             }
         }

@@ -53,7 +53,7 @@ class fame_ls(C.Structure):
 class fame_profile(C.Structure):
     _fields_ = [("ls_iter", C.c_int), ("es_iter", C.c_int), ("restarts", C.c_int), ("nd", C.c_int),
                 ("converged", C.c_int), ("qbq64", C.c_int), ("qbq32", C.c_int), ("pad", C.c_int),
-                ("es_time", C.c_double)]
+                ("es_time", C.c_double), ("ls_time", C.c_double)]
 
 
 class fame_lattice_in(C.Structure):
@@ -114,6 +114,8 @@ def lib():
     L.fame_residuals.argtypes = [vp, i, vp, vp, vp]
     L.fame_eig_grad.argtypes = [vp, i, vp, vp, vp, vp]
     L.fame_time_qbq_passes.argtypes = [vp, i, i, vp]
+    L.fame_set_refinement.argtypes = [vp, i, d, i]
+    L.fame_time_cg_passes.argtypes = [vp, i, i, vp]
     L.fame_time_qbq_passes.restype = C.c_int
     L.fame_set_stream.argtypes = [vp, vp]
     L.fame_query.argtypes = [vp, vp, vp, vp]
@@ -353,8 +355,13 @@ class Solver:
                                C.c_void_p(evr.data_ptr()) if evr is not None else None,
                                C.c_void_p(E.data_ptr()) if E is not None else None, prof), "fame_eigs")
         profs = [dict(ls_iter=p.ls_iter, es_iter=p.es_iter, restarts=p.restarts, nd=p.nd, converged=p.converged,
-                      qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time) for p in prof]
+                      qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time, ls_time=p.ls_time) for p in prof]
         return freq, evr, E, profs
+
+    def set_refinement(self, sweeps: int, tol: float = 1e-14, maxit: int = 2000):
+        """Inverse-iteration sweeps that eigs applies to the converged Ritz pairs
+        (0: the reference's algorithm); see fame_set_refinement."""
+        _check(lib().fame_set_refinement(self._ctx, int(sweeps), float(tol), int(maxit)), "fame_set_refinement")
 
     def restore(self, evr):
         """E fields (nk, nwant, 3N) recomputed from reduced Ritz vectors (nk, nwant, 2N)."""
@@ -371,6 +378,14 @@ class Solver:
         f = np.ascontiguousarray(freq, np.float64)
         _check(lib().fame_residuals(self._ctx, nwant, _ptr(f), C.c_void_p(E.data_ptr()), _ptr(res)), "fame_residuals")
         return res
+
+    def time_cg_passes(self, prec_bytes: int, reps: int = 10):
+        """Average microseconds of the kernels of one fused CG iteration (all wave
+        vectors active): inner expand, mid inverse, outer eps, alpha, mid forward,
+        inner contract, beta, whole iteration."""
+        us = np.zeros(8)
+        _check(lib().fame_time_cg_passes(self._ctx, prec_bytes, reps, _ptr(us)), "fame_time_cg_passes")
+        return us
 
     def autotune(self, prec_bytes: int = 8):
         """Tune the pass kernels on the current batch; returns {pass: (engine, lines)}."""
@@ -421,7 +436,7 @@ def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxi
     out = []
     for f, ps in zip(freqs, profs):
         out.append((f, [dict(ls_iter=p.ls_iter, es_iter=p.es_iter, restarts=p.restarts, nd=p.nd, converged=p.converged,
-                             qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time) for p in ps]))
+                             qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time, ls_time=p.ls_time) for p in ps]))
     return out
 
 

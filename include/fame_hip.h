@@ -60,6 +60,9 @@ typedef struct {
     int    qbq32;          /* FP32 QBQ applies (refinement inner solves)             */
     int    pad;
     double es_time;        /* seconds, eigensolve of the whole batch                 */
+    double ls_time;        /* seconds inside invAr (CG) solves, this wave vector's share
+                              of the batch's solves (the reference's Profile.ls_time,
+                              FAME_Matrix_Vector_Production_Isotropic_invAr.cu:29-43) */
 } fame_profile;
 
 enum { FAME_PREC_F64 = 0, FAME_PREC_F32_REFINE = 1 };
@@ -117,6 +120,16 @@ fame_status fame_cg(fame_ctx* ctx, int ik, const void* rhs_dev, void* x_dev, int
 fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                       void* evr_dev, void* E_dev, fame_profile* prof);
 
+/* Inverse-iteration sweeps that fame_eigs applies to the converged Ritz pairs
+ * before restoration (default 0: the reference's algorithm, whose stopping rule
+ * bounds the Ritz residual of A_r^{-1}, not of A_r).  Each sweep solves
+ * w = A_r^{-1} y for every band (one batched invAr per band, absolute CG
+ * tolerance tol, at most maxit iterations, in the solve's precision), takes
+ * lambda = Re<w,y>/<w,w> and y = w/||w||; it removes the high-frequency error
+ * that dominates the field residual ||C^H C E - lambda B E|| / ||E||.  The extra
+ * solves are counted in the profile.  No reference counterpart.             */
+fame_status fame_set_refinement(fame_ctx* ctx, int sweeps, double tol, int maxit);
+
 /* Restored E fields (device, nk*nwant*3N, reference scaling and Gamma rule, as
  * fame_eigs' E_dev) recomputed from the reduced Ritz vectors evr_dev (device,
  * nk*nwant*2N, as fame_eigs' evr_dev) with the permittivity and wave vectors
@@ -158,6 +171,15 @@ fame_status fame_eig_grad(fame_ctx* ctx, int nwant, const double* freq, const vo
  * counterpart.                                                                 */
 fame_status fame_time_qbq_passes(fame_ctx* ctx, int prec, int reps, double* us6);
 
+/* Average durations in microseconds (hipEvent timing on the context stream) of
+ * the kernels of one fused CG iteration with every wave vector active:
+ * us8 = {inner expand with the x and p updates, middle inverse, outer eps with
+ * the <p,Ap> sums, alpha, middle forward, inner contract with the r update and
+ * <r,r> sums, beta, whole iteration}; prec = 8 or 4.  FAME_ERR_UNSUPPORTED when
+ * an axis takes a Bluestein plan (that iteration form stores Ap).  Measurement
+ * helper, no reference counterpart.                                          */
+fame_status fame_time_cg_passes(fame_ctx* ctx, int prec, int reps, double* us8);
+
 /* Autotuner: time every feasible (line-transform engine, lines per block) of the
  * inner, middle and outer pass kernels of precision prec (8 or 4) on the
  * current batch and keep the fastest.  fame_eigs runs it once per context and
@@ -178,6 +200,20 @@ fame_status fame_set_stream(fame_ctx* ctx, void* stream);
 
 /* Problem sizes: N and the Stockham/Bluestein lengths of the three axes. */
 fame_status fame_query(const fame_ctx* ctx, int* N, int* nk, int* fft_len3);
+
+/* Host routines of the Lanczos restart (no device needed).
+ * fame_tridiag_eig: all eigenpairs of the symmetric tridiagonal (d[0..n-1],
+ * e[0..n-2]); w receives the eigenvalues in descending order, z (n x n,
+ * column-major) the eigenvectors.  Replaces LAPACKE_zpteqr(compz='I')
+ * (FAME_Main/Lanczos_Isotropic.cu:48-53).
+ * fame_tridiag_qr_step: one explicit shifted QR step T - mu I = QR, T <- RQ + mu I,
+ * in place on (diag, off); the rotation annihilating T(i+1,i) against the pivot
+ * a = T(i,i) is c[i] = a/r, s[i] = -T(i+1,i)/r, r = hypot(a, T(i+1,i)); the new
+ * off-diagonal is the mean of RQ's sub- and super-diagonal.  ru and rw are
+ * scratch of n doubles.  Replaces GVqrrq_g (FAME_Main/Lanczos_LockPurge_gpu.cu:95,245).
+ * Both return FAME_ERR_ARG for n < 1 or missing arrays.                        */
+int fame_tridiag_eig(int n, const double* d, const double* e, double* w, double* z);
+int fame_tridiag_qr_step(double* diag, double* off, double* ru, double* rw, double* c, double* s, double mu, int n);
 
 /* Message of the last error on this thread. */
 const char* fame_last_error(void);

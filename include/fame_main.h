@@ -1,12 +1,15 @@
 /*
- * fame_main.h -- source-level drop-in for the reference's top entry points.
+ * fame_main.h -- source- and binary-level drop-in for the reference's top entry points.
  *
- * The structs carry the same field names and meanings as the reference's
- * call-surface types (include/FAME_Internal_Common.h:62-211 in the reference
- * tree) so that a caller that fills a PAR for the reference can call this
- * library's FAME_Main_Code unchanged.  Device handles (CULIB_HANDLES,
- * LAMBDAS_CUDA, MTX_B, ...) do not appear: the context of include/fame_hip.h
- * owns everything on the device.
+ * The structs carry the same field names, types and order as the reference's
+ * call-surface types (include/FAME_Internal_Common.h:29-211 in the reference
+ * tree), so their layouts are identical (tests/test_layout.py compares every
+ * offset with a table compiled from the reference header) and a caller that
+ * fills a PAR for the reference can pass it to this library's FAME_Main_Code
+ * unchanged.  FAME_Main_Code has C++ linkage in C++, as the reference declares
+ * it (include/FAME_Main_Code.cuh:4), and a C-linkage symbol of the same name for
+ * C and FFI callers.  Device handles (CULIB_HANDLES, LAMBDAS_CUDA, MTX_B, ...)
+ * do not appear: the context of include/fame_hip.h owns everything on the device.
  */
 #ifndef FAME_MAIN_H
 #define FAME_MAIN_H
@@ -16,6 +19,14 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* Brillouin-zone vertices of the lattice, by letter, in units of 2 pi
+ * (FAME_Internal_Common.h:29-59; filled by FAME_Parameter_Brillouin_Zone_Point.cpp) */
+typedef struct {
+    double A[3], B[3], C[3], D[3], E[3], F[3], G[3], H[3], I[3], J[3], K[3], L[3], M[3];
+    double N[3], O[3], P[3], Q[3], R[3], S[3], T[3], U[3], V[3], W[3], X[3], Y[3], Z[3];
+    double s[3], t[3], k[3];
+} VERTEX;
 
 typedef struct { int grid_nums[3]; double edge_len[3]; double mesh_len[3]; } MESH;
 
@@ -53,6 +64,7 @@ typedef struct {
     double* WaveVector;          /* 3 * Wave_vec_num, computational frame, units of 2 pi */
     char path_string[50];
     double reciprocal_lattice_vector_b[9];
+    VERTEX vertex;
 } RECIP_LATTICE;
 
 typedef struct {
@@ -95,10 +107,12 @@ typedef struct {
  * Par.recip_lattice, in batches sized to the free device memory, FP64 (or the
  * mixed-precision solver when FAME_PRECISION=mixed is set in the environment).
  * Writes Data_Omega_array.txt (and Data_EV_kk_bb.txt per band when
- * Par.flag.save_eigen_vector is set) and fills Profile like the reference; a frequency
- * whose absolute residual (reference field scaling) exceeds 1e-10 is negated
- * (FAME_Main_Code.cu:383-421).  Returns 0, or a fame_status on failure.      */
-int FAME_Main_Code(PAR Par, PROFILE* Profile);
+ * Par.flag.save_eigen_vector is set) and fills Profile like the reference
+ * (ls_time: seconds inside the invAr solves, es_time: seconds of the eigensolve,
+ * per wave vector); a frequency whose absolute residual (reference field
+ * scaling) exceeds 1e-10 is negated (FAME_Main_Code.cu:383-421).  Returns 0, or
+ * a fame_status on failure.  Declared below with the linkage of the includer. */
+int fame_main_code_run(const PAR* Par, PROFILE* Profile);
 
 /* Writes Data_Profile.txt for rows 0..Profile.idx, in the format of the
  * reference's FAME_Create_Profile_txt (FAME_Tools/FAME_Profile.cpp:27-39),
@@ -114,6 +128,11 @@ int Lanczos_Isotropic(double* Freq_array, void* ev_dev, fame_ctx* ctx, ES es, LS
                       const char* flag_CompType, PROFILE* Profile);
 
 #ifdef __cplusplus
-}
+}  /* extern "C" */
+/* C++ linkage, as include/FAME_Main_Code.cuh:4 of the reference declares it */
+int FAME_Main_Code(PAR Par, PROFILE* Profile);
+#else
+/* C linkage alias of the same entry point, for C and FFI callers */
+int FAME_Main_Code(PAR Par, PROFILE* Profile);
 #endif
 #endif

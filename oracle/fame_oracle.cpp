@@ -245,7 +245,12 @@ static void build_pi(orc_tables* T, const std::vector<cplx>& Ldx, const std::vec
         cplx lx = T->Lx[i], ly = T->Ly[i], lz = T->Lz[i];
         double Lq = std::real(std::conj(lx) * lx + std::conj(ly) * ly + std::conj(lz) * lz);
         cplx p1[3], p2[3];
-        if (!T->general) {
+        /* Lambda parallel to (1,1,1): the General normalisers of the reference are
+         * 0/0 there (NaN); this oracle, like the product, takes the Simple gauge for
+         * such a mode.  Elsewhere the General branch is the reference's. */
+        const cplx e1 = lz - ly, e2 = lx - lz, e3 = ly - lx;
+        const double nd = std::norm(e1) + std::norm(e2) + std::norm(e3);
+        if (!T->general || !(nd > 1e-30 * Lq)) {
             /* Construct_Lambdas_Simple, FAME_Matrix_Lambdas.cu:184-215 (alpha=3, beta=5) */
             const double al = 3., be = 5.;
             cplx t1 = be * lz - ly, t2 = lx - al * lz, t3 = al * ly - be * lx;

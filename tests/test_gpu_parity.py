@@ -252,3 +252,121 @@ def test_batch_with_uneven_convergence(gpu):
         assert np.max(np.abs(f1[nz] ** 2 - freq[i][nz] ** 2) / f1[nz] ** 2) < 1e-9
     one.close()
     S.close()
+
+
+def _align_err(u, v):
+    """|| u/|u| - phase * v/|v| || for the best unit phase."""
+    u = u / np.linalg.norm(u)
+    v = v / np.linalg.norm(v)
+    ph = np.vdot(v, u)
+    return float(np.linalg.norm(u - ph / abs(ph) * v))
+
+
+@pytest.mark.parametrize("prec", [F.PREC_F64, F.PREC_F32_REFINE])
+@pytest.mark.parametrize("sname", list(STRUCTS))
+def test_fields_and_residuals_vs_oracle(gpu, sname, prec):
+    """The restored E fields of fame_eigs against the oracle's (each
+    non-degenerate band up to a global phase, and the multiply-by-the-norm
+    scale of FAME_Fast_Algorithms_Isotropic.cu:119-121), the Gamma fields, and
+    fame_residuals against the oracle's residual of the same field."""
+    lo, fl, ol, eps = make(sname)
+    ks = np.stack([KPTS["k"], KPTS["gamma"]])
+    nwant, nstep = 6, 20
+    S = F.Solver(fl, max_batch=len(ks))
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    freq, _, E, prof = S.eigs(nwant, nstep, precision=prec, want_fields=True)
+    res = S.residuals(freq, E)
+    Eh = E.cpu().numpy()
+    N3 = 3 * S.N
+    checked = 0
+    for i, k in enumerate(ks):
+        t = O.Tables(ol, k)
+        fo, Eo, _ = O.eigs(ol, eps, k, nwant, nstep, fields=True)
+        lam = fo ** 2
+        for b in range(nwant):
+            if lam[b] == 0:
+                const = np.full(N3, 1.0 / np.sqrt(N3))
+                assert np.abs(Eh[i, b] - const).max() == 0 and freq[i, b] == 0
+                continue
+            # the top band's partner above the wanted set is unknown, so its
+            # field may be any vector of a degenerate eigenspace: not aligned
+            gaps = [abs(lam[b] - lam[c]) for c in range(nwant) if c != b and lam[c] > 0]
+            if b < nwant - 1 and gaps and min(gaps) > 1e-6 * lam[b]:
+                assert _align_err(Eh[i, b], Eo[b]) < 1e-6, (sname, i, b)
+                nr = np.linalg.norm(Eh[i, b]) / np.linalg.norm(Eo[b])
+                assert abs(nr - 1) < 1e-6, (sname, i, b, nr)
+                checked += 1
+            # the GPU residual kernel against the oracle's residual of the same field
+            ro = t.rel_residual(eps, Eh[i, b], freq[i, b])
+            assert abs(res[i, b] - ro) <= 1e-3 * ro + 1e-11 * lam[b], (res[i, b], ro)
+    # at Gamma the cubic structures' bands come in degenerate sets; the generic
+    # wave vector has non-degenerate ones
+    assert checked >= 2
+    S.close()
+
+
+@pytest.mark.parametrize("sname", list(STRUCTS))
+def test_residual_kernel_on_arbitrary_fields(gpu, sname):
+    """fame_residuals on fields that are not eigenvectors (residual of order
+    ||C^H C||): the FFT curl-curl of the GPU against the oracle's sparse curl."""
+    import torch
+    lo, fl, ol, eps = make(sname)
+    ks = np.stack([KPTS["k"], KPTS["gamma"]])
+    S = F.Solver(fl, max_batch=2)
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    nw = 3
+    E = np.stack([np.stack([_rand(3 * S.N, 50 + 7 * i + b) for b in range(nw)]) for i in range(2)])
+    freq = np.array([[0.5, 1.7, 3.3], [0.9, 2.2, 4.1]])
+    res = S.residuals(freq, torch.tensor(E, device=gpu))
+    for i, k in enumerate(ks):
+        t = O.Tables(ol, k)
+        for b in range(nw):
+            ro = t.rel_residual(eps, E[i, b], freq[i, b])
+            assert abs(res[i, b] - ro) <= 1e-10 * ro, (sname, i, b, res[i, b], ro)
+    S.close()
+
+
+@pytest.mark.parametrize("prec", [F.PREC_F64, F.PREC_F32_REFINE])
+def test_krylov_dimension_above_64(gpu, prec):
+    """nstep = 3 nwant = 72 (the uneven-batch configuration's rule): the Gram
+    kernel takes its 128-column form; eigenvalues against the oracle."""
+    lo, fl, ol, eps = make("fcc")
+    k = KPTS["k"]
+    nwant, nstep = 24, 72
+    S = F.Solver(fl, max_batch=1)
+    S.set_eps(eps)
+    S.set_wave_vectors(k)
+    freq, _, E, prof = S.eigs(nwant, nstep, precision=prec, want_fields=True)
+    fo, _, _ = O.eigs(ol, eps, k, nwant, nstep)
+    assert prof[0]["converged"] == 1
+    assert np.max(np.abs(freq[0] ** 2 - fo ** 2) / fo ** 2) < 1e-8
+    res = S.residuals(freq, E)
+    assert np.all(res[0] < 1e-6 * freq[0] ** 2)
+    S.close()
+
+
+def test_general_gauge_on_modes_parallel_to_111(gpu):
+    """A cubic lattice solved through the General path (z-fastest Fourier
+    layout, skew twiddles with m = 0) at k along (1,1,1): the modes with
+    m_x = m_y = m_z have Lambda_x = Lambda_y = Lambda_z, where the reference's
+    General normalisers are 0/0.  The spectrum must be finite and equal the
+    Simple path's and the oracle's."""
+    lo, fl, ol, eps = make("sc")
+    k = np.array([0.11, 0.11, 0.11])
+    out = {}
+    for general in (0, 1):
+        fl.general = general
+        S = F.Solver(fl, max_batch=1)
+        S.set_eps(eps)
+        S.set_wave_vectors(k)
+        out[general] = S.eigs(8, 24)[0][0]
+        S.close()
+    fl.general = 0
+    assert np.all(np.isfinite(out[1]))
+    assert np.max(np.abs(out[1] ** 2 - out[0] ** 2) / out[0] ** 2) < 1e-9
+    og = O.lattice_from(fl)
+    og.general = 1
+    fo, _, _ = O.eigs(og, eps, k, 8, 24)
+    assert np.max(np.abs(out[1] ** 2 - fo ** 2) / fo ** 2) < 1e-9

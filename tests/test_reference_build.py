@@ -47,7 +47,9 @@ def _parse(path):
     return data
 
 
-def _run_ref(name, nx, part_num=4):
+def _run_ref(name, nx, part_num=4, grid_max=None):
+    """The reference's preprocessing of material `name` at Nx = nx; grid_max =
+    (mem_size GB, nwant, nstep) turns on its grid auto-sizing."""
     w = tempfile.mkdtemp()
     try:
         os.makedirs(os.path.join(w, "FAME_Material_Data"))
@@ -55,6 +57,12 @@ def _run_ref(name, nx, part_num=4):
         popt = open(os.path.join(REF_TREE, "Popt.txt")).read()
         popt = popt.replace(" 48 48 48", f" {nx} {nx} {nx}").replace("No221_SC_Demo", name)
         popt = popt.replace("# Part number\n 10", f"# Part number\n {part_num}")
+        if grid_max is not None:
+            mem, nwant, nstep = grid_max
+            popt = popt.replace("# Grid nums Max\n 0 16", f"# Grid nums Max\n 1 {mem}")
+            popt = popt.replace("# Desired eigenpair number\n 10", f"# Desired eigenpair number\n {nwant}")
+            popt = popt.replace("# Dimension of Krylov subspace for Lanczos\n 30",
+                                f"# Dimension of Krylov subspace for Lanczos\n {nstep}")
         open(os.path.join(w, "Popt.txt"), "w").write(popt)
         ro = RefOut()
         lib = _ref()
@@ -87,16 +95,46 @@ ALL = sorted(f[:-4] for f in os.listdir(os.path.join(REF_TREE, "FAME_Material_Da
 
 @pytest.mark.parametrize("name", ALL)
 def test_lattice_bit_exact(name):
+    """Lattice, grid, boundary points and reciprocal vectors, and the default
+    Brillouin-zone path of the lattice type (all 14 types and their variants
+    occur among the shipped structures), bit for bit."""
     d = _parse(os.path.join(REF_TREE, "FAME_Material_Data", name + ".txt"))
     lc = [float(x) for x in d["# Lattice constant"]]
     for nx in (16, 23):
-        _, ro = _run_ref(name, nx)
+        lib, ro = _run_ref(name, nx)
         lo = F.build_lattice(d["# Lattice type"][0], nx, *lc)
         for fld in ("grid_nums", "edge_len", "mesh_len", "lattice_vec_a", "lattice_vec_a_orig", "Omega", "Permutation",
                     "length_a", "theta", "t1", "t2", "t3", "t4", "recip_b"):
             assert np.array_equal(np.array(getattr(ro, fld)[:]), np.array(getattr(lo, fld)[:])), (name, nx, fld)
         for fld in ("m1", "m2", "m3", "m4", "flag"):
             assert getattr(ro, fld) == getattr(lo, fld), (name, nx, fld)
+        if ro.wave_vec_num == 0:
+            # the reference sets no path for this type (a-base-centred orthorhombic)
+            assert not F.lib().fame_bz_default_path(C.byref(lo)), name
+            continue
+        wv = np.zeros((ro.wave_vec_num, 3))
+        lib.ref_wave_vectors(wv.ctypes.data_as(C.c_void_p))
+        assert np.array_equal(F.bz_path(lo, None, 4), wv), (name, nx)
+
+
+@pytest.mark.parametrize("name,mem,nwant,nstep", [("No221_SC_Demo", 1, 10, 30), ("No227_FCC_DiamondStructure", 2, 16, 48),
+                                                   ("No229_BCC_W", 1, 8, 20), ("No191_Hexagonal_AlB2", 3, 12, 36),
+                                                   ("No139_BCT_Si", 1, 10, 30)])
+def test_grid_nums_max_bit_exact(name, mem, nwant, nstep):
+    """Popt's "Grid nums Max" auto-sizing (FAME_Parameter_Generator.cpp:114-148)."""
+    d = _parse(os.path.join(REF_TREE, "FAME_Material_Data", name + ".txt"))
+    lc = [float(x) for x in d["# Lattice constant"]]
+    for nx in (16, 64):
+        _, ro = _run_ref(name, nx, grid_max=(mem, nwant, nstep))
+        li = F.fame_lattice_in()
+        li.lattice_type = d["# Lattice type"][0].encode()
+        li.a, li.b, li.c, li.alpha, li.beta, li.gamma = lc
+        li.grid_nums[0] = li.grid_nums[1] = li.grid_nums[2] = nx
+        L = F.lib()
+        L.fame_grid_nums_max.argtypes = [C.POINTER(F.fame_lattice_in), C.c_int, C.c_int, C.c_int]
+        nx_new = L.fame_grid_nums_max(C.byref(li), nwant, nstep, mem)
+        lo = F.build_lattice(d["# Lattice type"][0], nx_new, *lc)
+        assert list(lo.grid_nums) == list(ro.grid_nums), (name, nx, nx_new, list(ro.grid_nums))
 
 
 @pytest.mark.parametrize("name", ["No221_SC_Demo", "No227_FCC_DiamondStructure", "No229_BCC_W",
@@ -121,10 +159,9 @@ def test_mask_wavevectors_curl_bit_exact(name):
                 mine[comp * nm * N + m * N + i] = 1
     # the reference marks only the first material containing a point
     assert np.array_equal(mine, binout)
-    if F.lib().fame_bz_default_path(C.byref(lo)):
-        wv = np.zeros((ro.wave_vec_num, 3))
-        lib.ref_wave_vectors(wv.ctypes.data_as(C.c_void_p))
-        assert np.array_equal(F.bz_path(lo, None, 4), wv)
+    wv = np.zeros((ro.wave_vec_num, 3))
+    lib.ref_wave_vectors(wv.ctypes.data_as(C.c_void_p))
+    assert np.array_equal(F.bz_path(lo, None, 4), wv)
     fl = F.to_fame_lattice(lo)
     ol = O.lattice_from(fl)
     k = np.array([0.13, -0.21, 0.05])
