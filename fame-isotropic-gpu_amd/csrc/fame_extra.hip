@@ -17,6 +17,7 @@
 #include "fame_kernels.h"
 
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -203,6 +204,48 @@ extern "C" fame_status fame_residuals(fame_ctx* c, int nwant, const double* freq
     const fame_status st = residuals_chunk(c, nwant, freq, E_dev, res, Ecopy, 0, nk);
     (void)hipFree(Ecopy);
     return st;
+}
+
+/* Scratch of fame_residuals_noalloc: the fields of as many wave vectors as fit,
+ * in the kernel layout.  A file-scope device array, so that the entry point
+ * allocates nothing; every device the library runs on holds it (128 MiB) until
+ * the process ends.  Host code never passes the array's name to a kernel: it
+ * uses the device address that hipGetSymbolAddress returns, once per device. */
+#define FAME_NOALLOC_ELEMS (8u << 20)    /* complex doubles: 128 MiB */
+static __device__ __align__(256) double2 scratch[FAME_NOALLOC_ELEMS];
+
+static std::mutex g_scratch_mu;
+static double2* g_scratch_dev[64] = {nullptr};
+
+/* the current device's address of `scratch` (the current device must be dev) */
+static fame_status scratch_address(int dev, double2** out) {
+    if (dev < 0 || dev >= 64) FAIL(FAME_ERR_ARG, "fame_residuals_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = (double2*)ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_residuals_noalloc(fame_ctx* c, int nwant, const double* freq, const void* E_dev,
+                                              double* res) {
+    if (!c || !freq || !E_dev || !res || nwant < 1) FAIL(FAME_ERR_ARG, "fame_residuals_noalloc: bad arguments");
+    if (!c->have_eps) FAIL(FAME_ERR_STATE, "fame_residuals_noalloc: permittivity not set");
+    const size_t N3 = 3 * (size_t)c->N;
+    if (N3 > FAME_NOALLOC_ELEMS)
+        FAIL(FAME_ERR_UNSUPPORTED, "fame_residuals_noalloc: 3N = " + std::to_string(N3) + " exceeds the static scratch of " +
+                                       std::to_string(FAME_NOALLOC_ELEMS) + " complex values (N <= " +
+                                       std::to_string(FAME_NOALLOC_ELEMS / 3) + ")");
+    HIPCHK(hipSetDevice(c->device));
+    double2* sc = nullptr;
+    CHK(scratch_address(c->device, &sc));
+    const int chunk = (int)std::min<size_t>((size_t)c->nk, FAME_NOALLOC_ELEMS / N3);
+    for (int p0 = 0; p0 < c->nk; p0 += chunk)
+        CHK(residuals_chunk(c, nwant, freq, E_dev, res, sc, p0, std::min(chunk, c->nk - p0)));
+    return FAME_OK;
 }
 
 /* Average duration (microseconds, hipEvent timing on the context stream) of each

@@ -112,6 +112,7 @@ def lib():
     L.fame_cg.argtypes = [vp, i, vp, vp, i, d, C.POINTER(i)]
     L.fame_eigs.argtypes = [vp, i, C.POINTER(fame_es), C.POINTER(fame_ls), vp, vp, vp, vp]
     L.fame_residuals.argtypes = [vp, i, vp, vp, vp]
+    L.fame_residuals_noalloc.argtypes = [vp, i, vp, vp, vp]
     L.fame_eig_grad.argtypes = [vp, i, vp, vp, vp, vp]
     L.fame_time_qbq_passes.argtypes = [vp, i, i, vp]
     L.fame_set_refinement.argtypes = [vp, i, d, i]
@@ -386,6 +387,16 @@ class Solver:
         us = np.zeros(8)
         _check(lib().fame_time_cg_passes(self._ctx, prec_bytes, reps, _ptr(us)), "fame_time_cg_passes")
         return us
+
+    def residuals_noalloc(self, freq, E):
+        """As residuals, through fame_residuals_noalloc (no device allocation;
+        the fields are staged in the library's static device scratch)."""
+        nk, nwant = freq.shape
+        res = np.zeros((nk, nwant))
+        f = np.ascontiguousarray(freq, np.float64)
+        _check(lib().fame_residuals_noalloc(self._ctx, nwant, _ptr(f), C.c_void_p(E.data_ptr()), _ptr(res)),
+               "fame_residuals_noalloc")
+        return res
 
     def autotune(self, prec_bytes: int = 8):
         """Tune the pass kernels on the current batch; returns {pass: (engine, lines)}."""

@@ -152,6 +152,17 @@ fame_status fame_eigs_many(int n, fame_ctx* const* ctxs, const int* precision, c
  * Check_Residual (FAME_Main/FAME_Main_Code.cu:383), reported relative.       */
 fame_status fame_residuals(fame_ctx* ctx, int nwant, const double* freq, const void* ev_dev, double* res);
 
+/* fame_residuals without any device allocation, for callers that cannot
+ * allocate (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The fields are staged
+ * in a file-scope device array of 8M complex doubles (128 MiB per device, held
+ * until the process ends), so 3N may not exceed 8,388,608 (N <= 2,796,202;
+ * larger grids return FAME_ERR_UNSUPPORTED); wave vectors are processed in
+ * chunks that fit.  Same arguments and results as fame_residuals.  Every call
+ * on one device shares that array: calls that overlap on one GPU (other host
+ * threads or streams) race.                                                  */
+fame_status fame_residuals_noalloc(fame_ctx* ctx, int nwant, const double* freq, const void* ev_dev, double* res);
+
 /* Vector-Jacobian product of the eigenvalues lambda_kj = omega_kj^2 with respect
  * to the permittivity of every Yee edge, by Hellmann-Feynman:
  *   grad_e = sum_kj w_kj d lambda_kj / d eps_e,
