@@ -429,9 +429,12 @@ class Solver:
 OP_INVAR_MIXED, OP_QBQ32 = 6, 7
 
 
-def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxit=1000, precision=PREC_F64):
+def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxit=1000, precision=PREC_F64,
+              want_fields=False):
     """Solve several structures (Solvers of any grid and batch) in one call;
-    nwant / nstep / precision may be per-solver lists.  Returns [(freq, profs)]."""
+    nwant / nstep / precision may be per-solver lists.  Returns [(freq, profs)],
+    or [(freq, profs, E)] with the restored fields (nk, nwant, 3N) on the GPU
+    when want_fields."""
     n = len(solvers)
     per = lambda v: list(v) if isinstance(v, (list, tuple)) else [v] * n
     nw, ns, pr = per(nwant), per(nstep), per(precision)
@@ -443,11 +446,18 @@ def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxi
     profs = [(fame_profile * len(s.nd))() for s in solvers]
     fp = (C.c_void_p * n)(*[f.ctypes.data for f in freqs])
     pp = (C.c_void_p * n)(*[C.addressof(p) for p in profs])
-    _check(lib().fame_eigs_many(n, ctxs, prec, esa, lsa, fp, None, None, pp), "fame_eigs_many")
+    Es, ep = None, None
+    if want_fields:
+        torch = _torch()
+        Es = [torch.empty((len(s.nd), nw[i], 3 * s.N), dtype=torch.complex128, device=torch.device("cuda", s.device))
+              for i, s in enumerate(solvers)]
+        ep = (C.c_void_p * n)(*[E.data_ptr() for E in Es])
+    _check(lib().fame_eigs_many(n, ctxs, prec, esa, lsa, fp, None, ep, pp), "fame_eigs_many")
     out = []
-    for f, ps in zip(freqs, profs):
-        out.append((f, [dict(ls_iter=p.ls_iter, es_iter=p.es_iter, restarts=p.restarts, nd=p.nd, converged=p.converged,
-                             qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time, ls_time=p.ls_time) for p in ps]))
+    for i, (f, ps) in enumerate(zip(freqs, profs)):
+        pd = [dict(ls_iter=p.ls_iter, es_iter=p.es_iter, restarts=p.restarts, nd=p.nd, converged=p.converged,
+                   qbq64=p.qbq64, qbq32=p.qbq32, es_time=p.es_time, ls_time=p.ls_time) for p in ps]
+        out.append((f, pd, Es[i]) if want_fields else (f, pd))
     return out
 
 

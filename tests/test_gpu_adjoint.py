@@ -128,3 +128,52 @@ def test_restore_matches_eigs_fields(gpu):
     E2 = S.restore(evr)
     assert float((E2 - E).abs().max()) <= 1e-13 * float(E.abs().max())
     S.close()
+
+
+def test_grad_of_degenerate_cluster_is_the_trace_derivative(gpu):
+    """A structure with cubic symmetry (SC, sphere at the cell centre, uniform
+    eps) at k along z: bands come in exactly degenerate pairs.  Inside a
+    cluster only the trace is differentiable; fame_eig_grad replaces the
+    weights of a cluster by their mean, so that
+      * weights (1, 1) on a pair give the derivative of the pair's trace, which
+        any orthonormal basis of the dense eigenspace reproduces, and
+      * weights (1, 0) give half of it."""
+    lo = F.build_lattice("simple_cubic", 6)
+    fl = F.to_fame_lattice(lo)
+    ol = O.lattice_from(fl)
+    N = 6 ** 3
+    mat = F.Material(sphere_centers=[[(0.5, 0.5, 0.5)]], sphere_radius=[[0.3]])
+    eps = F.material_eps(lo, mat, [13.0], 1.0)
+    k = np.array([0.0, 0.0, 0.17])
+    nwant = 6
+    r, c, v = O.curl_coo(ol, k)
+    Cm = sp.coo_matrix((v, (r, c)), shape=(3 * N, 3 * N)).toarray()
+    w, V = sl.eigh(Cm.conj().T @ Cm, np.diag(eps))
+    first = int(np.argmax(w > 1e-8 * w.max()))
+    lam = w[first:first + nwant + 1]
+    Vn = V[:, first:first + nwant + 1]
+    # the clusters among the wanted bands, from the dense spectrum
+    clusters, i = [], 0
+    while i < nwant:
+        j = i + 1
+        while j < nwant + 1 and abs(lam[j] - lam[i]) <= 1e-9 * lam[i]:
+            j += 1
+        clusters.append((i, j))
+        i = j
+    pairs = [(a, b) for a, b in clusters if b - a == 2 and b <= nwant]
+    assert pairs, (lam, clusters)                         # the scenario this test is about
+    a, b = pairs[0]
+
+    S = F.Solver(fl, max_batch=1)
+    S.set_eps(eps)
+    S.set_wave_vectors(k)
+    freq, _, E, _ = S.eigs(nwant, 20, want_fields=True)
+    assert np.max(np.abs(freq[0] ** 2 - lam[:nwant]) / lam[:nwant]) < 1e-8
+    trace_ref = -(np.abs(Vn[:, a:b]) ** 2).sum(axis=1) * lam[a]     # sum_j -lambda |E_j|^2 over the pair
+    w11 = np.zeros((1, nwant)); w11[0, a:b] = 1.0
+    w10 = np.zeros((1, nwant)); w10[0, a] = 1.0
+    g11 = S.eig_vjp(freq, E, w11).cpu().numpy()
+    g10 = S.eig_vjp(freq, E, w10).cpu().numpy()
+    assert np.abs(g11 - trace_ref).max() < 1e-8 * np.abs(trace_ref).max()
+    assert np.abs(g10 - 0.5 * g11).max() < 1e-12 * np.abs(g11).max()
+    S.close()

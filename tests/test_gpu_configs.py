@@ -2,7 +2,8 @@
 
 C1  SC a=1, sphere r=0.25 at the cell centre, eps 13/1, 32^3, one k =
     (pi/a)(0.5, 0.25, 0) i.e. (0.25, 0.125, 0) in units of 2 pi, 10 bands,
-    nstep 30: the GPU FP64 and FP32+refinement solves against the FP64 oracle.
+    nstep 30: the GPU FP64 and FP32+refinement solves against the FP64 oracle,
+    and the field residual after two inverse-iteration sweeps (<= 1e-10 in FP64).
 C2  FCC a=1 diamond network, 64 -> 64x55x52, 30 k on Gamma-X-W-L-Gamma, 12 bands,
     nstep 36: batched FP64 against the plain (one wave vector at a time) FP64
     solve, and batched FP32+refinement against FP64.
@@ -11,10 +12,16 @@ C3  BCC a=1, 8 spheres with seeded fractional centres U[0,1)^3 and radii
     Gamma-H-N-Gamma-P, 16 bands, nstep 48: FP32+refinement against FP64.
 C4  SC 64^3, eps = 1 + 11.96 sigmoid(4 z), z seeded N(0,1) smoothed with a
     periodic Gaussian of sigma = 2 cells, 8 k, objective lambda_6 - lambda_5:
-    the autograd gradient against central differences on eight seeded cells.
-C5  16 seeded structures of four lattices, grids 48-96, 8-24 bands, solved
-    concurrently in one call against the same structures one after another.
-At these sizes the oracle is too slow except for C1, so C2-C4 rely on
+    the autograd gradient against central differences (h = 1e-6) on 32 cells
+    drawn uniformly with seed 4.
+C5  64 seeded structures (seed 5) of four lattices (SC, FCC, BCC, hexagonal
+    c/a = 1.2), 1-8 spheres of radius U[0.08, 0.2], eps_in U[4, 13], nominal
+    grids drawn from {48, 64, 96, 128}, 8-24 bands with nstep = 3 nwant, solved
+    in four fame_eigs_many calls of 16 structures (one per test, so that each
+    test stays short), every band's residual checked and two structures per call
+    re-solved on their own.
+The oracle at these sizes is too slow except for C1 (the FCC 64/96 oracle
+fixtures of test_gpu_golden_fcc.py cover C2's grid), so C2-C5 rely on
 size-independent properties: plain == batched, FP32+refinement == FP64 to the
 1e-8 target, relative residuals, and finite differences.
 """
@@ -53,7 +60,21 @@ def test_c1_sc32_vs_oracle(gpu):
         assert _rel_err(f[0] ** 2, fo ** 2) < 1e-8, (prec, f[0], fo)
         res = S.residuals(f, E)
         assert np.all(res[0] < 1e-6 * f[0] ** 2)
+    # two inverse-iteration sweeps of the converged Ritz pairs reach the field
+    # residual the north star asks for, ||C^H C E - lambda B E|| / ||E|| <= 1e-10
+    # in FP64 (the FP32-refined solve stays within 100 eps ||C^H C||)
+    S.set_refinement(2, 1e-16, 6000)
+    for prec, bar in ((F.PREC_F64, 1e-10), (F.PREC_F32_REFINE, 100 * 2.2e-16 * _curlcurl_norm(fl))):
+        f, _, E, prof = S.eigs(10, 30, precision=prec, want_fields=True)
+        assert _rel_err(f[0] ** 2, fo ** 2) < 1e-8, (prec, f[0], fo)
+        res = S.residuals(f, E)
+        assert np.all(res[0] <= bar), (prec, res[0].max(), bar)
     S.close()
+
+
+def _curlcurl_norm(fl):
+    """||C^H C|| bound of the discrete curl-curl: sum over axes of (2/h)^2."""
+    return float(sum((2.0 / h) ** 2 for h in fl.mesh_len))
 
 
 def test_c2_fcc64_plain_vs_batched(gpu):
@@ -143,12 +164,12 @@ def test_c4_sc64_adjoint_fd(gpu):
     z = z0.clone().requires_grad_(True)
     J(z).backward()
     g = z.grad.clone().reshape(-1)
-    # eight seeded cells among those with the largest derivatives; h = 1e-3 in z
-    # (a step of 1e-6 would move the objective by ~1e-13 relative at 64^3, below
-    # the eigensolver's 1e-12 tolerance)
-    top = np.argsort(-np.abs(g.numpy()))[:64]
-    cells = rng.choice(top, 8, replace=False)
-    h = 1e-3
+    gmax = float(g.abs().max())
+    # 32 cells drawn uniformly; central differences with h = 1e-6 in z.  The
+    # eigenvalues of both solves are exact to rounding (~1e-15 relative), so the
+    # difference quotient carries ~1e-9 absolute noise and an O(h^2) bias.
+    cells = rng.choice(nx ** 3, 32, replace=False)
+    h = 1e-6
     for c in cells:
         e = torch.zeros(nx ** 3, dtype=torch.float64)
         e[c] = 1.0
@@ -156,39 +177,58 @@ def test_c4_sc64_adjoint_fd(gpu):
         with torch.no_grad():
             fd = (J(z0 + h * e) - J(z0 - h * e)).item() / (2 * h)
         an = float(g[c])
-        assert abs(fd - an) <= 2e-3 * abs(an), (int(c), fd, an)
+        assert abs(fd - an) <= 1e-3 * abs(an) + 1e-4 * gmax, (int(c), fd, an, gmax)
     S.close()
 
 
-def test_c5_uneven_batch(gpu):
-    """C5: 16 seeded structures of different lattices (SC, FCC, BCC, hexagonal
-    c/a = 1.2), nominal grids 48 / 64 / 96, 1-8 spheres of radius U[0.08, 0.2],
-    eps_in U[4, 13], 8-24 bands with nstep = 3 nwant (at most 60), two wave
-    vectors each, solved concurrently in one fame_eigs_many call and checked
-    against the same structures solved one after another.  (64 structures up to
-    128^3 would take minutes; this keeps every axis of the configuration.)"""
+def _c5_structures():
+    """The 64 seeded structures of C5 (inputs only: lattice, grid, material,
+    permittivity, wave vectors, bands)."""
     rng = np.random.default_rng(5)
-    lattices = [("simple_cubic", {}), ("face_centered_cubic", {}), ("body_centered_cubic", {}), ("hexagonal", {"c": 1.2})]
-    solvers, nw, ns, ks_all, eps_all, fls = [], [], [], [], [], []
-    for i in range(16):
-        lt, kw = lattices[i % 4]
-        nx = int(rng.choice([48, 64, 96], p=[0.5, 0.35, 0.15]))
-        lo = F.build_lattice(lt, nx, **kw)
-        fl = F.to_fame_lattice(lo)
+    lattices = [("simple_cubic", {}), ("face_centered_cubic", {}), ("body_centered_cubic", {}),
+                ("hexagonal", {"c": 1.2})]
+    out = []
+    for i in range(64):
+        lt, kw = lattices[int(rng.integers(0, 4))]
+        nx = int(rng.choice([48, 64, 96, 128]))
         nsph = int(rng.integers(1, 9))
-        mat = F.Material(sphere_centers=[[tuple(c) for c in rng.random((nsph, 3))]],
-                         sphere_radius=[list(rng.uniform(0.08, 0.2, nsph))])
-        eps = F.material_eps(lo, mat, [float(rng.uniform(4, 13))], 1.0)
-        ks = rng.uniform(-0.4, 0.4, (2, 3))
-        S = F.Solver(fl, max_batch=2)
+        centers = [tuple(c) for c in rng.random((nsph, 3))]
+        radii = list(rng.uniform(0.08, 0.2, nsph))
+        eps_in = float(rng.uniform(4, 13))
+        ks = rng.uniform(-0.4, 0.4, (1, 3))
+        nwant = int(rng.integers(8, 25))
+        out.append(dict(lattice=lt, kw=kw, nx=nx, centers=centers, radii=radii, eps_in=eps_in, ks=ks, nwant=nwant))
+    return out
+
+
+@pytest.mark.parametrize("group", range(4))
+def test_c5_uneven_batch(gpu, group):
+    """C5, structures 16 group .. 16 group + 15 of the 64, in one fame_eigs_many
+    call (each structure in its own context and stream, concurrently): every
+    band's relative residual, and two structures re-solved on their own must
+    agree with the concurrent solve."""
+    specs = _c5_structures()[16 * group:16 * group + 16]
+    solvers, nw, ns, grids = [], [], [], []
+    for sp in specs:
+        lo = F.build_lattice(sp["lattice"], sp["nx"], **sp["kw"])
+        fl = F.to_fame_lattice(lo)
+        mat = F.Material(sphere_centers=[sp["centers"]], sphere_radius=[sp["radii"]])
+        eps = F.material_eps(lo, mat, [sp["eps_in"]], 1.0)
+        S = F.Solver(fl, max_batch=len(sp["ks"]))
         S.set_eps(eps)
-        S.set_wave_vectors(ks)
-        w = int(rng.integers(8, 25))
-        solvers.append(S); nw.append(w); ns.append(min(3 * w, 60)); ks_all.append(ks); eps_all.append(eps); fls.append(fl)
-    out = F.eigs_many(solvers, nw, ns)
-    for i, (f, prof) in enumerate(out):
-        assert all(p["converged"] for p in prof)
+        S.set_wave_vectors(sp["ks"])
+        solvers.append(S); nw.append(sp["nwant"]); ns.append(3 * sp["nwant"]); grids.append(tuple(fl.grid_nums))
+    print(f"C5 group {group}: grids {grids}, bands {nw}", flush=True)
+    out = F.eigs_many(solvers, nw, ns, want_fields=True)
+    for i, (f, prof, E) in enumerate(out):
+        assert all(p["converged"] for p in prof), i
+        res = solvers[i].residuals(f, E)
+        nz = f > 0
+        # the theta-space stopping rule, amplified by lambda ||A_r|| at up to 128-class grids
+        assert np.all(res[nz] < 1e-5 * f[nz] ** 2), (i, res.max())
+    rng = np.random.default_rng(50 + group)
+    for i in rng.choice(16, 2, replace=False):
         ref = solvers[i].eigs(nw[i], ns[i])[0]
-        assert _rel_err(f ** 2, ref ** 2) < 1e-9, i
+        assert _rel_err(out[i][0] ** 2, ref ** 2) < 1e-9, i
     for S in solvers:
         S.close()

@@ -61,4 +61,17 @@ def test_fcc_path_vs_oracle_fixture(gpu, nx, cases):
         ores = np.array([c["rel_residual"] for c in cases])
         assert np.all(res < np.maximum(10 * ores, 1e-6 * lam)), (res, ores)
         del E
+    # two inverse-iteration sweeps of the Ritz pairs bring the field residual
+    # down to the FP64 floor, a small multiple of eps ||C^H C||
+    S.set_refinement(2, 1e-16, 6000)
+    bound = 100 * 2.220446049250313e-16 * float(sum((2.0 / h) ** 2 for h in fl.mesh_len))
+    for prec in (F.PREC_F64, F.PREC_F32_REFINE):
+        freq, _, E, prof = S.eigs(nwant, nstep, tol=cases[0]["tol"], precision=prec, want_fields=True)
+        err = np.abs(freq ** 2 - ref) / ref
+        res = S.residuals(freq, E)
+        print(f"  refined, precision {prec}: eigenvalue error {err.max():.2e}, residual {res.max():.2e} "
+              f"(bound {bound:.2e})", flush=True)
+        assert err.max() < 1e-8
+        assert np.all(res <= bound), (prec, res.max(), bound)
+        del E
     S.close()
