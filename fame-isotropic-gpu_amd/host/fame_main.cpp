@@ -153,6 +153,11 @@ extern "C" int fame_main_code_run(const PAR* ParP, PROFILE* Profile) {
     if (st != FAME_OK) { std::fprintf(stderr, "FAME_Main_Code: %s\n", fame_last_error()); return st; }
     st = fame_set_eps(ctx, eps.data());
     if (st != FAME_OK) { fame_destroy(ctx); return st; }
+    /* optional inverse-iteration sweeps of the converged Ritz pairs (no
+     * counterpart in the reference; 0 keeps its algorithm) */
+    if (const char* rs = std::getenv("FAME_RITZ_SWEEPS")) {
+        if ((st = fame_set_refinement(ctx, std::atoi(rs), 1e-16, 6000)) != FAME_OK) { fame_destroy(ctx); return st; }
+    }
 
     std::vector<double> freq_all((size_t)Nk * Nwant);
     void* E_dev = nullptr;

@@ -26,3 +26,19 @@ def make(name):
     fl = F.to_fame_lattice(lo)
     eps = F.material_eps(lo, mat, [ein], 1.0)
     return lo, fl, O.lattice_from(fl), eps
+
+
+def equal_mesh_general(shifts=(0, 0, 0, 0), n=8, seed=5):
+    """A General-lattice grid whose three mesh lengths are equal (n^3 cells of
+    h = 1/n, the given boundary shifts m1..m4, a monoclinic cell's boundary
+    translations) with a seeded permittivity in [1, 13] per Yee edge.  On such
+    grids Lambda is parallel to (1, 1, 1) at some nonzero modes (at Gamma with
+    these shifts), where the reference's General Pi normalisers are 0/0."""
+    lo = F.build_lattice("primitive_monoclinic", n, a=1.0, b=1.0, c=1 / np.sin(1.4), alpha=1.4)
+    fl = F.to_fame_lattice(lo)
+    assert fl.general == 1 and list(fl.grid_nums) == [n, n, n]
+    for i in range(3):
+        fl.mesh_len[i] = 1.0 / n
+    fl.m1, fl.m2, fl.m3, fl.m4 = shifts
+    eps = np.random.default_rng(seed).uniform(1.0, 13.0, 3 * n ** 3)
+    return fl, O.lattice_from(fl), eps

@@ -16,7 +16,7 @@ import pytest
 
 import fame_amd as F
 import oracle as O
-from cases import KPTS, STRUCTS, make
+from cases import KPTS, STRUCTS, equal_mesh_general, make
 
 pytestmark = pytest.mark.gpu
 
@@ -370,3 +370,32 @@ def test_general_gauge_on_modes_parallel_to_111(gpu):
     og.general = 1
     fo, _, _ = O.eigs(og, eps, k, 8, 24)
     assert np.max(np.abs(out[1] ** 2 - fo ** 2) / fo ** 2) < 1e-9
+
+
+@pytest.mark.parametrize("shifts", [(0, 0, 0, 0), (0, 2, 0, 2)])
+def test_operators_on_equal_mesh_general_grid(gpu, engine, shifts):
+    """A General grid whose three mesh lengths are equal takes the guarded Pi
+    basis in the inner kernels; at Gamma some of its modes have Lambda exactly
+    parallel to (1, 1, 1).  Operators in FP64 (1e-11) and the FP32 QBQ (2e-6)
+    against the oracle, at Gamma and at a generic wave vector."""
+    import torch
+    fl, ol, eps = equal_mesh_general(shifts, n=16)
+    ks = np.stack([np.zeros(3), np.array([0.21, -0.13, 0.07])])
+    S = F.Solver(fl, max_batch=2)
+    S.set_eps(eps)
+    nd = S.set_wave_vectors(ks)
+    invB = 1.0 / eps
+    for ik, k in enumerate(ks):
+        t = O.Tables(ol, k)
+        assert nd[ik] == t.Nd
+        x = _rand(2 * t.Nd, 10 + ik)
+        v = _rand(3 * t.N, 20 + ik)
+        xd = torch.tensor(x, device=gpu)
+        vd = torch.tensor(v, device=gpu)
+        assert _rel(S.apply(F.OP_QR, ik, xd).cpu().numpy(), t.qr(x)) < 1e-11
+        assert _rel(S.apply(F.OP_PR, ik, xd).cpu().numpy(), t.pr(x)) < 1e-11
+        assert _rel(S.apply(F.OP_QRS, ik, vd).cpu().numpy(), t.qrs(v)) < 1e-11
+        assert _rel(S.apply(F.OP_PRS, ik, vd).cpu().numpy(), t.prs(v)) < 1e-11
+        assert _rel(S.apply(F.OP_QBQ, ik, xd).cpu().numpy(), t.qbq(invB, x)) < 1e-11
+        assert _rel(S.apply(F.OP_QBQ32, ik, xd).cpu().numpy(), t.qbq(invB, x)) < 2e-6
+    S.close()

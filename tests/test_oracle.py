@@ -5,6 +5,7 @@ What each check can catch:
   FFT curl vs sparse curl           -> wrong Lambda formula, D twiddle, layout or Gamma rule
   C = P Sigma Q^H, Qrs Qr = I       -> wrong Pi_Pr / Pi_Qrs permutation or Sigma placement
   Pi orthonormal, Lambda^H Pi = 0   -> a dropped term or conjugate in the Pi formulas
+  equal mesh lengths, Lambda || (1,1,1) -> the General gauge's 0/0 modes (guarded basis)
   homogeneous QBQ = I / eps         -> a missing 1/N or 1/eps
   tridiagonal eigensolver vs LAPACK -> ordering, eigenvector columns
   GVqrrq vs a dense shifted QR step -> Givens sign convention, symmetrisation
@@ -21,7 +22,7 @@ import scipy.linalg as sl
 import scipy.sparse as sp
 
 import oracle as O
-from cases import KPTS, STRUCTS, make
+from cases import KPTS, STRUCTS, equal_mesh_general, make
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -179,3 +180,66 @@ def test_cg_and_invar_solve(sname):
     sig = np.tile(t.get(3), 2)
     y, _ = t.invar(invB, b, 2000, 1e-13)
     assert np.linalg.norm(sig * t.qbq(invB, sig * y) - b) <= 1e-10 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("shifts", [(0, 0, 0, 0), (0, 2, 0, 2)])
+@pytest.mark.parametrize("k", [(0.0, 0.0, 0.0), (0.3, 0.3, 0.3), (0.21, -0.13, 0.07)])
+def test_pi_gauge_where_lambda_is_parallel_to_ones(shifts, k):
+    """General grid with three equal mesh lengths: at Gamma some nonzero modes
+    have Lambda exactly parallel to (1, 1, 1), where the reference's General
+    normalisers (FAME_Matrix_Lambdas.cu:441-461) are 0/0; at (0.3, 0.3, 0.3)
+    some are nearly parallel.  The oracle's basis must stay orthonormal and
+    orthogonal to Lambda there, and the factorisation C = P Sigma Q^H hold."""
+    fl, ol, eps = equal_mesh_general(shifts)
+    k = np.array(k)
+    t = O.Tables(ol, k)
+    Lx, Ly, Lz = t.get(0), t.get(1), t.get(2)
+    L = np.stack([Lx, Ly, Lz])
+    Lq = np.sum(np.abs(L) ** 2, 0)
+    d = np.abs(Lz - Ly) ** 2 + np.abs(Lx - Lz) ** 2 + np.abs(Ly - Lx) ** 2
+    if not k.any():
+        assert np.sum(d <= 1e-30 * Lq) >= 1          # the degenerate modes are present
+    Pi = t.get(4).reshape(6, t.Nd)
+    assert np.isfinite(Pi).all()
+    P1, P2 = Pi[0:3], Pi[3:6]
+    # Away from the exact 0/0 the oracle keeps the reference's General formulas,
+    # whose normalisers cancel to |d|^2 / Lq: their rounding grows as eps Lq / |d|^2.
+    # The exactly parallel modes take the Simple gauge, accurate to rounding.
+    guarded = d <= 1e-30 * Lq
+    tol = 1e-13 + np.where(guarded, 0.0, 50 * 2.2e-16 * Lq / np.maximum(d, 1e-300))
+    assert np.all(np.abs(np.sum(np.abs(P1) ** 2, 0) - 1) < tol)
+    assert np.all(np.abs(np.sum(np.abs(P2) ** 2, 0) - 1) < tol)
+    assert np.all(np.abs(np.sum(P1.conj() * P2, 0)) < tol)
+    assert np.all(np.abs(np.sum(L.conj() * P1, 0) / np.sqrt(Lq)) < tol)
+    assert np.all(np.abs(np.sum(L.conj() * P2, 0) / np.sqrt(Lq)) < tol)
+    N = t.N
+    r, c, v = O.curl_coo(ol, k)
+    C = sp.coo_matrix((v, (r, c)), shape=(3 * N, 3 * N)).tocsr()
+    E = _rand(3 * N, 1)
+    CE = C @ E
+    q = t.qrs(E)
+    sig = t.get(3)
+    q[: t.Nd] *= sig
+    q[t.Nd:] *= sig
+    assert np.abs(t.pr(q) - CE).max() <= 1e-12 * np.abs(CE).max()
+    x = _rand(2 * t.Nd, 2)
+    # the nearly parallel modes' basis is orthonormal only to their tolerance above
+    assert np.abs(t.qrs(t.qr(x)) - x).max() <= 2e-11 * np.abs(x).max()
+
+
+def test_eigs_where_lambda_is_parallel_to_ones():
+    """The oracle's eigenvalues on the equal-mesh General grid at Gamma (with
+    exactly degenerate Pi modes) against a dense generalized eigensolve of
+    C^H C E = lambda B E with the sparse curl."""
+    fl, ol, eps = equal_mesh_general((0, 2, 0, 2), n=6)
+    k = np.zeros(3)
+    nwant = 8
+    freq, _, prof = O.eigs(ol, eps, k, nwant, 24)
+    N = ol.N[0] * ol.N[1] * ol.N[2]
+    r, c, v = O.curl_coo(ol, k)
+    C = sp.coo_matrix((v, (r, c)), shape=(3 * N, 3 * N)).toarray()
+    w = sl.eigh(C.conj().T @ C, np.diag(eps), eigvals_only=True)
+    w = np.sort(w[w > 1e-8 * w.max()])
+    lam = freq ** 2
+    assert np.all(lam[:2] == 0)
+    assert np.max(np.abs(lam[2:] - w[: nwant - 2]) / w[: nwant - 2]) < 1e-10

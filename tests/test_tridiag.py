@@ -44,7 +44,7 @@ def _tri(d, e):
     return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 12, 30, 48, 73])
+@pytest.mark.parametrize("n", [1, 2, 5, 12, 30, 48, 73, 127])
 def test_tridiag_eig_vs_lapack(n):
     rng = np.random.default_rng(100 + n)
     d = rng.uniform(1.0, 2.0, n)
@@ -84,7 +84,7 @@ def _dense_qr_step(T, mu):
     return Q, R, R @ Q + mu * np.eye(n)
 
 
-@pytest.mark.parametrize("n", [2, 3, 9, 30])
+@pytest.mark.parametrize("n", [2, 3, 9, 30, 127])
 def test_qr_step_matches_dense_shifted_qr(n):
     rng = np.random.default_rng(n)
     d = rng.uniform(1, 3, n)
