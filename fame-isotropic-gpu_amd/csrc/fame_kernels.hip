@@ -31,6 +31,10 @@ template <typename T> struct Inflight { static constexpr int v = sizeof(T) == 4 
 template <typename T> struct InflightEps { static constexpr int v = sizeof(T) == 4 ? 4 : 2; };
 template <typename T> struct InflightInner { static constexpr int v = 2; };
 
+/* vector-access preconditions (a null pointer is never dereferenced) */
+__device__ __forceinline__ bool aligned16(const void* q) { return ((uintptr_t)q & 15) == 0; }
+__device__ __forceinline__ bool aligned8(const void* q) { return ((uintptr_t)q & 7) == 0; }
+
 template <typename T2> __device__ __forceinline__ T2 from_d(double2 v);
 template <> __device__ __forceinline__ double2 from_d<double2>(double2 v) { return v; }
 template <> __device__ __forceinline__ float2 from_d<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
@@ -138,83 +142,148 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         }
         struct XIn { T2 x1, x2, r1, r2, s1, s2, a1, a2; };
         double rnorm = 0.0;   /* <r,r> of the updated residual, this thread's share */
-        batched_loop<InflightInner<T>::v>(
-            L * Wd,
-            [&](int idx) -> XIn {
-                const int line = udiv(idx, mW), a = idx - line * Wd;
-                const int gl = l0 + line;
-                XIn d;
-                if (gl < nlines && a < na) {
-                    const unsigned f = (unsigned)(gl * na + a);
-                    d.x1 = X[f]; d.x2 = X[N + f];
-                    if (FX) {
-                        const float2 u1 = FX32[f], u2 = FX32[N + f];
-                        d.s1.x = u1.x; d.s1.y = u1.y; d.s2.x = u2.x; d.s2.y = u2.y;
-                        if (fpend) {
-                            const float2 v1 = FP32[f], v2 = FP32[N + f];
-                            d.a1.x = v1.x; d.a1.y = v1.y; d.a2.x = v2.x; d.a2.y = v2.y;
-                        }
-                    }
-                    if (cgf) {
-                        d.r1 = R[f]; d.r2 = R[N + f];
-                        if (pend) { d.s1 = Xs[f]; d.s2 = Xs[N + f]; }
-                        if (rup) { d.a1 = AP[f]; d.a2 = AP[N + f]; }
-                    }
+        /* one mode (line, a) at offset f: the CG / refinement updates of the
+         * reduced vectors, then the Pi expansion into the three LDS lines */
+        auto expand_mode = [&](int line, int a, unsigned f, const XIn& d) {
+            T2 xv1 = d.x1, xv2 = d.x2;
+            if (FX) {
+                float2 u1 = make_float2((float)d.s1.x, (float)d.s1.y), u2 = make_float2((float)d.s2.x, (float)d.s2.y);
+                if (fpend) {
+                    const float2 v1 = make_float2((float)d.a1.x, (float)d.a1.y);
+                    const float2 v2 = make_float2((float)d.a2.x, (float)d.a2.y);
+                    u1.x += far * v1.x - fai * v1.y; u1.y += far * v1.y + fai * v1.x;
+                    u2.x += far * v2.x - fai * v2.y; u2.y += far * v2.y + fai * v2.x;
                 }
-                return d;
-            },
-            [&](int idx, XIn d) {
-                const int line = udiv(idx, mW), a = idx - line * Wd;
-                const int gl = l0 + line;
-                double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
-                if (gl < nlines && a < na) {
-                    const unsigned f = (unsigned)(gl * na + a);
-                    T2 xv1 = d.x1, xv2 = d.x2;
-                    if (FX) {
-                        float2 u1 = make_float2((float)d.s1.x, (float)d.s1.y), u2 = make_float2((float)d.s2.x, (float)d.s2.y);
-                        if (fpend) {
-                            const float2 v1 = make_float2((float)d.a1.x, (float)d.a1.y);
-                            const float2 v2 = make_float2((float)d.a2.x, (float)d.a2.y);
-                            u1.x += far * v1.x - fai * v1.y; u1.y += far * v1.y + fai * v1.x;
-                            u2.x += far * v2.x - fai * v2.y; u2.y += far * v2.y + fai * v2.x;
-                        }
-                        xv1.x += (T)u1.x; xv1.y += (T)u1.y; xv2.x += (T)u2.x; xv2.y += (T)u2.y;
-                        Pw[f] = xv1; Pw[N + f] = xv2;
-                    }
-                    if (cgf) {
-                        if (pend) {
-                            T2 s1 = d.s1, s2 = d.s2;
-                            s1.x += ar * xv1.x - ai * xv1.y; s1.y += ar * xv1.y + ai * xv1.x;
-                            s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
-                            Xs[f] = s1; Xs[N + f] = s2;
-                        }
-                        T2 r1 = d.r1, r2 = d.r2;
-                        if (rup) {
-                            r1.x -= ar * d.a1.x - ai * d.a1.y; r1.y -= ar * d.a1.y + ai * d.a1.x;
-                            r2.x -= ar * d.a2.x - ai * d.a2.y; r2.y -= ar * d.a2.y + ai * d.a2.x;
-                            T2* Rw = const_cast<T2*>(R);
-                            Rw[f] = r1; Rw[N + f] = r2;
-                        }
-                        rnorm += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
-                        xv1.x = r1.x + beta * xv1.x; xv1.y = r1.y + beta * xv1.y;
-                        xv2.x = r2.x + beta * xv2.x; xv2.y = r2.y + beta * xv2.y;
-                        Pw[f] = xv1; Pw[N + f] = xv2;
-                    }
-                    if (!(pr.gamma && f == 0)) {
-                        double2 lx, ly, lz, p1[3], p2[3];
-                        lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                        pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
-                        const double2 x1 = to_d(xv1), x2 = to_d(xv2);
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) {
-                            if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
-                            else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
-                        }
-                    }
+                xv1.x += (T)u1.x; xv1.y += (T)u1.y; xv2.x += (T)u2.x; xv2.y += (T)u2.y;
+                Pw[f] = xv1; Pw[N + f] = xv2;
+            }
+            if (cgf) {
+                if (pend) {
+                    T2 s1 = d.s1, s2 = d.s2;
+                    s1.x += ar * xv1.x - ai * xv1.y; s1.y += ar * xv1.y + ai * xv1.x;
+                    s2.x += ar * xv2.x - ai * xv2.y; s2.y += ar * xv2.y + ai * xv2.x;
+                    Xs[f] = s1; Xs[N + f] = s2;
                 }
+                T2 r1 = d.r1, r2 = d.r2;
+                if (rup) {
+                    r1.x -= ar * d.a1.x - ai * d.a1.y; r1.y -= ar * d.a1.y + ai * d.a1.x;
+                    r2.x -= ar * d.a2.x - ai * d.a2.y; r2.y -= ar * d.a2.y + ai * d.a2.x;
+                    T2* Rw = const_cast<T2*>(R);
+                    Rw[f] = r1; Rw[N + f] = r2;
+                }
+                rnorm += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
+                xv1.x = r1.x + beta * xv1.x; xv1.y = r1.y + beta * xv1.y;
+                xv2.x = r2.x + beta * xv2.x; xv2.y = r2.y + beta * xv2.y;
+                Pw[f] = xv1; Pw[N + f] = xv2;
+            }
+            double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
+            if (!(pr.gamma && f == 0)) {
+                double2 lx, ly, lz, p1[3], p2[3];
+                lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
+                const double2 x1 = to_d(xv1), x2 = to_d(xv2);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
-            });
+                for (int c = 0; c < 3; ++c) {
+                    if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
+                    else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(u[c]);
+        };
+        /* FP32 without Bluestein and with na even: two consecutive modes of a
+         * line per thread, each vector moved as one 16-byte access (a line
+         * starts at an even offset, and every vector at a 16-byte boundary) */
+        bool pairs = false;
+        if constexpr (sizeof(T) == 4)
+            pairs = !pl.blue && !(na & 1) && !FX && aligned16(X) && aligned16(R) && aligned16(Xs) && aligned16(AP) &&
+                    aligned16(F);
+        if constexpr (sizeof(T) == 4) {
+            if (pairs) {
+                struct XIn2 { float4 x1, x2, r1, r2, s1, s2, a1, a2; };
+                auto lo = [](const float4& v) { return make_float2(v.x, v.y); };
+                auto hi = [](const float4& v) { return make_float2(v.z, v.w); };
+                batched_loop<1>(
+                    (L * na) >> 1,
+                    [&](int idx) -> XIn2 {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        XIn2 d;
+                        if (l0 + line < nlines) {
+                            const unsigned f = (unsigned)((l0 + line) * na + a);
+                            d.x1 = *reinterpret_cast<const float4*>(X + f);
+                            d.x2 = *reinterpret_cast<const float4*>(X + N + f);
+                            if (cgf) {
+                                d.r1 = *reinterpret_cast<const float4*>(R + f);
+                                d.r2 = *reinterpret_cast<const float4*>(R + N + f);
+                                if (pend) {
+                                    d.s1 = *reinterpret_cast<const float4*>(Xs + f);
+                                    d.s2 = *reinterpret_cast<const float4*>(Xs + N + f);
+                                }
+                                if (rup) {
+                                    d.a1 = *reinterpret_cast<const float4*>(AP + f);
+                                    d.a2 = *reinterpret_cast<const float4*>(AP + N + f);
+                                }
+                            }
+                        }
+                        return d;
+                    },
+                    [&](int idx, const XIn2& d) {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        if (l0 + line < nlines) {
+                            const unsigned f = (unsigned)((l0 + line) * na + a);
+                            XIn e0, e1;
+                            e0.x1 = lo(d.x1); e0.x2 = lo(d.x2); e1.x1 = hi(d.x1); e1.x2 = hi(d.x2);
+                            e0.r1 = lo(d.r1); e0.r2 = lo(d.r2); e1.r1 = hi(d.r1); e1.r2 = hi(d.r2);
+                            e0.s1 = lo(d.s1); e0.s2 = lo(d.s2); e1.s1 = hi(d.s1); e1.s2 = hi(d.s2);
+                            e0.a1 = lo(d.a1); e0.a2 = lo(d.a2); e1.a1 = hi(d.a1); e1.a2 = hi(d.a2);
+                            expand_mode(line, a, f, e0);
+                            expand_mode(line, a + 1, f + 1, e1);
+                        } else {
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) {
+                                S[(c * L + line) * P + a] = make_float2(0.f, 0.f);
+                                S[(c * L + line) * P + a + 1] = make_float2(0.f, 0.f);
+                            }
+                        }
+                    });
+            }
+        }
+        if (!pairs)
+            batched_loop<InflightInner<T>::v>(
+                L * Wd,
+                [&](int idx) -> XIn {
+                    const int line = udiv(idx, mW), a = idx - line * Wd;
+                    const int gl = l0 + line;
+                    XIn d;
+                    if (gl < nlines && a < na) {
+                        const unsigned f = (unsigned)(gl * na + a);
+                        d.x1 = X[f]; d.x2 = X[N + f];
+                        if (FX) {
+                            const float2 u1 = FX32[f], u2 = FX32[N + f];
+                            d.s1.x = u1.x; d.s1.y = u1.y; d.s2.x = u2.x; d.s2.y = u2.y;
+                            if (fpend) {
+                                const float2 v1 = FP32[f], v2 = FP32[N + f];
+                                d.a1.x = v1.x; d.a1.y = v1.y; d.a2.x = v2.x; d.a2.y = v2.y;
+                            }
+                        }
+                        if (cgf) {
+                            d.r1 = R[f]; d.r2 = R[N + f];
+                            if (pend) { d.s1 = Xs[f]; d.s2 = Xs[N + f]; }
+                            if (rup) { d.a1 = AP[f]; d.a2 = AP[N + f]; }
+                        }
+                    }
+                    return d;
+                },
+                [&](int idx, const XIn& d) {
+                    const int line = udiv(idx, mW), a = idx - line * Wd;
+                    const int gl = l0 + line;
+                    if (gl < nlines && a < na) {
+                        expand_mode(line, a, (unsigned)(gl * na + a), d);
+                    } else {
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = from_d<T2>(make_double2(0, 0));
+                    }
+                });
         /* inside CG with the stored Ap: block sum of <r,r> into slot 5 of
          * partials[p][tile] (the contracting pass fills slots 0..2); with the
          * residual updated by the contracting pass, that pass sums <r,r> */
@@ -232,129 +301,246 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             args.partials[((size_t)p * nblk + tile) * 6 + 5] = sum;
         }
         run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
-        for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
-            const int line = udiv(idx, g.m_na), a = idx - line * na;
-            const int gl = l0 + line;
-            if (gl >= nlines) continue;
-            const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
-            const unsigned f = (unsigned)(gl * na + a);
-#pragma unroll
-            for (int c = 0; c < 3; ++c)
-                F[c * N + f] = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w);
-        }
-    } else {
-        struct FIn { T2 v[3]; };
-        batched_loop<InflightInner<T>::v>(
-            L * Wd,
-            [&](int idx) -> FIn {
-                const int line = udiv(idx, mW), a = idx - line * Wd;
-                const int gl = l0 + line;
-                FIn d;
-#pragma unroll
-                for (int c = 0; c < 3; ++c) { d.v[c].x = 0; d.v[c].y = 0; }
-                if (gl < nlines && a < na) {
+        if constexpr (sizeof(T) == 4) {
+            if (pairs)
+                for (int idx = threadIdx.x; idx < ((L * na) >> 1); idx += FFT_THREADS) {
+                    const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                    const int gl = l0 + line;
+                    if (gl >= nlines) continue;
+                    /* a even: a and a + 1 share the high twiddle factor */
+                    const float2 th = twt[line * tpl + 16 + (a >> 4)];
+                    const float2 w0 = cmul(th, twt[line * tpl + (a & 15)]), w1 = cmul(th, twt[line * tpl + (a & 15) + 1]);
                     const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) d.v[c] = F[c * N + f];
+                    for (int c = 0; c < 3; ++c) {
+                        const float2 v0 = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w0);
+                        const float2 v1 = cmul(post_mul<T2, T>(pl, tb, a + 1, S[(c * L + line) * P + a + 1]), w1);
+                        *reinterpret_cast<float4*>(F + c * N + f) = make_float4(v0.x, v0.y, v1.x, v1.y);
+                    }
                 }
-                return d;
-            },
-            [&](int idx, FIn d) {
-                const int line = udiv(idx, mW), a = idx - line * Wd;
+        }
+        if (!pairs)
+            for (int idx = threadIdx.x; idx < L * na; idx += FFT_THREADS) {
+                const int line = udiv(idx, g.m_na), a = idx - line * na;
                 const int gl = l0 + line;
-                if (gl < nlines && a < na) {
-                    const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
+                if (gl >= nlines) continue;
+                const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
+                const unsigned f = (unsigned)(gl * na + a);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) d.v[c] = cmulc(d.v[c], w);
-                }
-#pragma unroll
-                for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = d.v[c];
-            });
-        __syncthreads();
-        run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
+                for (int c = 0; c < 3; ++c)
+                    F[c * N + f] = cmul(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]), w);
+            }
+    } else {
+        /* FP32 without Bluestein, na even and 16-byte aligned vectors: two
+         * consecutive modes of a line per thread, as in the expanding pass */
         T2* Y = reinterpret_cast<T2*>(args.y) + (size_t)p * args.xstride;
         const T2* D = args.dotvec ? reinterpret_cast<const T2*>(args.dotvec) + (size_t)p * args.xstride : nullptr;
+        const bool RU = args.cg_r != nullptr;
+        T2* const RW = RU ? reinterpret_cast<T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
+        const bool RB = args.rb != nullptr;
+        bool pairs = false;
+        if constexpr (sizeof(T) == 4)
+            pairs = !pl.blue && !(na & 1) && !RB && aligned16(F) && aligned16(Y) && aligned16(D) && aligned16(RW);
+        auto lo = [](const float4& v) { return make_float2(v.x, v.y); };
+        auto hi = [](const float4& v) { return make_float2(v.z, v.w); };
+        struct FIn { T2 v[3]; };
+        if constexpr (sizeof(T) == 4) {
+            if (pairs) {
+                struct FIn2 { float4 v[3]; };
+                batched_loop<1>(
+                    (L * na) >> 1,
+                    [&](int idx) -> FIn2 {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        FIn2 d;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) d.v[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        if (l0 + line < nlines) {
+                            const unsigned f = (unsigned)((l0 + line) * na + a);
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) d.v[c] = *reinterpret_cast<const float4*>(F + c * N + f);
+                        }
+                        return d;
+                    },
+                    [&](int idx, const FIn2& d) {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        float2 w0 = make_float2(1.f, 0.f), w1 = w0;
+                        if (l0 + line < nlines) {
+                            const float2 th = twt[line * tpl + 16 + (a >> 4)];
+                            w0 = cmul(th, twt[line * tpl + (a & 15)]);
+                            w1 = cmul(th, twt[line * tpl + (a & 15) + 1]);
+                        }
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) {
+                            S[(c * L + line) * P + a] = cmulc(lo(d.v[c]), w0);
+                            S[(c * L + line) * P + a + 1] = cmulc(hi(d.v[c]), w1);
+                        }
+                    });
+            }
+        }
+        if (!pairs)
+            batched_loop<InflightInner<T>::v>(
+                L * Wd,
+                [&](int idx) -> FIn {
+                    const int line = udiv(idx, mW), a = idx - line * Wd;
+                    const int gl = l0 + line;
+                    FIn d;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) { d.v[c].x = 0; d.v[c].y = 0; }
+                    if (gl < nlines && a < na) {
+                        const unsigned f = (unsigned)(gl * na + a);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) d.v[c] = F[c * N + f];
+                    }
+                    return d;
+                },
+                [&](int idx, FIn d) {
+                    const int line = udiv(idx, mW), a = idx - line * Wd;
+                    const int gl = l0 + line;
+                    if (gl < nlines && a < na) {
+                        const T2 w = cmul(twt[line * tpl + 16 + (a >> 4)], twt[line * tpl + (a & 15)]);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) d.v[c] = cmulc(d.v[c], w);
+                    }
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) S[(c * L + line) * P + a] = d.v[c];
+                });
+        __syncthreads();
+        run_stages<T2, MaxE<T>::v, ENG>(S, 3 * L, P, pl, tb, NoMid());
         double dre = 0.0, dim = 0.0, yy = 0.0, rn = 0.0;
         const bool CGD = args.cgdots != 0;
         /* inside CG with cg_r set: r -= alpha y (alpha of the problem's CgScal)
          * is stored in place of y, with <r,r> of the updated residual */
-        const bool RU = args.cg_r != nullptr;
-        T2* const RW = RU ? reinterpret_cast<T2*>(args.cg_r) + (size_t)p * args.xstride : nullptr;
         T ar = 0, ai = 0;
         if (RU) { ar = (T)args.cgsc[p].alpha_re; ai = (T)args.cgsc[p].alpha_im; }
         /* refinement residual: rb - y becomes the FP32 start r32 = p32, x32 = 0 */
-        const bool RB = args.rb != nullptr;
         const T2* const RBv = RB ? reinterpret_cast<const T2*>(args.rb) + (size_t)p * args.xstride : nullptr;
         float2* const R32 = RB ? reinterpret_cast<float2*>(args.r32) + (size_t)p * args.xstride : nullptr;
         float2* const P32 = RB ? reinterpret_cast<float2*>(args.p32) + (size_t)p * args.xstride : nullptr;
         float2* const X32 = RB ? reinterpret_cast<float2*>(args.x32) + (size_t)p * args.xstride : nullptr;
-        struct DIn { T2 d1, d2, r1, r2; };
-        batched_loop<InflightInner<T>::v>(
-            L * na,
-            [&](int idx) -> DIn {
-                const int line = udiv(idx, g.m_na), a = idx - line * na;
-                const int gl = l0 + line;
-                DIn d;
-                if (gl < nlines) {
-                    const unsigned f = (unsigned)(gl * na + a);
-                    if (D) { d.d1 = D[f]; d.d2 = D[N + f]; }
-                    if (RU) { d.r1 = RW[f]; d.r2 = RW[N + f]; }
-                    if (RB) { d.r1 = RBv[f]; d.r2 = RBv[N + f]; }
-                }
-                return d;
-            },
-            [&](int idx, DIn dd) {
-                const int line = udiv(idx, g.m_na), a = idx - line * na;
-                const int gl = l0 + line;
-                if (gl >= nlines) return;
-                const unsigned f = (unsigned)(gl * na + a);
-                double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
-                if (!(pr.gamma && f == 0)) {
-                    double2 lx, ly, lz, p1[3], p2[3];
-                    lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                    pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
+        /* Pi^H contraction of mode (line, a) at offset f from the three LDS lines */
+        auto contract = [&](int line, int a, unsigned f, T2& o1, T2& o2) {
+            double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
+            if (!(pr.gamma && f == 0)) {
+                double2 lx, ly, lz, p1[3], p2[3];
+                lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
+                pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
-                        if (args.pikind == 0) {
-                            y1 = cadd(y1, cmul(cconj(p1[c]), w));
-                            y2 = cadd(y2, cmul(cconj(p2[c]), w));
-                        } else {
-                            y1 = csub(y1, cmul(p2[c], w));
-                            y2 = cadd(y2, cmul(p1[c], w));
+                for (int c = 0; c < 3; ++c) {
+                    const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
+                    if (args.pikind == 0) {
+                        y1 = cadd(y1, cmul(cconj(p1[c]), w));
+                        y2 = cadd(y2, cmul(cconj(p2[c]), w));
+                    } else {
+                        y1 = csub(y1, cmul(p2[c], w));
+                        y2 = cadd(y2, cmul(p1[c], w));
+                    }
+                }
+            }
+            o1 = from_d<T2>(y1); o2 = from_d<T2>(y2);
+        };
+        /* dot partials of one mode: conj(d) . y and y . y */
+        auto dots = [&](const T2& o1, const T2& o2, const T2& e1, const T2& e2) {
+            if (D || CGD) {
+                const double2 q1 = to_d(o1), q2 = to_d(o2);
+                if (D) {
+                    const double2 d1 = to_d(e1), d2 = to_d(e2);
+                    dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
+                    dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                }
+                if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
+            }
+        };
+        /* r -= alpha y of one mode, with its <r,r> */
+        auto rstep = [&](T2& r1, T2& r2, const T2& o1, const T2& o2) {
+            r1.x -= ar * o1.x - ai * o1.y; r1.y -= ar * o1.y + ai * o1.x;
+            r2.x -= ar * o2.x - ai * o2.y; r2.y -= ar * o2.y + ai * o2.x;
+            rn += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
+        };
+        if constexpr (sizeof(T) == 4) {
+            if (pairs) {
+                struct DIn2 { float4 d1, d2, r1, r2; };
+                batched_loop<1>(
+                    (L * na) >> 1,
+                    [&](int idx) -> DIn2 {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        DIn2 d;
+                        if (l0 + line < nlines) {
+                            const unsigned f = (unsigned)((l0 + line) * na + a);
+                            if (D) {
+                                d.d1 = *reinterpret_cast<const float4*>(D + f);
+                                d.d2 = *reinterpret_cast<const float4*>(D + N + f);
+                            }
+                            if (RU) {
+                                d.r1 = *reinterpret_cast<const float4*>(RW + f);
+                                d.r2 = *reinterpret_cast<const float4*>(RW + N + f);
+                            }
                         }
+                        return d;
+                    },
+                    [&](int idx, const DIn2& dd) {
+                        const int line = udiv(2 * idx, g.m_na), a = 2 * idx - line * na;
+                        if (l0 + line >= nlines) return;
+                        const unsigned f = (unsigned)((l0 + line) * na + a);
+                        float2 o1a, o2a, o1b, o2b;
+                        contract(line, a, f, o1a, o2a);
+                        contract(line, a + 1, f + 1, o1b, o2b);
+                        if (RU) {
+                            float2 r1a = lo(dd.r1), r2a = lo(dd.r2), r1b = hi(dd.r1), r2b = hi(dd.r2);
+                            rstep(r1a, r2a, o1a, o2a);
+                            rstep(r1b, r2b, o1b, o2b);
+                            *reinterpret_cast<float4*>(RW + f) = make_float4(r1a.x, r1a.y, r1b.x, r1b.y);
+                            *reinterpret_cast<float4*>(RW + N + f) = make_float4(r2a.x, r2a.y, r2b.x, r2b.y);
+                        } else {
+                            *reinterpret_cast<float4*>(Y + f) = make_float4(o1a.x, o1a.y, o1b.x, o1b.y);
+                            *reinterpret_cast<float4*>(Y + N + f) = make_float4(o2a.x, o2a.y, o2b.x, o2b.y);
+                        }
+                        dots(o1a, o2a, lo(dd.d1), lo(dd.d2));
+                        dots(o1b, o2b, hi(dd.d1), hi(dd.d2));
+                    });
+            }
+        }
+        struct DIn { T2 d1, d2, r1, r2; };
+        if (!pairs)
+            batched_loop<InflightInner<T>::v>(
+                L * na,
+                [&](int idx) -> DIn {
+                    const int line = udiv(idx, g.m_na), a = idx - line * na;
+                    const int gl = l0 + line;
+                    DIn d;
+                    if (gl < nlines) {
+                        const unsigned f = (unsigned)(gl * na + a);
+                        if (D) { d.d1 = D[f]; d.d2 = D[N + f]; }
+                        if (RU) { d.r1 = RW[f]; d.r2 = RW[N + f]; }
+                        if (RB) { d.r1 = RBv[f]; d.r2 = RBv[N + f]; }
                     }
-                }
-                const T2 o1 = from_d<T2>(y1), o2 = from_d<T2>(y2);
-                if (RU) {
-                    T2 r1 = dd.r1, r2 = dd.r2;
-                    r1.x -= ar * o1.x - ai * o1.y; r1.y -= ar * o1.y + ai * o1.x;
-                    r2.x -= ar * o2.x - ai * o2.y; r2.y -= ar * o2.y + ai * o2.x;
-                    RW[f] = r1; RW[N + f] = r2;
-                    rn += (double)r1.x * r1.x + (double)r1.y * r1.y + (double)r2.x * r2.x + (double)r2.y * r2.y;
-                } else if (RB) {
-                    const double2 b1 = to_d(dd.r1), b2 = to_d(dd.r2), q1 = to_d(o1), q2 = to_d(o2);
-                    const float2 e1 = make_float2((float)(b1.x - q1.x), (float)(b1.y - q1.y));
-                    const float2 e2 = make_float2((float)(b2.x - q2.x), (float)(b2.y - q2.y));
-                    R32[f] = e1; R32[N + f] = e2;
-                    P32[f] = e1; P32[N + f] = e2;
-                    X32[f] = make_float2(0.0f, 0.0f); X32[N + f] = make_float2(0.0f, 0.0f);
-                    const float s1 = e1.x * e1.x + e1.y * e1.y, s2 = e2.x * e2.x + e2.y * e2.y;
-                    rn += (double)s1 + (double)s2;
-                } else {
-                    Y[f] = o1;
-                    Y[N + f] = o2;
-                }
-                if (D || CGD) {
-                    const double2 q1 = to_d(o1), q2 = to_d(o2);
-                    if (D) {
-                        const double2 d1 = to_d(dd.d1), d2 = to_d(dd.d2);
-                        dre += d1.x * q1.x + d1.y * q1.y + d2.x * q2.x + d2.y * q2.y;
-                        dim += d1.x * q1.y - d1.y * q1.x + d2.x * q2.y - d2.y * q2.x;
+                    return d;
+                },
+                [&](int idx, DIn dd) {
+                    const int line = udiv(idx, g.m_na), a = idx - line * na;
+                    const int gl = l0 + line;
+                    if (gl >= nlines) return;
+                    const unsigned f = (unsigned)(gl * na + a);
+                    T2 o1, o2;
+                    contract(line, a, f, o1, o2);
+                    if (RU) {
+                        T2 r1 = dd.r1, r2 = dd.r2;
+                        rstep(r1, r2, o1, o2);
+                        RW[f] = r1; RW[N + f] = r2;
+                    } else if (RB) {
+                        const double2 b1 = to_d(dd.r1), b2 = to_d(dd.r2), q1 = to_d(o1), q2 = to_d(o2);
+                        const float2 e1 = make_float2((float)(b1.x - q1.x), (float)(b1.y - q1.y));
+                        const float2 e2 = make_float2((float)(b2.x - q2.x), (float)(b2.y - q2.y));
+                        R32[f] = e1; R32[N + f] = e2;
+                        P32[f] = e1; P32[N + f] = e2;
+                        X32[f] = make_float2(0.0f, 0.0f); X32[N + f] = make_float2(0.0f, 0.0f);
+                        const float s1 = e1.x * e1.x + e1.y * e1.y, s2 = e2.x * e2.x + e2.y * e2.y;
+                        rn += (double)s1 + (double)s2;
+                    } else {
+                        Y[f] = o1;
+                        Y[N + f] = o2;
                     }
-                    if (CGD) yy += q1.x * q1.x + q1.y * q1.y + q2.x * q2.x + q2.y * q2.y;
-                }
-            });
+                    dots(o1, o2, dd.d1, dd.d2);
+                });
         if (D || CGD || RU || RB) {
             /* block sums into partials[p][tile]: conj(d) . y (slots 0, 1), and
              * inside CG y . y (slot 2) or <r,r> of the updated residual (slot
@@ -396,8 +582,16 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
  * faster on gfx950.  The others keep FFT_THREADS. */
 template <typename T, int ENG> struct MidThreads { static constexpr int v = (ENG == ENG_MFMA && sizeof(T) == 8) ? 512 : FFT_THREADS; };
 
+/* waves per SIMD the compiler must leave room for: the FP32 matrix-core
+ * middle kernel then holds 51 VGPRs and no AGPRs (68 + 16 otherwise), and the
+ * FP32 outer kernel 80 VGPRs (87), without spills; the FP32 outer pass ran 5%
+ * faster on gfx950 at 96x83x78 */
+template <typename T, int ENG> struct MidWaves { static constexpr int v = sizeof(T) == 4 ? (ENG == ENG_MFMA ? 6 : 5) : 1; };
+template <typename T> struct OuterWaves { static constexpr int v = sizeof(T) == 4 ? 6 : 1; };
+
 template <typename T, int DIR, int ENG>
-__global__ void __launch_bounds__((MidThreads<T, ENG>::v)) k_mid(KAxis args) {
+__global__ void __launch_bounds__((MidThreads<T, ENG>::v)) __attribute__((amdgpu_waves_per_eu(MidWaves<T, ENG>::v)))
+k_mid(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
@@ -420,6 +614,52 @@ __global__ void __launch_bounds__((MidThreads<T, ENG>::v)) k_mid(KAxis args) {
     for (int b = threadIdx.x; b < nb; b += blockDim.x) twb[b] = tw_expipi<T2>(phc * (double)b);
     __syncthreads();
     const int Wd = pl.blue ? pl.M : nb;
+    /* FP32 with L and na even: each thread moves two consecutive a of a row
+     * as one 16-byte access (every pair starts at an even element offset of a
+     * 16-byte aligned plane), halving the index arithmetic per element */
+    if constexpr (sizeof(T) == 4) {
+        const int Lh = L >> 1;
+        if (!(L & 1) && !(na & 1) && aligned16(F) && (int)blockDim.x % Lh == 0) {
+            /* each thread keeps one pair column ta and steps the rows b by
+             * blockDim / Lh: no division per element */
+            const int ta = 2 * ((int)threadIdx.x % Lh), rs = (int)blockDim.x / Lh, a = a0 + ta;
+            const bool aon = a < na;
+            const unsigned rowq = (unsigned)(na >> 1);   /* row pitch in 16-byte units */
+            const float4* Fa = reinterpret_cast<const float4*>(F + a);
+            constexpr int U = Inflight<T>::v / 2;
+            for (int b0 = (int)threadIdx.x / Lh; b0 < Wd; b0 += U * rs) {
+                float4 v[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int b = b0 + u * rs;
+                    v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (aon && b < nb) v[u] = Fa[(unsigned)b * rowq];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int b = b0 + u * rs;
+                    if (b < Wd) {
+                        float2 v0 = make_float2(v[u].x, v[u].y), v1 = make_float2(v[u].z, v[u].w);
+                        if (DIR < 0 && b < nb) { const float2 w = twb[b]; v0 = cmulc(v0, w); v1 = cmulc(v1, w); }
+                        S[ta * P + b] = v0;
+                        S[(ta + 1) * P + b] = v1;
+                    }
+                }
+            }
+            __syncthreads();
+            run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
+            if (aon) {
+                float4* Fo = reinterpret_cast<float4*>(F + a);
+                for (int b = (int)threadIdx.x / Lh; b < nb; b += rs) {
+                    float2 v0 = post_mul<T2, T>(pl, tb, b, S[ta * P + b]);
+                    float2 v1 = post_mul<T2, T>(pl, tb, b, S[(ta + 1) * P + b]);
+                    if (DIR > 0) { const float2 w = twb[b]; v0 = cmul(v0, w); v1 = cmul(v1, w); }
+                    Fo[(unsigned)b * rowq] = make_float4(v0.x, v0.y, v1.x, v1.y);
+                }
+            }
+            return;
+        }
+    }
     batched_loop<Inflight<T>::v>(
         L * Wd,
         [&](int idx) -> T2 {
@@ -456,7 +696,7 @@ __global__ void __launch_bounds__((MidThreads<T, ENG>::v)) k_mid(KAxis args) {
  * are conjugate unit factors whose product is 1; both are left out and only
  * the 1/M remains in the elementwise step.                                  */
 template <typename T, int MODE, int ENG>
-__global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
+__global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(OuterWaves<T>::v))) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
@@ -478,20 +718,48 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
     const int q0 = tile * L;
     T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N;
     const int Wd = pl.blue ? pl.M : nc;
-    batched_loop<Inflight<T>::v>(
-        L * Wd,
-        [&](int idx) -> T2 {
-            const int c = udiv(idx, args.mL), q = q0 + (idx - c * L);
-            T2 v;
-            v.x = 0; v.y = 0;
-            if (q < plane && c < nc) v = F[(unsigned)(c * plane + q)];
-            return v;
-        },
-        [&](int idx, T2 v) {
-            const int c = udiv(idx, args.mL), tq = idx - c * L;
-            if (MODE == OUTER_QRS && c < nc) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
-            S[tq * P + c] = v;
-        });
+    /* FP32 with L and the plane even: two consecutive q per 16-byte access (and
+     * per 8-byte load of 1/eps), as in the middle-axis kernel */
+    bool pairs = false;
+    const int Lh = L >> 1;
+    if constexpr (sizeof(T) == 4)
+        pairs = !(L & 1) && !(plane & 1) && aligned16(F) && (MODE != OUTER_EPS || aligned8(args.invB));
+    if constexpr (sizeof(T) == 4) {
+        if (pairs)
+            batched_loop<Inflight<T>::v / 2>(
+                Lh * Wd,
+                [&](int idx) -> float4 {
+                    const int c = udiv(2 * idx, args.mL), q = q0 + (2 * idx - c * L);
+                    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (q < plane && c < nc) v = *reinterpret_cast<const float4*>(F + (unsigned)(c * plane + q));
+                    return v;
+                },
+                [&](int idx, float4 v) {
+                    const int c = udiv(2 * idx, args.mL), tq = 2 * idx - c * L;
+                    float2 v0 = make_float2(v.x, v.y), v1 = make_float2(v.z, v.w);
+                    if (MODE == OUTER_QRS && c < nc) {
+                        const float2 w = tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc);
+                        v0 = cmulc(v0, w); v1 = cmulc(v1, w);
+                    }
+                    S[tq * P + c] = v0;
+                    S[(tq + 1) * P + c] = v1;
+                });
+    }
+    if (!pairs)
+        batched_loop<Inflight<T>::v>(
+            L * Wd,
+            [&](int idx) -> T2 {
+                const int c = udiv(idx, args.mL), q = q0 + (idx - c * L);
+                T2 v;
+                v.x = 0; v.y = 0;
+                if (q < plane && c < nc) v = F[(unsigned)(c * plane + q)];
+                return v;
+            },
+            [&](int idx, T2 v) {
+                const int c = udiv(idx, args.mL), tq = idx - c * L;
+                if (MODE == OUTER_QRS && c < nc) v = cmulc(v, tw_expipi<T2>(2.0 * pr.r0 * (double)c / (double)nc));
+                S[tq * P + c] = v;
+            });
     __syncthreads();
     if (MODE == OUTER_EPS) {
         const T* invB = reinterpret_cast<const T*>(args.invB) + (size_t)comp * g.N;
@@ -499,18 +767,38 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         const unsigned mL = args.mL;
         double vw = 0.0;
         auto eps_step = [&](T2* Sx) {
-            batched_loop<InflightEps<T>::v>(
-                L * Wd,
-                [&](int idx) -> T {
-                    const int c = udiv(idx, mL), q = q0 + (idx - c * L);
-                    return (q < plane && c < nc) ? invB[(unsigned)(c * plane + q)] : (T)0;
-                },
-                [&](int idx, T e) {
-                    const int c = udiv(idx, mL), tq = idx - c * L;
-                    const T2 v = Sx[tq * P + c], w = cscale(v, e * scale);
-                    Sx[tq * P + c] = w;
-                    if (pap) vw += (double)v.x * (double)w.x + (double)v.y * (double)w.y;
-                });
+            if constexpr (sizeof(T) == 4) {
+                if (pairs)
+                    batched_loop<InflightEps<T>::v / 2>(
+                        Lh * Wd,
+                        [&](int idx) -> float2 {
+                            const int c = udiv(2 * idx, mL), q = q0 + (2 * idx - c * L);
+                            return (q < plane && c < nc) ? *reinterpret_cast<const float2*>(invB + (unsigned)(c * plane + q))
+                                                         : make_float2(0.f, 0.f);
+                        },
+                        [&](int idx, float2 e) {
+                            const int c = udiv(2 * idx, mL), tq = 2 * idx - c * L;
+                            const float2 v0 = Sx[tq * P + c], w0 = cscale(v0, e.x * scale);
+                            const float2 v1 = Sx[(tq + 1) * P + c], w1 = cscale(v1, e.y * scale);
+                            Sx[tq * P + c] = w0;
+                            Sx[(tq + 1) * P + c] = w1;
+                            if (pap) vw += (double)v0.x * (double)w0.x + (double)v0.y * (double)w0.y +
+                                           (double)v1.x * (double)w1.x + (double)v1.y * (double)w1.y;
+                        });
+            }
+            if (!pairs)
+                batched_loop<InflightEps<T>::v>(
+                    L * Wd,
+                    [&](int idx) -> T {
+                        const int c = udiv(idx, mL), q = q0 + (idx - c * L);
+                        return (q < plane && c < nc) ? invB[(unsigned)(c * plane + q)] : (T)0;
+                    },
+                    [&](int idx, T e) {
+                        const int c = udiv(idx, mL), tq = idx - c * L;
+                        const T2 v = Sx[tq * P + c], w = cscale(v, e * scale);
+                        Sx[tq * P + c] = w;
+                        if (pap) vw += (double)v.x * (double)w.x + (double)v.y * (double)w.y;
+                    });
             __syncthreads();
         };
         run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
@@ -530,13 +818,28 @@ __global__ void __launch_bounds__(FFT_THREADS) k_outer(KAxis args) {
         run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
     }
     const double invN = 1.0 / (double)g.N;
-    for (int idx = threadIdx.x; idx < L * nc; idx += FFT_THREADS) {
-        const int c = udiv(idx, args.mL), tq = idx - c * L, q = q0 + tq;
-        if (q >= plane) continue;
-        T2 v = post_mul<T2, T>(pl, tb, c, S[tq * P + c]);
-        if (MODE == OUTER_QR) v = cmul(v, from_d<T2>(cscale(expipi(2.0 * pr.r0 * (double)c / (double)nc), invN)));
-        F[(unsigned)(c * plane + q)] = v;
+    if constexpr (sizeof(T) == 4) {
+        if (pairs)
+            for (int idx = threadIdx.x; idx < Lh * nc; idx += FFT_THREADS) {
+                const int c = udiv(2 * idx, args.mL), tq = 2 * idx - c * L, q = q0 + tq;
+                if (q >= plane) continue;
+                float2 v0 = post_mul<T2, T>(pl, tb, c, S[tq * P + c]);
+                float2 v1 = post_mul<T2, T>(pl, tb, c, S[(tq + 1) * P + c]);
+                if (MODE == OUTER_QR) {
+                    const float2 w = from_d<T2>(cscale(expipi(2.0 * pr.r0 * (double)c / (double)nc), invN));
+                    v0 = cmul(v0, w); v1 = cmul(v1, w);
+                }
+                *reinterpret_cast<float4*>(F + (unsigned)(c * plane + q)) = make_float4(v0.x, v0.y, v1.x, v1.y);
+            }
     }
+    if (!pairs)
+        for (int idx = threadIdx.x; idx < L * nc; idx += FFT_THREADS) {
+            const int c = udiv(idx, args.mL), tq = idx - c * L, q = q0 + tq;
+            if (q >= plane) continue;
+            T2 v = post_mul<T2, T>(pl, tb, c, S[tq * P + c]);
+            if (MODE == OUTER_QR) v = cmul(v, from_d<T2>(cscale(expipi(2.0 * pr.r0 * (double)c / (double)nc), invN)));
+            F[(unsigned)(c * plane + q)] = v;
+        }
 }
 
 /* ------------------------------------------------------------------------- */
