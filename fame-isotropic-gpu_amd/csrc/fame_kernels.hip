@@ -685,6 +685,149 @@ k_mid(KAxis args) {
     }
 }
 
+/* FP32 middle pass whose plan is one direct DFT on the matrix cores (n = nb <
+ * 128, no Bluestein), with the DFT's in-place prologue and its result step
+ * fused into the memory phases of the pass:
+ *   - the gather loads rows t and n - t of a pair column together and writes
+ *     s_t = x_t + x_{n-t} and d_t = x_t - x_{n-t} (dft_mfma's first step) into
+ *     the LDS lines, so the DFT starts on them after one barrier;
+ *   - each wave stores its outputs X_q and X_{n-q} from the accumulators
+ *     straight to HBM (16 lanes write 16 consecutive lines of one row, 128
+ *     bytes), with the middle twiddle, instead of through LDS and a store loop.
+ * The A operands, K steps and result layout are those of dft_mfma.  The host
+ * launches it with L a multiple of 16, na even, 16-byte aligned planes and one
+ * wave per 16-row output tile (blockDim = 64 dqt).                          */
+template <int DIR>
+__global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_mid_dft32(KAxis args) {
+    typedef Mfma<float>::acc_t acc_t;
+    const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
+    if (args.active && !args.active[p]) return;
+    const int ntile = (args.g.na + args.L - 1) / args.L;
+    const int tile = xcd_tile();
+    if (tile >= ntile * args.g.nc) return;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const GridDev& g = args.g;
+    const ProbDev pr = args.prob[p];
+    const LinePlanDev& pl = args.plan;
+    const StageDev& st = pl.st[0];
+    const int na = g.na, n = g.nb, L = args.L, P = args.P;
+    float2* base = reinterpret_cast<float2*>(smem_raw);
+    const Tabs<float2> tb = load_tabs<float2>(pl, base);
+    float2* twb = base + lds_head(pl);
+    float2* S = twb + n;
+    const int cf = tile / ntile, a0 = (tile - cf * ntile) * L;
+    float2* F = reinterpret_cast<float2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * n * na;
+    const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)n;
+    for (int b = threadIdx.x; b < n; b += blockDim.x) twb[b] = tw_expipi<float2>(phc * (double)b);
+    __syncthreads();
+    const int m = st.dm;
+    const bool even = (n & 1) == 0;
+    /* gather: unit u = 0 is row 0, u = 1..m the rows u and n - u, and for even
+     * n u = m + 1 the row n / 2; each thread keeps one pair column */
+    {
+        const int Lh = L >> 1;
+        const int ta = 2 * ((int)threadIdx.x % Lh), us = (int)blockDim.x / Lh;
+        const bool aon = a0 + ta < na;
+        const unsigned rowq = (unsigned)(na >> 1);
+        const float4* Fa = reinterpret_cast<const float4*>(F + a0 + ta);
+        const int nu = m + 1 + (even ? 1 : 0);
+        for (int u0 = (int)threadIdx.x / Lh; u0 < nu; u0 += 2 * us) {
+            float4 x[2], y[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = u0 + k * us;
+                const int r1 = u <= m ? u : (n >> 1), r2 = (u >= 1 && u <= m) ? n - u : -1;
+                x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                y[k] = x[k];
+                if (u < nu && aon) {
+                    x[k] = Fa[(unsigned)r1 * rowq];
+                    if (r2 >= 0) y[k] = Fa[(unsigned)r2 * rowq];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = u0 + k * us;
+                if (u >= nu) continue;
+                const int r1 = u <= m ? u : (n >> 1), r2 = (u >= 1 && u <= m) ? n - u : -1;
+                float2 x0 = make_float2(x[k].x, x[k].y), x1 = make_float2(x[k].z, x[k].w);
+                float2 y0 = make_float2(y[k].x, y[k].y), y1 = make_float2(y[k].z, y[k].w);
+                if (DIR < 0) {
+                    const float2 w1 = twb[r1];
+                    x0 = cmulc(x0, w1); x1 = cmulc(x1, w1);
+                }
+                if (r2 >= 0) {
+                    if (DIR < 0) {
+                        const float2 w2 = twb[r2];
+                        y0 = cmulc(y0, w2); y1 = cmulc(y1, w2);
+                    }
+                    S[ta * P + r1] = cadd(x0, y0);
+                    S[ta * P + r2] = csub(x0, y0);
+                    S[(ta + 1) * P + r1] = cadd(x1, y1);
+                    S[(ta + 1) * P + r2] = csub(x1, y1);
+                } else {
+                    S[ta * P + r1] = x0;
+                    S[(ta + 1) * P + r1] = x1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    /* the DFT: wave w owns output rows 16 w .. 16 w + 15 */
+    const int Q = st.dQ, nks = st.dK >> 2;
+    const bool inv = st.sign > 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row = lane & 15, kq = lane >> 4;
+    const int q = w * 16 + row;
+    const bool qon = q < Q;
+    const int dr = (4 * q) % n;
+    const int r0 = (q * (kq + 1)) % n;
+    float2* Fw = F + a0;
+    for (int g0 = 0; g0 < (L >> 4); ++g0) {
+        const int line = g0 * 16 + row;
+        const float2* Sb = S + line * P;
+        acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+        const float2 x0 = Sb[0];
+        float2 h = make_float2(0.f, 0.f);
+        if (even) h = Sb[n >> 1];
+        int r = r0;
+        for (int ks = 0; ks < nks; ++ks) {
+            const int t = min(ks * 4 + kq + 1, max(m, 1));
+            const float2 bs = Sb[t], bd = Sb[n - t];
+            float a_c = 0.f, a_s = 0.f;
+            if (qon && ks * 4 + kq + 1 <= m) {
+                const float2 wv = tb.W[r];
+                a_c = wv.x; a_s = wv.y;
+            }
+            r += dr;
+            if (r >= n) r -= n;
+            cr = Mfma<float>::mac(a_c, bs.x, cr);
+            ci = Mfma<float>::mac(a_c, bs.y, ci);
+            sr = Mfma<float>::mac(a_s, bd.x, sr);
+            si = Mfma<float>::mac(a_s, bd.y, si);
+        }
+        if (a0 + line < na) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int qq = w * 16 + Mfma<float>::row(kq, v);
+                if (qq < Q) {
+                    const float Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
+                    const float Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
+                    const float Bx = -sr[v], By = -si[v];
+                    float2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
+                    if (!inv) { lo.x = Ax + By; lo.y = Ay - Bx; hi.x = Ax - By; hi.y = Ay + Bx; }
+                    else      { lo.x = Ax - By; lo.y = Ay + Bx; hi.x = Ax + By; hi.y = Ay - Bx; }
+                    if (DIR > 0) lo = cmul(lo, twb[qq]);
+                    Fw[(unsigned)(qq * na + line)] = lo;
+                    if (qq != 0 && 2 * qq != n) {
+                        if (DIR > 0) hi = cmul(hi, twb[n - qq]);
+                        Fw[(unsigned)((n - qq) * na + line)] = hi;
+                    }
+                }
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /*  P3: outer axis, per component                                            */
 /* ------------------------------------------------------------------------- */
@@ -950,6 +1093,15 @@ static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) 
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(xcd_round(ntile * a.g.nc), 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
+    /* FP32 single direct DFT: the fused kernel (its preconditions, see there) */
+    if (ENG == ENG_MFMA && sizeof(T) == 4 && a.plan.nst == 1 && a.plan.st[0].R == DFT_MFMA && !a.plan.blue &&
+        a.plan.post == POST_NONE && a.plan.st[0].dn == a.g.nb && a.L % 16 == 0 && a.g.na % 2 == 0 &&
+        ((uintptr_t)a.f & 15) == 0 && a.fstride % 2 == 0 && a.plan.st[0].dqt <= FFT_THREADS / 64) {
+        const dim3 blk(64 * a.plan.st[0].dqt);
+        if (dir > 0) hipLaunchKernelGGL(k_mid_dft32<1>, grid, blk, lds, s, a);
+        else hipLaunchKernelGGL(k_mid_dft32<-1>, grid, blk, lds, s, a);
+        return hipGetLastError();
+    }
     /* the matrix-core DFT keeps one wave per (16-row output tile, 16-line group)
      * busy: launch those waves, at most MidThreads / 64 */
     int nthr = FFT_THREADS;
