@@ -757,4 +757,44 @@ __device__ __forceinline__ void pi_basis(const GridDev& g, double2 lx, double2 l
     p2[2] = cscale(cconj(t3), i2);
 }
 
+/* The General gauge applied without forming Pi (pi_basis's non-GUARD General
+ * branch, same normalisers i1, i2, same rounding of them).  With S = Lx + Ly + Lz,
+ * Pi1_c = (Lq - L_c conj(S)) i1 and Pi2_c = conj(d_c) i2, d = (Lz - Ly, Lx - Lz, Ly - Lx):
+ *   expansion   u_c  = Lq i1 x1 - L_c (i1 conj(S) x1) + conj(d_c) (i2 x2),
+ *   contraction Pi1^H w = i1 (Lq sum_c w_c - S sum_c conj(L_c) w_c),
+ *               Pi2^H w = i2 (Lx (w_y - w_z) + Ly (w_z - w_x) + Lz (w_x - w_y)).
+ * Mathematically the same products as with pi_basis's vectors, with fewer
+ * operations and live values (the contraction is issue-bound).             */
+template <bool FAST>
+__device__ __forceinline__ void pi_norms_general(double2 lx, double2 ly, double2 lz, double& Lq, double2& Ls,
+                                                 double& i1, double& i2) {
+    Lq = lx.x * lx.x + lx.y * lx.y + ly.x * ly.x + ly.y * ly.y + lz.x * lz.x + lz.y * lz.y;
+    Ls = cadd(cadd(lx, ly), lz);
+    const double Lp = Ls.x * Ls.x + Ls.y * Ls.y;
+    i1 = rsq<FAST>(3.0 * Lq * Lq - Lq * Lp);
+    i2 = rsq<FAST>(3.0 * Lq - Lp);
+}
+template <bool FAST>
+__device__ __forceinline__ void pi_expand_general(double2 lx, double2 ly, double2 lz, double2 x1, double2 x2,
+                                                  double2 u[3]) {
+    double Lq, i1, i2;
+    double2 Ls;
+    pi_norms_general<FAST>(lx, ly, lz, Lq, Ls, i1, i2);
+    const double2 al = cscale(x1, Lq * i1), be = cscale(cmulc(x1, Ls), i1), ga = cscale(x2, i2);
+    u[0] = cadd(csub(al, cmul(lx, be)), cmulc(ga, csub(lz, ly)));
+    u[1] = cadd(csub(al, cmul(ly, be)), cmulc(ga, csub(lx, lz)));
+    u[2] = cadd(csub(al, cmul(lz, be)), cmulc(ga, csub(ly, lx)));
+}
+template <bool FAST>
+__device__ __forceinline__ void pi_contract_general(double2 lx, double2 ly, double2 lz, double2 w0, double2 w1,
+                                                    double2 w2, double2& y1, double2& y2) {
+    double Lq, i1, i2;
+    double2 Ls;
+    pi_norms_general<FAST>(lx, ly, lz, Lq, Ls, i1, i2);
+    const double2 sw = cadd(cadd(w0, w1), w2);
+    const double2 t = cadd(cadd(cmulc(w0, lx), cmulc(w1, ly)), cmulc(w2, lz));
+    y1 = cscale(csub(cscale(sw, Lq), cmul(Ls, t)), i1);
+    y2 = cscale(cadd(cadd(cmul(lx, csub(w1, w2)), cmul(ly, csub(w2, w0))), cmul(lz, csub(w0, w1))), i2);
+}
+
 }  // namespace fame

@@ -60,8 +60,16 @@ template <> __device__ __forceinline__ float2 tw_expipi<float2>(double x) {
  * pikind = 0 : Pi_Qr = [Pi1 Pi2]; 1 : Pi_Pr = [-conj(Pi2) conj(Pi1)]
  *              (FAME_Matrix_Lambdas.cu:217-243)
  * A block owns L lines (b, c) and their three components: 3L LDS lines.      */
-template <typename T, int EXPAND, int ENG, bool GUARD>
+/* Pi forms compiled into an inner kernel: PI_ANY forms the basis vectors with
+ * pi_basis (Simple gauge, or the Pr / Prs variants), PI_GUARD the same with the
+ * equal-mesh guard, PI_GEN the General gauge of Qr / Qrs applied without forming
+ * the vectors (pi_expand_general / pi_contract_general).  Each is its own
+ * instance, so that registers are allocated for the one form it runs.      */
+enum { PI_ANY = 0, PI_GUARD = 1, PI_GEN = 2 };
+
+template <typename T, int EXPAND, int ENG, int PIK>
 __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
+    constexpr bool GUARD = PIK == PI_GUARD;
     typedef typename Vec<T>::T2 T2;
     const int p = blockIdx.y;
     if (args.active && !args.active[p]) return;
@@ -178,14 +186,19 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
             }
             double2 u[3] = {make_double2(0, 0), make_double2(0, 0), make_double2(0, 0)};
             if (!(pr.gamma && f == 0)) {
-                double2 lx, ly, lz, p1[3], p2[3];
+                double2 lx, ly, lz;
                 lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
                 const double2 x1 = to_d(xv1), x2 = to_d(xv2);
+                if constexpr (PIK == PI_GEN) {
+                    pi_expand_general<sizeof(T) == 4>(lx, ly, lz, x1, x2, u);
+                } else {
+                    double2 p1[3], p2[3];
+                    pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
-                    else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+                    for (int c = 0; c < 3; ++c) {
+                        if (args.pikind == 0) u[c] = cadd(cmul(p1[c], x1), cmul(p2[c], x2));
+                        else u[c] = cadd(cmul(make_double2(-p2[c].x, p2[c].y), x1), cmul(cconj(p1[c]), x2));
+                    }
                 }
             }
 #pragma unroll
@@ -421,18 +434,26 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
         auto contract = [&](int line, int a, unsigned f, T2& o1, T2& o2) {
             double2 y1 = make_double2(0, 0), y2 = make_double2(0, 0);
             if (!(pr.gamma && f == 0)) {
-                double2 lx, ly, lz, p1[3], p2[3];
+                double2 lx, ly, lz;
                 lambdas_line_tab(g, lsym[line], ea[a], ephi[line], lx, ly, lz);
-                pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
+                if constexpr (PIK == PI_GEN) {
+                    const double2 w0 = to_d(post_mul<T2, T>(pl, tb, a, S[(0 * L + line) * P + a]));
+                    const double2 w1 = to_d(post_mul<T2, T>(pl, tb, a, S[(1 * L + line) * P + a]));
+                    const double2 w2 = to_d(post_mul<T2, T>(pl, tb, a, S[(2 * L + line) * P + a]));
+                    pi_contract_general<sizeof(T) == 4>(lx, ly, lz, w0, w1, w2, y1, y2);
+                } else {
+                    double2 p1[3], p2[3];
+                    pi_basis<sizeof(T) == 4, GUARD>(g, lx, ly, lz, p1, p2);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
-                    if (args.pikind == 0) {
-                        y1 = cadd(y1, cmul(cconj(p1[c]), w));
-                        y2 = cadd(y2, cmul(cconj(p2[c]), w));
-                    } else {
-                        y1 = csub(y1, cmul(p2[c], w));
-                        y2 = cadd(y2, cmul(p1[c], w));
+                    for (int c = 0; c < 3; ++c) {
+                        const double2 w = to_d(post_mul<T2, T>(pl, tb, a, S[(c * L + line) * P + a]));
+                        if (args.pikind == 0) {
+                            y1 = cadd(y1, cmul(cconj(p1[c]), w));
+                            y2 = cadd(y2, cmul(cconj(p2[c]), w));
+                        } else {
+                            y1 = csub(y1, cmul(p2[c], w));
+                            y2 = cadd(y2, cmul(p1[c], w));
+                        }
                     }
                 }
             }
@@ -1079,11 +1100,16 @@ static hipError_t launch_inner(const KInner& a, int expand, int nprob, hipStream
     dim3 grid(xcd_round((nlines + a.L - 1) / a.L), nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), 3 * a.L, a.P, inner_lds_extra(a.g.na, (int)sizeof(T), a.L));
     if (a.g.pi_guard) {
-        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, true>), grid, dim3(FFT_THREADS), lds, s, a);
-        else hipLaunchKernelGGL((k_inner<T, 0, ENG, true>), grid, dim3(FFT_THREADS), lds, s, a);
+        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, PI_GUARD>), grid, dim3(FFT_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_inner<T, 0, ENG, PI_GUARD>), grid, dim3(FFT_THREADS), lds, s, a);
+    } else if (a.g.general && a.pikind == 0 && (sizeof(T) == 4 || !expand)) {
+        /* the FP64 expansion keeps the vector form: 5% faster there on gfx950 */
+        if constexpr (sizeof(T) == 4)
+            if (expand) { hipLaunchKernelGGL((k_inner<T, 1, ENG, PI_GEN>), grid, dim3(FFT_THREADS), lds, s, a); return hipGetLastError(); }
+        hipLaunchKernelGGL((k_inner<T, 0, ENG, PI_GEN>), grid, dim3(FFT_THREADS), lds, s, a);
     } else {
-        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, false>), grid, dim3(FFT_THREADS), lds, s, a);
-        else hipLaunchKernelGGL((k_inner<T, 0, ENG, false>), grid, dim3(FFT_THREADS), lds, s, a);
+        if (expand) hipLaunchKernelGGL((k_inner<T, 1, ENG, PI_ANY>), grid, dim3(FFT_THREADS), lds, s, a);
+        else hipLaunchKernelGGL((k_inner<T, 0, ENG, PI_ANY>), grid, dim3(FFT_THREADS), lds, s, a);
     }
     return hipGetLastError();
 }
