@@ -205,7 +205,7 @@ def passes_only(F, fl, eps, ks, args, torch, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, default=96)
     ap.add_argument("--nk", type=int, default=30)
@@ -294,9 +294,10 @@ def main():
     # the profile); its kernels are timed as the solver launches them, fused
     # vector updates included, with all wave vectors active.
     tiles = {"fp32": S.autotune(4), "fp64": S.autotune(8)}    # the choices the solve used (same batch shape)
-    us32 = S.time_qbq_passes(4, 10)
-    us64 = S.time_qbq_passes(8, 10)
-    cg32 = S.time_cg_passes(4, 10)
+    us32 = S.time_qbq_passes(4, 20)
+    us64 = S.time_qbq_passes(8, 20)
+    S.time_cg_passes(4, 20)            # the first pass brings clocks and caches to the solve's state
+    cg32 = S.time_cg_passes(4, 20)
     N = S.N
     s32, r32 = 8, 4
     cg_names = ["cg_inner_expand", "cg_mid_inverse", "cg_outer_eps", "cg_alpha", "cg_mid_forward",

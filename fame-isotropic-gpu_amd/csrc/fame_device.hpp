@@ -370,8 +370,19 @@ __host__ __device__ constexpr int nbt_of(int R, int MAXE) { return MAXE / R >= 1
 /* one in-place radix-R Stockham pass over nl lines.
  * Butterfly j of a line (k = j mod Ns): inputs x[j + t nb] (times the input
  * multiplier, then W_{Ns R}^{t k}), outputs y[(j / Ns) Ns R + k + t Ns].       */
-template <typename T2, int R, int MAXE>
-__device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb) {
+/* A `mid` step (R = 0 stage) may instead be folded into the loads of the pass
+ * that follows it: a Mid type with `fold = true` provides mul(line, m, v), the
+ * step's value for element m of a line, and run_stages then skips the step's
+ * own LDS round trip and barrier.                                            */
+template <typename M, typename = void> struct FoldOf { static constexpr bool v = false; };
+template <typename M> struct FoldOf<M, decltype((void)M::fold)> { static constexpr bool v = M::fold; };
+
+struct NoFoldMul { };
+/* returns the sum that a folded step's mul() accumulates (0 otherwise) */
+template <typename T2, int R, int MAXE, typename Mul = NoFoldMul>
+__device__ __forceinline__ double fft_pass(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb,
+                                           const Mul& mul = Mul(), bool fold = false) {
+    double facc = 0.0;
     constexpr int NBT = nbt_of(R, MAXE);
     const int nb = st.nb, Ns = st.Ns, tstep = st.tstep;
     const int total = nl * nb;
@@ -393,6 +404,12 @@ __device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& s
             const T2* a = S + line * P;
 #pragma unroll
             for (int t = 0; t < R; ++t) v[i][t] = a[j + t * nb];
+            if constexpr (FoldOf<Mul>::v) {
+                if (fold) {
+#pragma unroll
+                    for (int t = 0; t < R; ++t) v[i][t] = mul.mul(line, j + t * nb, v[i][t], facc);
+                }
+            }
             if (pre) {
 #pragma unroll
                 for (int t = 0; t < R; ++t) {
@@ -422,6 +439,7 @@ __device__ __forceinline__ void fft_pass(T2* S, int nl, int P, const StageDev& s
         }
     }
     __syncthreads();
+    return facc;
 }
 
 /* ------------------------------------------------------------------------- */
@@ -573,35 +591,44 @@ struct NoMid { template <typename T2> __device__ void operator()(T2*) const {} }
  * allocated for its own engine only.                                        */
 enum { ENG_STOCKHAM = 0, ENG_MFMA = 1 };
 template <typename T2, int MAXE, int ENG, typename Mid>
-__device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanDev& pl, const Tabs<T2>& tb,
-                                           const Mid& mid) {
+__device__ __forceinline__ double run_stages(T2* S, int nl, int P, const LinePlanDev& pl, const Tabs<T2>& tb,
+                                             const Mid& mid) {
+    double facc = 0.0;   /* what a folded mid step accumulates */
+    constexpr bool FOLD = FoldOf<Mid>::v && ENG != ENG_MFMA;
+    bool fold_next = false;
     for (int s = 0; s < pl.nst; ++s) {
         const StageDev& st = pl.st[s];
-        if (st.R == 0) { mid(S); continue; }
+        if (st.R == 0) {
+            if (FOLD && s + 1 < pl.nst) { fold_next = true; continue; }
+            mid(S);
+            continue;
+        }
         if constexpr (ENG == ENG_MFMA) {
             dft_mfma<T2, sizeof(S[0].x) == 8>(S, nl, P, st, tb);
         } else {
+            const bool fm = fold_next;
+            fold_next = false;
             /* the compound radices exist in the FP32 kernels only (the FP64
              * plans never use them, and their registers would cost waves) */
             constexpr bool CMP = MAXE >= 16;
             switch (st.R) {
-                case 2: fft_pass<T2, 2, MAXE>(S, nl, P, st, tb); break;
-                case 3: fft_pass<T2, 3, MAXE>(S, nl, P, st, tb); break;
-                case 4: fft_pass<T2, 4, MAXE>(S, nl, P, st, tb); break;
-                case 5: fft_pass<T2, 5, MAXE>(S, nl, P, st, tb); break;
-                case 7: fft_pass<T2, 7, MAXE>(S, nl, P, st, tb); break;
-                case 8: fft_pass<T2, 8, MAXE>(S, nl, P, st, tb); break;
-                case 11: fft_pass<T2, 11, MAXE>(S, nl, P, st, tb); break;
-                case 13: fft_pass<T2, 13, MAXE>(S, nl, P, st, tb); break;
+                case 2: facc += fft_pass<T2, 2, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 3: facc += fft_pass<T2, 3, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 4: facc += fft_pass<T2, 4, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 5: facc += fft_pass<T2, 5, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 7: facc += fft_pass<T2, 7, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 8: facc += fft_pass<T2, 8, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 11: facc += fft_pass<T2, 11, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 13: facc += fft_pass<T2, 13, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
                 default:
                     if constexpr (CMP) {
                         switch (st.R) {
-                            case 6: fft_pass<T2, 6, MAXE>(S, nl, P, st, tb); break;
-                            case 9: fft_pass<T2, 9, MAXE>(S, nl, P, st, tb); break;
-                            case 10: fft_pass<T2, 10, MAXE>(S, nl, P, st, tb); break;
-                            case 12: fft_pass<T2, 12, MAXE>(S, nl, P, st, tb); break;
-                            case 15: fft_pass<T2, 15, MAXE>(S, nl, P, st, tb); break;
-                            case 16: fft_pass<T2, 16, MAXE>(S, nl, P, st, tb); break;
+                            case 6: facc += fft_pass<T2, 6, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 9: facc += fft_pass<T2, 9, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 10: facc += fft_pass<T2, 10, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 12: facc += fft_pass<T2, 12, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 15: facc += fft_pass<T2, 15, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 16: facc += fft_pass<T2, 16, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
                             default: break;
                         }
                     }
@@ -609,6 +636,7 @@ __device__ __forceinline__ void run_stages(T2* S, int nl, int P, const LinePlanD
             }
         }
     }
+    return facc;
 }
 
 /* engine of a stage list (host and device) */

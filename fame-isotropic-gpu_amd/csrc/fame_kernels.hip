@@ -859,6 +859,37 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
  * In EPS mode with Bluestein the inverse post-chirp and the forward pre-chirp
  * are conjugate unit factors whose product is 1; both are left out and only
  * the 1/M remains in the elementwise step.                                  */
+/* The FP32 outer pass's 1/eps step as a foldable stage of the Stockham
+ * engine: mul() is its value for element c of line `line` (the scaled 1/eps
+ * product, adding this thread's share of <v, eps^-1 v / N>), applied by the
+ * first forward pass as it loads; operator() runs the same step as a pass of
+ * its own (used only if no pass follows the step).                         */
+struct EpsFold32 {
+    static constexpr bool fold = true;
+    const float* invB;
+    int q0, plane, nc, L, P, Wd;
+    unsigned mL;
+    float scale;
+    bool pap;
+    __device__ float2 mul(int line, int c, float2 v, double& acc) const {
+        const int q = q0 + line;
+        const float e = (c < nc && q < plane) ? invB[(unsigned)(c * plane + q)] : 0.f;
+        const float2 w = cscale(v, e * scale);
+        if (pap) acc += (double)v.x * (double)w.x + (double)v.y * (double)w.y;
+        return w;
+    }
+    /* the step as a pass of its own; its sum is lost, so a plan that needs
+     * this (a step with no pass after it) must not ask for <p, Ap> */
+    __device__ void operator()(float2* Sx) const {
+        double acc = 0.0;
+        for (int idx = threadIdx.x; idx < L * Wd; idx += blockDim.x) {
+            const int c = udiv(idx, mL), tq = idx - c * L;
+            Sx[tq * P + c] = mul(tq, c, Sx[tq * P + c], acc);
+        }
+        __syncthreads();
+    }
+};
+
 template <typename T, int MODE, int ENG>
 __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(OuterWaves<T>::v))) k_outer(KAxis args) {
     typedef typename Vec<T>::T2 T2;
@@ -965,7 +996,16 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
                     });
             __syncthreads();
         };
-        run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
+        /* FP32 Stockham plans fold the step into the loads of the first forward
+         * pass (no LDS round trip of its own); 1/eps is shared by all problems,
+         * so its scattered reads there are mostly served by the caches */
+        bool folded = false;
+        if constexpr (sizeof(T) == 4 && ENG != ENG_MFMA) {
+            const EpsFold32 em{invB, q0, plane, nc, L, P, Wd, mL, scale, pap != nullptr};
+            vw += run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, em);
+            folded = true;
+        }
+        if (!folded) run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
         if (pap) {
             /* block sum in a fixed order: wave shuffles, then the wave sums */
             __shared__ double red[FFT_THREADS / 64];
