@@ -35,7 +35,8 @@ std::string& fame_err_ref();
  * (one stage list: forward transform, the curl-curl step, inverse transform;
  * with Bluestein the forward post-chirp and inverse pre-chirp cancel and only
  * 1/M remains in the step) */
-template <int ENG>
+enum { STEP_CURLCURL = 0, STEP_LAMBDA = 1 };
+template <int ENG, int STEP>
 __global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
     typedef double2 T2;
     const int p = blockIdx.y;
@@ -75,6 +76,10 @@ __global__ void __launch_bounds__(FFT_THREADS) k_curlcurl(KInner args) {
                 lambdas(g, pr, a, bf, cf, l[0], l[1], l[2]);
                 double2 e[3];
                 for (int c = 0; c < 3; ++c) e[c] = cscale(Sx[(c * L + line) * P + a], sc);
+                if (STEP == STEP_LAMBDA) {   /* component c times Lambda_c */
+                    for (int c = 0; c < 3; ++c) Sx[(c * L + line) * P + a] = cmul(l[c], e[c]);
+                    continue;
+                }
                 double2 u[3];   /* u = L x e */
                 u[0] = csub(cmul(l[1], e[2]), cmul(l[2], e[1]));
                 u[1] = csub(cmul(l[2], e[0]), cmul(l[0], e[2]));
@@ -174,8 +179,9 @@ static fame_status residuals_chunk(fame_ctx* c, int nwant, const double* freq, c
         HIPCHK(fame_launch_mid(c->kmid(8, p0, nullptr, LIST_FWD), -1, 8, np, c->stream));
         dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, np);
         const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
-        if (plan_engine(ki.plan) == ENG_MFMA) hipLaunchKernelGGL(k_curlcurl<ENG_MFMA>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
-        else hipLaunchKernelGGL(k_curlcurl<ENG_STOCKHAM>, grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        if (plan_engine(ki.plan) == ENG_MFMA)
+            hipLaunchKernelGGL((k_curlcurl<ENG_MFMA, STEP_CURLCURL>), grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        else hipLaunchKernelGGL((k_curlcurl<ENG_STOCKHAM, STEP_CURLCURL>), grid, dim3(FFT_THREADS), lds, c->stream, ki);
         HIPCHK(hipGetLastError());
         HIPCHK(fame_launch_mid(c->kmid(8, p0, nullptr, LIST_INV), +1, 8, np, c->stream));
         HIPCHK(fame_launch_outer(c->kout(8, p0, nullptr, LIST_INV), OUTER_QR, 8, np, c->stream));
@@ -404,4 +410,236 @@ extern "C" fame_status fame_eig_grad(fame_ctx* c, int nwant, const double* freq,
     }
     HIPCHK(hipStreamSynchronize(c->stream));
     return FAME_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/*  Sparse curl and the check of the factorisation                            */
+/* ------------------------------------------------------------------------- */
+/* The forward differences C_1, C_2, C_3 of FAME_Matrix_Curl
+ * (FAME_Main/FAME_Matrix_Curl.cpp:90-443) as a rule per grid point: row (x, y, z)
+ * (x fastest) holds -1/h on the diagonal and w/h on one neighbour, w = 1
+ * inside the cell and a Bloch factor exp(i theta) across its boundary.
+ *   C_1: x + 1; at x = n1 - 1, x = 0 with theta = 2 pi a1.k.
+ *   C_2: y + 1; at y = n2 - 1, y = 0, and for General lattices x shifted by m1:
+ *        x < m1 -> x + n1 - m1 with theta = 2 pi (a2 + (rho - 1) a1).k, else
+ *        x - m1 with 2 pi (a2 + rho a1).k, rho = 1 when the computational a2
+ *        points to negative x (:259-280).
+ *   C_3: z + 1; at z = n3 - 1, z = 0, and for General lattices (y, x) shifted
+ *        by (m3, m4 or m2) with the boundary-point phases t1..t4 (:296-373):
+ *        y < m3 -> y + n2 - m3, x < m4 ? x + n1 - m4 (t3) : x - m4 (t4);
+ *        else   -> y - m3,      x < m2 ? x + n1 - m2 (t2) : x - m2 (t1).
+ * The full curl is [0, -C3, C2; C3, 0, -C1; -C2, C1, 0] (:105-129).        */
+struct CurlGeo {
+    int n1, n2, n3, general, m1, m2, m3, m4, rho;
+    double ih[3];
+    double ta[3];   /* 2 pi a_i . k */
+    double tt[4];   /* 2 pi t_j . k */
+};
+
+/* the boundary shifts keep every neighbour inside the grid */
+static bool curl_geo_ok(const fame_lattice& L) {
+    const int n1 = L.grid_nums[0], n2 = L.grid_nums[1], n3 = L.grid_nums[2];
+    if (n1 < 1 || n2 < 1 || n3 < 1) return false;
+    if (!L.general) return true;
+    return L.m1 >= 0 && L.m1 <= n1 && L.m2 >= 0 && L.m2 <= n1 && L.m4 >= 0 && L.m4 <= n1 && L.m3 >= 0 && L.m3 <= n2;
+}
+
+static CurlGeo curl_geo(const fame_lattice& L, const double* k) {
+    CurlGeo G;
+    G.n1 = L.grid_nums[0]; G.n2 = L.grid_nums[1]; G.n3 = L.grid_nums[2];
+    G.general = L.general ? 1 : 0;
+    G.m1 = L.m1; G.m2 = L.m2; G.m3 = L.m3; G.m4 = L.m4;
+    G.rho = L.lattice_vec_a[3] < 0 ? 1 : 0;
+    const double tp = 2.0 * 3.14159265358979323846;
+    for (int i = 0; i < 3; ++i) {
+        G.ih[i] = 1.0 / L.mesh_len[i];
+        G.ta[i] = tp * (L.lattice_vec_a[3 * i] * k[0] + L.lattice_vec_a[3 * i + 1] * k[1] + L.lattice_vec_a[3 * i + 2] * k[2]);
+    }
+    const double* t[4] = {L.t1, L.t2, L.t3, L.t4};
+    for (int j = 0; j < 4; ++j) G.tt[j] = tp * (t[j][0] * k[0] + t[j][1] * k[1] + t[j][2] * k[2]);
+    return G;
+}
+
+/* neighbour of (x, y, z) in C_{ax+1} and the angle of its factor (0 inside);
+ * `wrap` tells whether the factor is exp(i angle) or 1 */
+__host__ __device__ inline int curl_nbr(const CurlGeo& G, int ax, int x, int y, int z, double& ang, bool& wrap) {
+    const int n1 = G.n1, n2 = G.n2, n3 = G.n3;
+    ang = 0.0;
+    wrap = false;
+    if (ax == 0) {
+        if (x < n1 - 1) return x + 1 + n1 * (y + n2 * z);
+        wrap = true; ang = G.ta[0];
+        return n1 * (y + n2 * z);
+    }
+    if (ax == 1) {
+        if (y < n2 - 1) return x + n1 * (y + 1 + n2 * z);
+        wrap = true;
+        if (!G.general) { ang = G.ta[1]; return x + n1 * n2 * z; }
+        if (x < G.m1) { ang = G.ta[1] + G.ta[0] * (double)(G.rho - 1); return x + n1 - G.m1 + n1 * n2 * z; }
+        ang = G.ta[1] + G.ta[0] * (double)G.rho;
+        return x - G.m1 + n1 * n2 * z;
+    }
+    if (z < n3 - 1) return x + n1 * (y + n2 * (z + 1));
+    wrap = true;
+    if (!G.general) { ang = G.ta[2]; return x + n1 * y; }
+    int xs, ys;
+    if (y < G.m3) {
+        ys = y + n2 - G.m3;
+        if (x < G.m4) { xs = x + n1 - G.m4; ang = G.ta[2] + G.tt[2]; }
+        else { xs = x - G.m4; ang = G.ta[2] + G.tt[3]; }
+    } else {
+        ys = y - G.m3;
+        if (x < G.m2) { xs = x + n1 - G.m2; ang = G.ta[2] + G.tt[1]; }
+        else { xs = x - G.m2; ang = G.ta[2] + G.tt[0]; }
+    }
+    return xs + n1 * ys;
+}
+
+/* dif = C_i x_i (component i) and cur = C x, reference layout, one point per thread */
+__global__ void __launch_bounds__(256) k_curl_stencil(const double2* __restrict__ X, double2* __restrict__ dif,
+                                                      double2* __restrict__ cur, CurlGeo G) {
+    const size_t N = (size_t)G.n1 * G.n2 * G.n3;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < N; p += (size_t)gridDim.x * 256) {
+        const int x = (int)(p % G.n1), y = (int)((p / G.n1) % G.n2), z = (int)(p / ((size_t)G.n1 * G.n2));
+        double2 d[3][3];   /* d[ax][c] = (C_{ax+1} x_c)(p) */
+        for (int ax = 0; ax < 3; ++ax) {
+            double ang;
+            bool wrap;
+            const int q = curl_nbr(G, ax, x, y, z, ang, wrap);
+            double2 w = make_double2(1.0, 0.0);
+            if (wrap) sincos(ang, &w.y, &w.x);
+            for (int c = 0; c < 3; ++c)
+                d[ax][c] = cscale(csub(cmul(w, X[c * N + q]), X[c * N + p]), G.ih[ax]);
+        }
+        for (int c = 0; c < 3; ++c) dif[c * N + p] = d[c][c];
+        cur[p] = csub(d[1][2], d[2][1]);
+        cur[N + p] = csub(d[2][0], d[0][2]);
+        cur[2 * N + p] = csub(d[0][1], d[1][0]);
+    }
+}
+
+/* partials of ||a_c - b_c||^2 per component c (one block row per component) */
+__global__ void __launch_bounds__(256) k_diff_norm(const double2* a, const double2* b, double* partials, size_t n) {
+    const int c = blockIdx.y;
+    double s = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const double2 u = a[c * n + i], v = b[c * n + i];
+        const double dx = u.x - v.x, dy = u.y - v.y;
+        s += dx * dx + dy * dy;
+    }
+    __shared__ double r[256];
+    r[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) r[threadIdx.x] += r[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[(size_t)c * gridDim.x + blockIdx.x] = r[0];
+}
+
+extern "C" fame_status fame_curl_coo(const fame_lattice* lat, const double* k, int* rows, int* cols, double* vals) {
+    if (!lat || !k || !rows || !cols || !vals) FAIL(FAME_ERR_ARG, "fame_curl_coo: bad arguments");
+    if (!curl_geo_ok(*lat)) FAIL(FAME_ERR_UNSUPPORTED, "fame_curl_coo: boundary shifts outside the grid");
+    const CurlGeo G = curl_geo(*lat, k);
+    const int N = G.n1 * G.n2 * G.n3;
+    if (N < 1) FAIL(FAME_ERR_ARG, "fame_curl_coo: empty grid");
+    /* block (row component, column component, sign) of each C_{ax+1} in C */
+    const int blk[3][2][3] = {{{1, 2, -1}, {2, 1, +1}},    /* C_1: -C1 at (y, z), +C1 at (z, y) */
+                              {{0, 2, +1}, {2, 0, -1}},    /* C_2: +C2 at (x, z), -C2 at (z, x) */
+                              {{0, 1, -1}, {1, 0, +1}}};   /* C_3: -C3 at (x, y), +C3 at (y, x) */
+    size_t e = 0;
+    for (int ax = 0; ax < 3; ++ax)
+        for (int z = 0; z < G.n3; ++z)
+            for (int y = 0; y < G.n2; ++y)
+                for (int x = 0; x < G.n1; ++x) {
+                    const int p = x + G.n1 * (y + G.n2 * z);
+                    double ang;
+                    bool wrap;
+                    const int q = curl_nbr(G, ax, x, y, z, ang, wrap);
+                    const double wr = wrap ? std::cos(ang) : 1.0, wi = wrap ? std::sin(ang) : 0.0;
+                    for (int b = 0; b < 2; ++b) {
+                        const int rc = blk[ax][b][0], cc = blk[ax][b][1];
+                        const double sg = blk[ax][b][2] * G.ih[ax];
+                        rows[e] = rc * N + p; cols[e] = cc * N + p; vals[2 * e] = -sg; vals[2 * e + 1] = 0.0; ++e;
+                        rows[e] = rc * N + p; cols[e] = cc * N + q; vals[2 * e] = sg * wr; vals[2 * e + 1] = sg * wi; ++e;
+                    }
+                }
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_check_eigendecomp(fame_ctx* c, int ik, unsigned long long seed, double* err) {
+    if (!c || !err || ik < 0 || ik >= c->nk) FAIL(FAME_ERR_ARG, "fame_check_eigendecomp: bad arguments");
+    if (!curl_geo_ok(c->lat)) FAIL(FAME_ERR_UNSUPPORTED, "fame_check_eigendecomp: boundary shifts outside the grid");
+    HIPCHK(hipSetDevice(c->device));
+    const size_t N = (size_t)c->N, N3 = 3 * N;
+    /* the seeded field: uniform real and imaginary parts in [0, 1) (splitmix64) */
+    std::vector<double2> xh(N3);
+    unsigned long long st = seed;
+    auto next = [&st]() {
+        unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return (double)((z ^ (z >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (size_t i = 0; i < N3; ++i) { xh[i].x = next(); xh[i].y = next(); }
+    double2 *X = nullptr, *D = nullptr, *Cx = nullptr, *Y = nullptr;
+    double* part = nullptr;
+    const int NB = 256;
+    fame_status rs = FAME_OK;
+    auto run = [&]() -> fame_status {
+        HIPCHK(hipMalloc(&X, 16 * N3));
+        HIPCHK(hipMalloc(&D, 16 * N3));
+        HIPCHK(hipMalloc(&Cx, 16 * N3));
+        HIPCHK(hipMalloc(&Y, 16 * N3));
+        HIPCHK(hipMalloc(&part, sizeof(double) * 3 * NB));
+        HIPCHK(hipMemcpyAsync(X, xh.data(), 16 * N3, hipMemcpyHostToDevice, c->stream));
+        /* sparse side */
+        const CurlGeo G = curl_geo(c->lat, &c->kvec[3 * (size_t)ik]);
+        hipLaunchKernelGGL(k_curl_stencil, dim3(1024), dim3(256), 0, c->stream, X, D, Cx, G);
+        HIPCHK(hipGetLastError());
+        /* T Lambda_i T* x_i for the three components at once (the curl-curl
+         * pipeline with the Lambda step) */
+        CHK(c->field_from_ref(X, c->F64 + (size_t)ik * N3, 1));
+        KInner ki = c->kin(8, ik, nullptr, LIST_FI);
+        HIPCHK(fame_launch_outer(c->kout(8, ik, nullptr, LIST_FWD), OUTER_QRS, 8, 1, c->stream));
+        HIPCHK(fame_launch_mid(c->kmid(8, ik, nullptr, LIST_FWD), -1, 8, 1, c->stream));
+        dim3 grid((c->g.nb * c->g.nc + ki.L - 1) / ki.L, 1);
+        const size_t lds = fame_line_lds(ki.plan, 8, 3 * ki.L, ki.P, 0);
+        if (plan_engine(ki.plan) == ENG_MFMA)
+            hipLaunchKernelGGL((k_curlcurl<ENG_MFMA, STEP_LAMBDA>), grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        else hipLaunchKernelGGL((k_curlcurl<ENG_STOCKHAM, STEP_LAMBDA>), grid, dim3(FFT_THREADS), lds, c->stream, ki);
+        HIPCHK(hipGetLastError());
+        HIPCHK(fame_launch_mid(c->kmid(8, ik, nullptr, LIST_INV), +1, 8, 1, c->stream));
+        HIPCHK(fame_launch_outer(c->kout(8, ik, nullptr, LIST_INV), OUTER_QR, 8, 1, c->stream));
+        CHK(c->field_to_ref(c->F64 + (size_t)ik * N3, Y, 1, N3));
+        hipLaunchKernelGGL(k_diff_norm, dim3(NB, 3), dim3(256), 0, c->stream, D, Y, part, N);
+        HIPCHK(hipGetLastError());
+        std::vector<double> hp(3 * NB);
+        HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * NB, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < 3; ++i) {
+            double a = 0;
+            for (int t = 0; t < NB; ++t) a += hp[(size_t)i * NB + t];
+            err[i] = std::sqrt(a);
+        }
+        /* P Sigma Q^H x: Qrs, the singular values |Lambda|, Pr */
+        double2* q = (double2*)c->ref64[1];
+        CHK(c->field_from_ref(X, c->F64 + (size_t)ik * N3, 1));
+        CHK(c->qrs_field(q, ik, 1, 0, nullptr));
+        HIPCHK(fame_scale_real_mod(q, q, c->sig + (size_t)ik * N, 2 * N, 2 * N, N, N, 1, 8, c->stream));
+        CHK(c->qr_field(q, ik, 1, 1, nullptr));
+        CHK(c->field_to_ref(c->F64 + (size_t)ik * N3, Y, 1, N3));
+        hipLaunchKernelGGL(k_diff_norm, dim3(NB, 3), dim3(256), 0, c->stream, Cx, Y, part, N);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * NB, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        double a = 0;
+        for (double v : hp) a += v;
+        err[3] = std::sqrt(a);
+        return FAME_OK;
+    };
+    rs = run();
+    (void)hipStreamSynchronize(c->stream);
+    for (void* ptr : {(void*)X, (void*)D, (void*)Cx, (void*)Y, (void*)part}) if (ptr) (void)hipFree(ptr);
+    return rs;
 }

@@ -47,6 +47,7 @@ struct fame_ctx {
     /* wave vectors */
     int nk = 0;
     std::vector<fame::ProbDev> prob_h;
+    std::vector<double> kvec;    /* the wave vectors as given (3 per problem) */
     std::vector<int> nd;
     fame::ProbDev* prob_d = nullptr;
     double* sig = nullptr;       /* nk x N  |Lambda|          */

@@ -487,6 +487,7 @@ extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* 
     if (nk > c->maxb) FAIL(FAME_ERR_ARG, "fame_set_wave_vectors: more wave vectors than max_batch");
     HIPCHK(hipSetDevice(c->device));
     c->nk = nk;
+    c->kvec.assign(k, k + 3 * (size_t)nk);
     c->prob_h.resize(nk);
     c->nd.resize(nk);
     for (int i = 0; i < nk; ++i) {

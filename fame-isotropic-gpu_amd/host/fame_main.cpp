@@ -170,6 +170,22 @@ extern "C" int fame_main_code_run(const PAR* ParP, PROFILE* Profile) {
     for (int k0 = 0; k0 < Nk && st == FAME_OK; k0 += batch) {
         const int nb = std::min(batch, Nk - k0);
         if ((st = fame_set_wave_vectors(ctx, nb, Par.recip_lattice.WaveVector + 3 * (size_t)k0, nullptr)) != FAME_OK) break;
+        /* Check_Eigendecomp per wave vector (FAME_Main_Code.cu:100-102, 217-381),
+         * with a fixed seed instead of the time; the reference asserts, this
+         * returns FAME_ERR_STATE */
+        for (int p = 0; p < nb && st == FAME_OK; ++p) {
+            double ed[4];
+            if ((st = fame_check_eigendecomp(ctx, p, 0x5EEDull + (unsigned long long)(k0 + p), ed)) != FAME_OK) break;
+            std::printf("          EigDecomp_test_C1 = %e\n", ed[0]);
+            std::printf("          EigDecomp_test_C2 = %e\n", ed[1]);
+            std::printf("          EigDecomp_test_C3 = %e\n", ed[2]);
+            std::printf("          SVD_test_C        = %e\n", ed[3]);
+            if (ed[0] > 1e-6 || ed[1] > 1e-6 || ed[2] > 1e-6 || ed[3] > 1e-6) {
+                std::fprintf(stderr, "FAME_Main_Code: the eigen decomposition is not correct (wave vector %d)\n", k0 + p + 1);
+                st = FAME_ERR_STATE;
+            }
+        }
+        if (st != FAME_OK) break;
         std::vector<fame_profile> prof(nb);
         auto t0 = std::chrono::steady_clock::now();
         double* freq = &freq_all[(size_t)k0 * Nwant];

@@ -113,6 +113,8 @@ def lib():
     L.fame_eigs.argtypes = [vp, i, C.POINTER(fame_es), C.POINTER(fame_ls), vp, vp, vp, vp]
     L.fame_residuals.argtypes = [vp, i, vp, vp, vp]
     L.fame_residuals_noalloc.argtypes = [vp, i, vp, vp, vp]
+    L.fame_check_eigendecomp.argtypes = [vp, i, C.c_ulonglong, vp]
+    L.fame_curl_coo.argtypes = [C.POINTER(fame_lattice), vp, vp, vp, vp]
     L.fame_eig_grad.argtypes = [vp, i, vp, vp, vp, vp]
     L.fame_time_qbq_passes.argtypes = [vp, i, i, vp]
     L.fame_set_refinement.argtypes = [vp, i, d, i]
@@ -388,6 +390,13 @@ class Solver:
         _check(lib().fame_time_cg_passes(self._ctx, prec_bytes, reps, _ptr(us)), "fame_time_cg_passes")
         return us
 
+    def check_eigendecomp(self, ik: int, seed: int = 1):
+        """[||C_i x_i - T Lambda_i T* x_i|| for i = x, y, z, ||C x - P Sigma Q^H x||]
+        for a seeded random field x (the reference's Check_Eigendecomp)."""
+        err = np.zeros(4)
+        _check(lib().fame_check_eigendecomp(self._ctx, ik, seed, _ptr(err)), "fame_check_eigendecomp")
+        return err
+
     def residuals_noalloc(self, freq, E):
         """As residuals, through fame_residuals_noalloc (no device allocation;
         the fields are staged in the library's static device scratch)."""
@@ -427,6 +436,17 @@ class Solver:
         return g
 
 OP_INVAR_MIXED, OP_QBQ32 = 6, 7
+
+
+def curl_coo(lat, k):
+    """The sparse curl of FAME_Matrix_Curl for wave vector k: (rows, cols, complex vals), 12N entries."""
+    N = int(lat.grid_nums[0]) * int(lat.grid_nums[1]) * int(lat.grid_nums[2])
+    r = np.zeros(12 * N, np.int32)
+    c = np.zeros(12 * N, np.int32)
+    v = np.zeros(12 * N, np.complex128)
+    kk = np.ascontiguousarray(k, np.float64)
+    _check(lib().fame_curl_coo(C.byref(lat), _ptr(kk), _ptr(r), _ptr(c), _ptr(v)), "fame_curl_coo")
+    return r, c, v
 
 
 def eigs_many(solvers, nwant, nstep, tol=1e-12, maxit=300, ls_tol=1e-12, ls_maxit=1000, precision=PREC_F64,

@@ -152,6 +152,23 @@ fame_status fame_eigs_many(int n, fame_ctx* const* ctxs, const int* precision, c
  * Check_Residual (FAME_Main/FAME_Main_Code.cu:383), reported relative.       */
 fame_status fame_residuals(fame_ctx* ctx, int nwant, const double* freq, const void* ev_dev, double* res);
 
+/* GPU counterpart of Check_Eigendecomp (FAME_Main/FAME_Main_Code.cu:217-381)
+ * for wave vector ik of the context: with a random field x (real and imaginary
+ * parts uniform in [0, 1), drawn from `seed` instead of the reference's time
+ * seed), err[i] = ||C_i x_i - T Lambda_i T* x_i|| for i = x, y, z and
+ * err[3] = ||C x - P Sigma Q^H x||, absolute 2-norms as the reference prints
+ * them (EigDecomp_test_C1..C3, SVD_test_C).  C_i and C are the sparse forward
+ * differences and curl of FAME_Matrix_Curl applied as a stencil; the other
+ * side is this library's factorised operators.  Allocates 4 x 3N complex
+ * doubles of scratch for the call.                                           */
+fame_status fame_check_eigendecomp(fame_ctx* ctx, int ik, unsigned long long seed, double err[4]);
+
+/* The sparse curl C (3N x 3N) of FAME_Matrix_Curl (FAME_Main/FAME_Matrix_Curl.cpp:
+ * 90-443) for one wave vector k (3 doubles): 12N entries (host arrays rows,
+ * cols, and vals as interleaved complex doubles), in no particular order; the
+ * same rule as the stencil of fame_check_eigendecomp.                        */
+fame_status fame_curl_coo(const fame_lattice* lat, const double k[3], int* rows, int* cols, double* vals);
+
 /* fame_residuals without any device allocation, for callers that cannot
  * allocate (code under stream capture, tight serving loops, harnesses that
  * forbid hipMalloc).  No counterpart in the reference.  The fields are staged

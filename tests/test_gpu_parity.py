@@ -12,6 +12,7 @@ creation; the default picks the matrix cores for axis lengths 16..127).
 import os
 
 import numpy as np
+import scipy.sparse as sp
 import pytest
 
 import fame_amd as F
@@ -398,4 +399,37 @@ def test_operators_on_equal_mesh_general_grid(gpu, engine, shifts):
         assert _rel(S.apply(F.OP_PRS, ik, vd).cpu().numpy(), t.prs(v)) < 1e-11
         assert _rel(S.apply(F.OP_QBQ, ik, xd).cpu().numpy(), t.qbq(invB, x)) < 1e-11
         assert _rel(S.apply(F.OP_QBQ32, ik, xd).cpu().numpy(), t.qbq(invB, x)) < 2e-6
+    S.close()
+
+
+@pytest.mark.parametrize("sname", list(STRUCTS))
+def test_check_eigendecomp(gpu, engine, sname):
+    """fame_check_eigendecomp (the reference's Check_Eigendecomp on the GPU):
+    the sparse forward differences and curl against the factorised operators,
+    at a generic wave vector and at Gamma, below the reference's 1e-6 and at
+    FP64 rounding relative to ||C x||; and its sparse side equals the product's
+    COO curl applied on the host."""
+    import torch
+    lo, fl, ol, eps = make(sname)
+    ks = np.stack([KPTS["k"], KPTS["gamma"]])
+    S = F.Solver(fl, max_batch=2)
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    N = S.N
+    for ik, k in enumerate(ks):
+        err = S.check_eigendecomp(ik, seed=7 + ik)
+        r, c, v = F.curl_coo(fl, k)
+        Cm = sp.coo_matrix((v, (r, c)), shape=(3 * N, 3 * N)).tocsr()
+        x = np.random.default_rng(3).random(3 * N) + 1j * np.random.default_rng(4).random(3 * N)
+        scale = np.linalg.norm(Cm @ x)
+        assert np.all(err < 1e-6), (sname, ik, err)
+        assert np.all(err < 1e-13 * scale), (sname, ik, err, scale)
+        # the factorised side through the public operators: Pr(Sigma Qrs x) = C x
+        q = S.apply(F.OP_QRS, ik, torch.tensor(x, device=gpu)).cpu().numpy()
+        t = O.Tables(ol, k)
+        sig = t.get(3)
+        q[: t.Nd] *= sig
+        q[t.Nd:] *= sig
+        y = S.apply(F.OP_PR, ik, torch.tensor(q, device=gpu)).cpu().numpy()
+        assert np.linalg.norm(y - Cm @ x) <= 1e-13 * scale
     S.close()
