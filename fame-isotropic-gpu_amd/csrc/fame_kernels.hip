@@ -706,21 +706,23 @@ k_mid(KAxis args) {
     }
 }
 
-/* FP32 middle pass whose plan is one direct DFT on the matrix cores (n = nb <
+/* Middle pass whose plan is one direct DFT on the matrix cores (n = nb <
  * 128, no Bluestein), with the DFT's in-place prologue and its result step
  * fused into the memory phases of the pass:
- *   - the gather loads rows t and n - t of a pair column together and writes
- *     s_t = x_t + x_{n-t} and d_t = x_t - x_{n-t} (dft_mfma's first step) into
- *     the LDS lines, so the DFT starts on them after one barrier;
+ *   - the gather loads rows t and n - t of a column (FP32: a pair of columns)
+ *     together and writes s_t = x_t + x_{n-t} and d_t = x_t - x_{n-t}
+ *     (dft_mfma's first step) into the LDS lines, so the DFT starts on them
+ *     after one barrier;
  *   - each wave stores its outputs X_q and X_{n-q} from the accumulators
- *     straight to HBM (16 lanes write 16 consecutive lines of one row, 128
- *     bytes), with the middle twiddle, instead of through LDS and a store loop.
+ *     straight to HBM (16 lanes write 16 consecutive lines of one row), with
+ *     the middle twiddle, instead of through LDS and a store loop.
  * The A operands, K steps and result layout are those of dft_mfma.  The host
- * launches it with L a multiple of 16, na even, 16-byte aligned planes and one
- * wave per 16-row output tile (blockDim = 64 dqt).                          */
-template <int DIR>
-__global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_mid_dft32(KAxis args) {
-    typedef Mfma<float>::acc_t acc_t;
+ * launches it with L a multiple of 16, one wave per 16-row output tile
+ * (blockDim = 64 dqt), and for FP32 na even and 16-byte aligned planes.    */
+template <typename T, int DIR>
+__global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_eu(6))) k_mid_dft(KAxis args) {
+    typedef typename Vec<T>::T2 T2;
+    typedef typename Mfma<T>::acc_t acc_t;
     const int p = blockIdx.y / 3, comp = blockIdx.y - 3 * (blockIdx.y / 3);
     if (args.active && !args.active[p]) return;
     const int ntile = (args.g.na + args.L - 1) / args.L;
@@ -732,26 +734,27 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     const LinePlanDev& pl = args.plan;
     const StageDev& st = pl.st[0];
     const int na = g.na, n = g.nb, L = args.L, P = args.P;
-    float2* base = reinterpret_cast<float2*>(smem_raw);
-    const Tabs<float2> tb = load_tabs<float2>(pl, base);
-    float2* twb = base + lds_head(pl);
-    float2* S = twb + n;
+    T2* base = reinterpret_cast<T2*>(smem_raw);
+    const Tabs<T2> tb = load_tabs<T2>(pl, base);
+    T2* twb = base + lds_head(pl);
+    T2* S = twb + n;
     const int cf = tile / ntile, a0 = (tile - cf * ntile) * L;
-    float2* F = reinterpret_cast<float2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * n * na;
+    T2* F = reinterpret_cast<T2*>(args.f) + (size_t)p * args.fstride + (size_t)comp * g.N + (size_t)cf * n * na;
     const double phc = 2.0 * (pr.s0 + pr.s1 * (double)cf) / (double)n;
-    for (int b = threadIdx.x; b < n; b += blockDim.x) twb[b] = tw_expipi<float2>(phc * (double)b);
+    for (int b = threadIdx.x; b < n; b += blockDim.x) twb[b] = tw_expipi<T2>(phc * (double)b);
     __syncthreads();
     const int m = st.dm;
     const bool even = (n & 1) == 0;
     /* gather: unit u = 0 is row 0, u = 1..m the rows u and n - u, and for even
-     * n u = m + 1 the row n / 2; each thread keeps one pair column */
-    {
+     * n u = m + 1 the row n / 2; each thread keeps one column (FP32: a pair of
+     * columns, one 16-byte access) */
+    const int nu = m + 1 + (even ? 1 : 0);
+    if constexpr (sizeof(T) == 4) {
         const int Lh = L >> 1;
         const int ta = 2 * ((int)threadIdx.x % Lh), us = (int)blockDim.x / Lh;
         const bool aon = a0 + ta < na;
         const unsigned rowq = (unsigned)(na >> 1);
         const float4* Fa = reinterpret_cast<const float4*>(F + a0 + ta);
-        const int nu = m + 1 + (even ? 1 : 0);
         for (int u0 = (int)threadIdx.x / Lh; u0 < nu; u0 += 2 * us) {
             float4 x[2], y[2];
 #pragma unroll
@@ -791,6 +794,39 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
                 }
             }
         }
+    } else {
+        const int ta = (int)threadIdx.x % L, us = (int)blockDim.x / L;
+        const bool aon = a0 + ta < na;
+        const T2* Fa = F + a0 + ta;
+        for (int u0 = (int)threadIdx.x / L; u0 < nu; u0 += 2 * us) {
+            T2 x[2], y[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = u0 + k * us;
+                const int r1 = u <= m ? u : (n >> 1), r2 = (u >= 1 && u <= m) ? n - u : -1;
+                x[k].x = 0; x[k].y = 0;
+                y[k] = x[k];
+                if (u < nu && aon) {
+                    x[k] = Fa[(unsigned)(r1 * na)];
+                    if (r2 >= 0) y[k] = Fa[(unsigned)(r2 * na)];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int u = u0 + k * us;
+                if (u >= nu) continue;
+                const int r1 = u <= m ? u : (n >> 1), r2 = (u >= 1 && u <= m) ? n - u : -1;
+                T2 x0 = x[k], y0 = y[k];
+                if (DIR < 0) x0 = cmulc(x0, twb[r1]);
+                if (r2 >= 0) {
+                    if (DIR < 0) y0 = cmulc(y0, twb[r2]);
+                    S[ta * P + r1] = cadd(x0, y0);
+                    S[ta * P + r2] = csub(x0, y0);
+                } else {
+                    S[ta * P + r1] = x0;
+                }
+            }
+        }
     }
     __syncthreads();
     /* the DFT: wave w owns output rows 16 w .. 16 w + 15 */
@@ -802,39 +838,40 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
     const bool qon = q < Q;
     const int dr = (4 * q) % n;
     const int r0 = (q * (kq + 1)) % n;
-    float2* Fw = F + a0;
+    T2* Fw = F + a0;
     for (int g0 = 0; g0 < (L >> 4); ++g0) {
         const int line = g0 * 16 + row;
-        const float2* Sb = S + line * P;
+        const T2* Sb = S + line * P;
         acc_t cr = {0, 0, 0, 0}, ci = {0, 0, 0, 0}, sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
-        const float2 x0 = Sb[0];
-        float2 h = make_float2(0.f, 0.f);
+        const T2 x0 = Sb[0];
+        T2 h;
+        h.x = 0; h.y = 0;
         if (even) h = Sb[n >> 1];
         int r = r0;
         for (int ks = 0; ks < nks; ++ks) {
             const int t = min(ks * 4 + kq + 1, max(m, 1));
-            const float2 bs = Sb[t], bd = Sb[n - t];
-            float a_c = 0.f, a_s = 0.f;
+            const T2 bs = Sb[t], bd = Sb[n - t];
+            T a_c = 0, a_s = 0;
             if (qon && ks * 4 + kq + 1 <= m) {
-                const float2 wv = tb.W[r];
+                const T2 wv = tb.W[r];
                 a_c = wv.x; a_s = wv.y;
             }
             r += dr;
             if (r >= n) r -= n;
-            cr = Mfma<float>::mac(a_c, bs.x, cr);
-            ci = Mfma<float>::mac(a_c, bs.y, ci);
-            sr = Mfma<float>::mac(a_s, bd.x, sr);
-            si = Mfma<float>::mac(a_s, bd.y, si);
+            cr = Mfma<T>::mac(a_c, bs.x, cr);
+            ci = Mfma<T>::mac(a_c, bs.y, ci);
+            sr = Mfma<T>::mac(a_s, bd.x, sr);
+            si = Mfma<T>::mac(a_s, bd.y, si);
         }
         if (a0 + line < na) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int qq = w * 16 + Mfma<float>::row(kq, v);
+                const int qq = w * 16 + Mfma<T>::row(kq, v);
                 if (qq < Q) {
-                    const float Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
-                    const float Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
-                    const float Bx = -sr[v], By = -si[v];
-                    float2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
+                    const T Ax = x0.x + ((qq & 1) ? -h.x : h.x) + cr[v];
+                    const T Ay = x0.y + ((qq & 1) ? -h.y : h.y) + ci[v];
+                    const T Bx = -sr[v], By = -si[v];
+                    T2 lo, hi;   /* forward: X_q = A - iB, X_{n-q} = A + iB */
                     if (!inv) { lo.x = Ax + By; lo.y = Ay - Bx; hi.x = Ax - By; hi.y = Ay + Bx; }
                     else      { lo.x = Ax - By; lo.y = Ay + Bx; hi.x = Ax + By; hi.y = Ay - Bx; }
                     if (DIR > 0) lo = cmul(lo, twb[qq]);
@@ -1159,13 +1196,15 @@ static hipError_t launch_mid(const KAxis& a, int dir, int nprob, hipStream_t s) 
     const int ntile = (a.g.na + a.L - 1) / a.L;
     dim3 grid(xcd_round(ntile * a.g.nc), 3 * nprob);
     const size_t lds = fame_line_lds(a.plan, (int)sizeof(T), a.L, a.P, a.g.nb);
-    /* FP32 single direct DFT: the fused kernel (its preconditions, see there) */
-    if (ENG == ENG_MFMA && sizeof(T) == 4 && a.plan.nst == 1 && a.plan.st[0].R == DFT_MFMA && !a.plan.blue &&
-        a.plan.post == POST_NONE && a.plan.st[0].dn == a.g.nb && a.L % 16 == 0 && a.g.na % 2 == 0 &&
-        ((uintptr_t)a.f & 15) == 0 && a.fstride % 2 == 0 && a.plan.st[0].dqt <= FFT_THREADS / 64) {
+    /* a single direct DFT: the fused kernel (its preconditions, see there; FP32
+     * also needs na even and 16-byte aligned planes for its paired accesses) */
+    if (ENG == ENG_MFMA && a.plan.nst == 1 && a.plan.st[0].R == DFT_MFMA && !a.plan.blue &&
+        a.plan.post == POST_NONE && a.plan.st[0].dn == a.g.nb && a.L % 16 == 0 &&
+        a.plan.st[0].dqt <= FFT_THREADS / 64 &&
+        (sizeof(T) == 8 || (a.g.na % 2 == 0 && ((uintptr_t)a.f & 15) == 0 && a.fstride % 2 == 0))) {
         const dim3 blk(64 * a.plan.st[0].dqt);
-        if (dir > 0) hipLaunchKernelGGL(k_mid_dft32<1>, grid, blk, lds, s, a);
-        else hipLaunchKernelGGL(k_mid_dft32<-1>, grid, blk, lds, s, a);
+        if (dir > 0) hipLaunchKernelGGL((k_mid_dft<T, 1>), grid, blk, lds, s, a);
+        else hipLaunchKernelGGL((k_mid_dft<T, -1>), grid, blk, lds, s, a);
         return hipGetLastError();
     }
     /* the matrix-core DFT keeps one wave per (16-row output tile, 16-line group)
