@@ -896,28 +896,30 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
  * In EPS mode with Bluestein the inverse post-chirp and the forward pre-chirp
  * are conjugate unit factors whose product is 1; both are left out and only
  * the 1/M remains in the elementwise step.                                  */
-/* The FP32 outer pass's 1/eps step as a foldable stage of the Stockham
- * engine: mul() is its value for element c of line `line` (the scaled 1/eps
- * product, adding this thread's share of <v, eps^-1 v / N>), applied by the
- * first forward pass as it loads; operator() runs the same step as a pass of
- * its own (used only if no pass follows the step).                         */
-struct EpsFold32 {
+/* The outer pass's 1/eps step as a foldable stage of the Stockham engine:
+ * mul() is its value for element c of line `line` (the scaled 1/eps product,
+ * adding this thread's share of <v, eps^-1 v / N>), applied by the first
+ * forward pass as it loads; operator() runs the same step as a pass of its
+ * own (used only if no pass follows the step).                             */
+template <typename T>
+struct EpsFold {
+    typedef typename Vec<T>::T2 T2;
     static constexpr bool fold = true;
-    const float* invB;
+    const T* invB;
     int q0, plane, nc, L, P, Wd;
     unsigned mL;
-    float scale;
+    T scale;
     bool pap;
-    __device__ float2 mul(int line, int c, float2 v, double& acc) const {
+    __device__ T2 mul(int line, int c, T2 v, double& acc) const {
         const int q = q0 + line;
-        const float e = (c < nc && q < plane) ? invB[(unsigned)(c * plane + q)] : 0.f;
-        const float2 w = cscale(v, e * scale);
+        const T e = (c < nc && q < plane) ? invB[(unsigned)(c * plane + q)] : (T)0;
+        const T2 w = cscale(v, e * scale);
         if (pap) acc += (double)v.x * (double)w.x + (double)v.y * (double)w.y;
         return w;
     }
     /* the step as a pass of its own; its sum is lost, so a plan that needs
      * this (a step with no pass after it) must not ask for <p, Ap> */
-    __device__ void operator()(float2* Sx) const {
+    __device__ void operator()(T2* Sx) const {
         double acc = 0.0;
         for (int idx = threadIdx.x; idx < L * Wd; idx += blockDim.x) {
             const int c = udiv(idx, mL), tq = idx - c * L;
@@ -1037,8 +1039,8 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
          * pass (no LDS round trip of its own); 1/eps is shared by all problems,
          * so its scattered reads there are mostly served by the caches */
         bool folded = false;
-        if constexpr (sizeof(T) == 4 && ENG != ENG_MFMA) {
-            const EpsFold32 em{invB, q0, plane, nc, L, P, Wd, mL, scale, pap != nullptr};
+        if constexpr (ENG != ENG_MFMA) {
+            const EpsFold<T> em{invB, q0, plane, nc, L, P, Wd, mL, scale, pap != nullptr};
             vw += run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, em);
             folded = true;
         }
