@@ -122,6 +122,7 @@ def cpu_baseline(fl, eps, k, es_iter_per_k, nwant, nstep, reps=5):
     del U
     return {"value": 1.0 / per_k, "unit": "k-pts/s", "cores": cores, "kind": "port",
             "value_1thread": 1.0 / per_k1,
+            "qbq_applies_per_s": 1.0 / t_qbq, "qbq_applies_per_s_1thread": 1.0 / t_qbq1,
             "sample": f"FP64 CPU restatement (oracle/) at {tuple(fl.grid_nums)}: median of {reps} complete invAr "
                       f"solves on {cores} threads ({t_inv:.3f} s, {int(np.median(its))} CG iterations) and of {reps} "
                       f"Gram-Schmidt steps against {m} columns ({t_ro:.3f} s), x {es_iter_per_k} invAr per k-point "
@@ -259,9 +260,13 @@ def main():
         solve(S, F.PREC_F32_REFINE)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    step_s = []
     for _ in range(args.steps):
+        ts = time.perf_counter()
         S.set_wave_vectors(ks)
         freq_b, res_b, prof_b = solve(S, F.PREC_F32_REFINE)
+        torch.cuda.synchronize()
+        step_s.append(time.perf_counter() - ts)
     torch.cuda.synchronize()
     t_b = time.perf_counter() - t0
     kps_b = args.nk * args.steps / t_b
@@ -327,7 +332,7 @@ def main():
         solve(P, F.PREC_F64)                                                        # warm
         freq_p = np.zeros((plain_nk, args.nwant))
         res_p = np.zeros((plain_nk, args.nwant))
-        invar_per_k = []
+        invar_per_k, cg_per_k = [], []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for i in range(plain_nk):
@@ -335,6 +340,7 @@ def main():
             f, r, pr = solve(P, F.PREC_F64)
             freq_p[i], res_p[i] = f[0], r[0]
             invar_per_k.append(pr[0]["es_iter"])
+            cg_per_k.append(pr[0]["ls_iter"])
         torch.cuda.synchronize()
         t_p = time.perf_counter() - t0
         kps_p = plain_nk / t_p
@@ -364,6 +370,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1000.0 * t_b / args.steps,
+        "step_ms": [1000.0 * v for v in step_s],
+        "median_step_ms": 1000.0 * float(np.median(step_s)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -375,7 +383,10 @@ def main():
                    "global_batch": args.nk, "parallelism": "single GPU, all wave vectors in one batch"},
         "plain_fp64": {"value": kps_p, "unit": "k-pts/s", "nk_timed": plain_nk,
                        "speedup_batched_vs_plain": (kps_b / kps_p) if kps_p else None,
-                       "invar_per_k": qbq_per_k},
+                       "invar_per_k": qbq_per_k,
+                       "qbq_applies_per_s": (float(np.sum(cg_per_k)) / t_p) if kps_p else None},
+        "qbq_applies_per_s": {"batched": (prof_b[0]["qbq32"] + prof_b[0]["qbq64"]) * args.nk / (t_b / args.steps),
+                              "definition": "QBQ matvecs (one structure at one k each; FP32 and FP64) per second"},
         "max_rel_eig_err_vs_fp64": err,
         "max_rel_residual": {"batched": max_rel_res(freq_b, res_b)[1],
                              "plain_fp64": max_rel_res(freq_p, res_p)[1] if freq_p is not None else None,
