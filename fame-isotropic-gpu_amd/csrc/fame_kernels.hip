@@ -603,10 +603,11 @@ __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
  * faster on gfx950.  The others keep FFT_THREADS. */
 template <typename T, int ENG> struct MidThreads { static constexpr int v = (ENG == ENG_MFMA && sizeof(T) == 8) ? 512 : FFT_THREADS; };
 
-/* waves per SIMD the compiler must leave room for: the FP32 matrix-core
- * middle kernel then holds 51 VGPRs and no AGPRs (68 + 16 otherwise), and the
- * FP32 outer kernel 80 VGPRs (87), without spills; the FP32 outer pass ran 5%
- * faster on gfx950 at 96x83x78 */
+/* waves per SIMD the compiler must leave room for, without spills: the FP32
+ * general middle kernel on the matrix cores 6 (62 VGPRs, no AGPRs, instead of
+ * 68 + 16), with Stockham 5 (96), and the FP32 outer kernel 6 (80 VGPRs
+ * instead of 87; the FP32 outer pass ran 5% faster on gfx950 at 96x83x78).
+ * The FP64 kernels need more registers than any such bound leaves.        */
 template <typename T, int ENG> struct MidWaves { static constexpr int v = sizeof(T) == 4 ? (ENG == ENG_MFMA ? 6 : 5) : 1; };
 template <typename T> struct OuterWaves { static constexpr int v = sizeof(T) == 4 ? 6 : 1; };
 
