@@ -1,11 +1,12 @@
-"""The FP32 pass kernels under every code path that a grid and a tile choice
+"""The pass kernels under every code path that a grid and a tile choice
 select, against the FP64 oracle: two modes per thread with 16-byte accesses
 (even line lengths, planes and lines per block) or one mode per thread (odd
 ones), the fused matrix-core middle kernel (one direct DFT, 16 or 32 lines per
 block) or the general middle kernel (Stockham, or fewer lines), and the General
 gauge applied without forming the basis.  Tiles are fixed with set_tiles, so
 each case runs the configuration it names; the FP32 QBQ is held to 2e-6 of the
-oracle's FP64 QBQ, and an FP32-refined eigensolve to 1e-8 in lambda."""
+oracle's FP64 QBQ, the FP64 QBQ to 1e-11 (its fused matrix-core middle kernel
+among the paths), and an FP32-refined eigensolve to 1e-8 in lambda."""
 import itertools
 
 import numpy as np
@@ -41,9 +42,10 @@ def _setup(lt, nx, kw):
     return fl, O.lattice_from(fl), eps
 
 
-def _choices(S):
-    """Feasible FP32 tile sets: engine 0 / 1 and several lines per block for
-    each pass, combined so that every (pass, engine, lines) value occurs."""
+def _choices(S, prec=4):
+    """Feasible tile sets of one precision (4: FP32, 8: FP64): engine 0 / 1 and
+    several lines per block for each pass, combined so that every (pass,
+    engine, lines) value occurs."""
     opts = {}
     for name in ("inner", "mid", "outer"):
         opts[name] = []
@@ -51,7 +53,7 @@ def _choices(S):
             base = {"inner": (0, 1), "mid": (0, 1), "outer": (0, 1)}
             base[name] = (eng, L)
             try:
-                S.set_tiles(4, base)
+                S.set_tiles(prec, base)
             except RuntimeError:
                 continue
             opts[name].append((eng, L))
@@ -94,4 +96,25 @@ def test_fp32_refined_eigs_every_tile_path(gpu, lt, nx, kw):
         f, _, _, prof = S.eigs(6, 20, precision=F.PREC_F32_REFINE, want_fields=False)
         assert prof[0]["converged"] == 1, tiles
         assert np.max(np.abs(f[0] ** 2 - fo ** 2) / fo ** 2) < 1e-8, (tiles, f[0], fo)
+    S.close()
+
+
+@pytest.mark.parametrize("lt,nx,kw", GRIDS, ids=[f"{g[0]}-{g[1]}" for g in GRIDS])
+def test_fp64_qbq_every_tile_path(gpu, lt, nx, kw):
+    import torch
+    fl, ol, eps = _setup(lt, nx, kw)
+    ks = np.stack([np.array([0.21, -0.13, 0.07]), np.zeros(3)])
+    S = F.Solver(fl, max_batch=2)
+    S.set_eps(eps)
+    nd = S.set_wave_vectors(ks)
+    tabs = [O.Tables(ol, k) for k in ks]
+    xs = [_rand(2 * nd[i], 60 + i) for i in range(2)]
+    refs = [tabs[i].qbq(1.0 / eps, xs[i]) for i in range(2)]
+    sets = _choices(S, 8)
+    assert sets, "no feasible FP64 tiles"
+    for tiles in sets:
+        S.set_tiles(8, tiles)
+        for i in range(2):
+            y = S.apply(F.OP_QBQ, i, torch.tensor(xs[i], device=gpu)).cpu().numpy()
+            assert _rel(y, refs[i]) < 1e-11, (tiles, i, _rel(y, refs[i]))
     S.close()
