@@ -378,10 +378,19 @@ template <typename M, typename = void> struct FoldOf { static constexpr bool v =
 template <typename M> struct FoldOf<M, decltype((void)M::fold)> { static constexpr bool v = M::fold; };
 
 struct NoFoldMul { };
-/* returns the sum that a folded step's mul() accumulates (0 otherwise) */
-template <typename T2, int R, int MAXE, typename Mul = NoFoldMul>
+/* Lane order of a pass.  Along the line (LF = false: butterfly j fastest,
+ * then the line) or across the lines (LF = true: the line fastest, mNl the
+ * udiv magic of nl).  With an odd line pitch P, lanes on consecutive lines at
+ * one position hit distinct LDS banks, so LF makes every read and write of
+ * every stage conflict-free once a wave spans enough lines (32 lines for
+ * 8-byte elements); along the line, a stage with few butterflies per line
+ * (nb = 8 or 12 at M = 96) puts neighbouring lines on the same banks (P = 1
+ * mod 32) and its strided writes (Ns = 1) on every R-th bank.  LF also makes
+ * a folded step's per-element loads consecutive across lanes.
+ * returns the sum that a folded step's mul() accumulates (0 otherwise)     */
+template <typename T2, int R, int MAXE, typename Mul = NoFoldMul, bool LF = false>
 __device__ __forceinline__ double fft_pass(T2* S, int nl, int P, const StageDev& st, const Tabs<T2>& tb,
-                                           const Mul& mul = Mul(), bool fold = false) {
+                                           const Mul& mul = Mul(), bool fold = false, unsigned mNl = 0) {
     double facc = 0.0;
     constexpr int NBT = nbt_of(R, MAXE);
     const int nb = st.nb, Ns = st.Ns, tstep = st.tstep;
@@ -397,8 +406,19 @@ __device__ __forceinline__ double fft_pass(T2* S, int nl, int P, const StageDev&
         const int idx = threadIdx.x + i * FFT_THREADS;
         ob[i] = -1;
         if (idx < total) {
-            const int line = udiv(idx, st.mNb);
-            const int j = idx - line * nb;
+            int line, j;
+            if constexpr (LF) {
+                /* (line, j) do not depend on the stage here: keep the compiler
+                 * from hoisting them for all NBT butterflies out of the stage
+                 * loop, where they would stay live across it (and spill) */
+                int id = idx;
+                asm volatile("" : "+v"(id));
+                j = udiv(id, mNl);
+                line = id - j * nl;
+            } else {
+                line = udiv(idx, st.mNb);
+                j = idx - line * nb;
+            }
             const int qd = udiv(j, st.mNs);
             const int k = j - qd * Ns;
             const T2* a = S + line * P;
@@ -590,9 +610,9 @@ struct NoMid { template <typename T2> __device__ void operator()(T2*) const {} }
  * direct-DFT or all Stockham), so that each kernel variant is register-
  * allocated for its own engine only.                                        */
 enum { ENG_STOCKHAM = 0, ENG_MFMA = 1 };
-template <typename T2, int MAXE, int ENG, typename Mid>
+template <typename T2, int MAXE, int ENG, typename Mid, bool LF = false>
 __device__ __forceinline__ double run_stages(T2* S, int nl, int P, const LinePlanDev& pl, const Tabs<T2>& tb,
-                                             const Mid& mid) {
+                                             const Mid& mid, unsigned mNl = 0) {
     double facc = 0.0;   /* what a folded mid step accumulates */
     constexpr bool FOLD = FoldOf<Mid>::v && ENG != ENG_MFMA;
     bool fold_next = false;
@@ -612,23 +632,23 @@ __device__ __forceinline__ double run_stages(T2* S, int nl, int P, const LinePla
              * plans never use them, and their registers would cost waves) */
             constexpr bool CMP = MAXE >= 16;
             switch (st.R) {
-                case 2: facc += fft_pass<T2, 2, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 3: facc += fft_pass<T2, 3, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 4: facc += fft_pass<T2, 4, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 5: facc += fft_pass<T2, 5, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 7: facc += fft_pass<T2, 7, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 8: facc += fft_pass<T2, 8, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 11: facc += fft_pass<T2, 11, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                case 13: facc += fft_pass<T2, 13, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                case 2: facc += fft_pass<T2, 2, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 3: facc += fft_pass<T2, 3, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 4: facc += fft_pass<T2, 4, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 5: facc += fft_pass<T2, 5, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 7: facc += fft_pass<T2, 7, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 8: facc += fft_pass<T2, 8, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 11: facc += fft_pass<T2, 11, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                case 13: facc += fft_pass<T2, 13, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
                 default:
                     if constexpr (CMP) {
                         switch (st.R) {
-                            case 6: facc += fft_pass<T2, 6, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                            case 9: facc += fft_pass<T2, 9, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                            case 10: facc += fft_pass<T2, 10, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                            case 12: facc += fft_pass<T2, 12, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                            case 15: facc += fft_pass<T2, 15, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
-                            case 16: facc += fft_pass<T2, 16, MAXE, Mid>(S, nl, P, st, tb, mid, fm); break;
+                            case 6: facc += fft_pass<T2, 6, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                            case 9: facc += fft_pass<T2, 9, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                            case 10: facc += fft_pass<T2, 10, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                            case 12: facc += fft_pass<T2, 12, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                            case 15: facc += fft_pass<T2, 15, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
+                            case 16: facc += fft_pass<T2, 16, MAXE, Mid, LF>(S, nl, P, st, tb, mid, fm, mNl); break;
                             default: break;
                         }
                     }

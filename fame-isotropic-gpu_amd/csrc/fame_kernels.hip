@@ -67,6 +67,13 @@ template <> __device__ __forceinline__ float2 tw_expipi<float2>(double x) {
  * instance, so that registers are allocated for the one form it runs.      */
 enum { PI_ANY = 0, PI_GUARD = 1, PI_GEN = 2 };
 
+/* Stockham lane order of the outer pass (fft_pass): across its L lines in
+ * FP64, along them in FP32.  On the FCC 96 grid (L = 16, M = 96) it took the
+ * FP64 outer pass from 564 to 489 us; the FP32 one (L = 32) kept its time at
+ * 5 waves per SIMD and spilled at its 6.  The inner passes, with 3L = 18 FP64
+ * lines per block, were 1-3 us slower with it (gfx950).                    */
+template <typename T> struct OuterLF { static constexpr bool v = sizeof(T) == 8; };
+
 template <typename T, int EXPAND, int ENG, int PIK>
 __global__ void __launch_bounds__(FFT_THREADS) k_inner(KInner args) {
     constexpr bool GUARD = PIK == PI_GUARD;
@@ -1042,10 +1049,10 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
         bool folded = false;
         if constexpr (ENG != ENG_MFMA) {
             const EpsFold<T> em{invB, q0, plane, nc, L, P, Wd, mL, scale, pap != nullptr};
-            vw += run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, em);
+            vw += run_stages<T2, MaxE<T>::v, ENG, EpsFold<T>, OuterLF<T>::v>(S, L, P, pl, tb, em, args.mL);
             folded = true;
         }
-        if (!folded) run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, eps_step);
+        if (!folded) run_stages<T2, MaxE<T>::v, ENG, decltype(eps_step), OuterLF<T>::v>(S, L, P, pl, tb, eps_step, args.mL);
         if (pap) {
             /* block sum in a fixed order: wave shuffles, then the wave sums */
             __shared__ double red[FFT_THREADS / 64];
@@ -1059,7 +1066,7 @@ __global__ void __launch_bounds__(FFT_THREADS) __attribute__((amdgpu_waves_per_e
             }
         }
     } else {
-        run_stages<T2, MaxE<T>::v, ENG>(S, L, P, pl, tb, NoMid());
+        run_stages<T2, MaxE<T>::v, ENG, NoMid, OuterLF<T>::v>(S, L, P, pl, tb, NoMid(), args.mL);
     }
     const double invN = 1.0 / (double)g.N;
     if constexpr (sizeof(T) == 4) {
