@@ -112,7 +112,7 @@ struct fame_ctx {
      * its pending update) on the first pass, then QBQ64 xr, whose last pass
      * stores the next FP32 solve's start r = p = FP32(b - QBQ64 xr), x = 0, and
      * its norm partials (partials[problem][inner block]) */
-    fame_status qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act);
+    fame_status qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act, int nact);
     /* x: problem stride xstride (0: 2N); out (optional): the result is written
      * there, problem stride ostride, for the problems of mask_dev only, and y
      * serves as the contiguous solve buffer; mask_host: a host copy of

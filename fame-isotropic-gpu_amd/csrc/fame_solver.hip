@@ -564,7 +564,8 @@ fame_status fame_ctx::qbq(int prec, const void* x, void* y, int p0, int nprob, c
     return FAME_OK;
 }
 
-fame_status fame_ctx::qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act) {
+fame_status fame_ctx::qbq_residual32(const void* xr, const void* b, void* x32, int p0, int nprob, const int* act,
+                                     int nact) {
     const size_t es = 8 * 2 * (size_t)N;   /* FP32 complex vector bytes per problem */
     KInner ki = kin(8, p0, act, LIST_INV);
     ki.x = xr; ki.pikind = 0;
@@ -579,7 +580,7 @@ fame_status fame_ctx::qbq_residual32(const void* xr, const void* b, void* x32, i
     kc.r32 = (char*)cg32[0] + es * p0; kc.p32 = (char*)cg32[1] + es * p0;
     kc.rpart = partials;
     HIPCHK(fame_launch_inner(kc, 0, 8, nprob, stream));
-    cnt_qbq64 += nprob;
+    cnt_qbq64 += nact;
     return FAME_OK;
 }
 
@@ -816,7 +817,13 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
     }
     if (!defer_finish) HIPCHK(fame_cg_finish_x(x, p, cgsc, n, st, nprob, prec, stream));
     int mx = 0;
-    for (int i = 0; i < nprob; ++i) { if (iters) iters[i] = sc_h[i].iters; mx = std::max(mx, sc_h[i].iters); }
+    long long applied = 0;   /* QBQ applications: iters - 1 per problem (masked ones make none) */
+    for (int i = 0; i < nprob; ++i) {
+        if (iters) iters[i] = sc_h[i].iters;
+        mx = std::max(mx, sc_h[i].iters);
+        applied += sc_h[i].iters - 1;
+    }
+    if (prec == 8) cnt_qbq64 += applied; else cnt_qbq32 += applied;
     cg_predict[pix(prec)] = std::max(2, mx - 1);
     return FAME_OK;
 }
@@ -825,7 +832,9 @@ fame_status fame_ctx::cg(int prec, const void* b, void* x, int p0, int nprob, do
  * previous iteration's x, r and p updates and whose last pass sums <p,Ap> and
  * <Ap,Ap>, then alpha, beta and the flags (six kernels) */
 fame_status fame_ctx::cg_step_ap(int prec, void* p, void* r, void* x, void* Ap, int p0, int nprob) {
+    const long long c64 = cnt_qbq64, c32 = cnt_qbq32;
     CHK(qbq(prec, p, Ap, p0, nprob, active, p, r, x, Ap, true));
+    cnt_qbq64 = c64; cnt_qbq32 = c32;
     HIPCHK(fame_cg_alpha_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, stream));
     return FAME_OK;
 }
@@ -851,7 +860,6 @@ fame_status fame_ctx::cg_step_rc(int prec, void* p, void* r, void* x, int p0, in
     kc.cg_r = r; kc.cgsc = cgsc; kc.partials = partials2;
     HIPCHK(fame_launch_inner(kc, 0, prec, nprob, stream));
     HIPCHK(fame_cg_beta(cgsc, partials2, nblk_in[pix(prec)], nprob, active, stream));
-    if (prec == 8) cnt_qbq64 += nprob; else cnt_qbq32 += nprob;
     return FAME_OK;
 }
 
@@ -886,9 +894,7 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int cou
     if (itg == cg_graphs.end()) {
         if (!cap_stream) HIPCHK(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
         HIPCHK(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
-        const long long c64 = cnt_qbq64, c32 = cnt_qbq32;
         fame_status q = launch(cap_stream);
-        cnt_qbq64 = c64; cnt_qbq32 = c32;
         hipGraph_t gr = nullptr;
         hipError_t e = hipStreamEndCapture(cap_stream, &gr);
         if (q != FAME_OK) { if (gr) (void)hipGraphDestroy(gr); return q; }
@@ -900,7 +906,6 @@ fame_status fame_ctx::cg_iteration(int prec, void* x, int p0, int nprob, int cou
         itg = cg_graphs.emplace(key, ex).first;
     }
     HIPCHK(hipGraphLaunch(itg->second, stream));
-    if (prec == 8) cnt_qbq64 += (long long)nprob * count; else cnt_qbq32 += (long long)nprob * count;
     return FAME_OK;
 }
 
@@ -965,7 +970,7 @@ fame_status fame_ctx::invar(int mode, const void* x, void* y, int p0, int nprob,
             CHK(cg(4, b, d32, p0, nprob, ls.tol, refine_reltol, ls.maxit, mask, it32.data(), true, true, nbp));
             for (int i = 0; i < nprob; ++i) if (msk[i]) it[i] += it32[i] - 1;
             /* the next sweep's start from r = b - QBQ64 xr, with its norm */
-            CHK(qbq_residual32(xr, b, d32, p0, nprob, mask));
+            CHK(qbq_residual32(xr, b, d32, p0, nprob, mask, (int)std::count(msk.begin(), msk.end(), 1)));
             nbp = nblk_in[pix(8)];
         }
         for (int i = 0; i < nprob; ++i) it[i] += 1;

@@ -255,6 +255,31 @@ def test_batch_with_uneven_convergence(gpu):
     S.close()
 
 
+def test_qbq_counts_are_applications(gpu):
+    """The profile's QBQ counts are operator applications, not launch slots:
+    in a batch whose problems finish their CG solves at different iterations,
+    the FP64 count equals the CG iterations (ls_iter minus one per solve) and
+    the FP32 count of the refined solve equals its FP32 iterations, averaged
+    over the batch's wave vectors as the profile reports them."""
+    lo, fl, ol, eps = make("fcc")
+    ks = np.array([[0.21, -0.13, 0.07], [0.0, 0.0, 0.0], [0.4, 0.1, 0.2], [0.33, 0.33, 0.0]])
+    S = F.Solver(fl, max_batch=len(ks))
+    S.set_eps(eps)
+    S.set_wave_vectors(ks)
+    for prec in (F.PREC_F64, F.PREC_F32_REFINE):
+        _, _, _, prof = S.eigs(8, 14, precision=prec, want_fields=False)
+        cg_its = sum(p["ls_iter"] - p["es_iter"] for p in prof)
+        solves = sum(p["es_iter"] for p in prof)
+        n = len(ks)
+        if prec == F.PREC_F64:
+            assert prof[0]["qbq64"] == cg_its // n and prof[0]["qbq32"] == 0, (prof[0], cg_its)
+        else:
+            assert prof[0]["qbq32"] == cg_its // n, (prof[0], cg_its)
+            # one FP64 residual pass per sweep, at least one sweep per solve
+            assert solves // n <= prof[0]["qbq64"] < prof[0]["qbq32"], (prof[0], solves)
+    S.close()
+
+
 def _align_err(u, v):
     """|| u/|u| - phase * v/|v| || for the best unit phase."""
     u = u / np.linalg.norm(u)
