@@ -223,6 +223,9 @@ def main():
     ap.add_argument("--cg-only", action="store_true",
                     help="with --passes-only: time only the kernels of one fused FP32 CG iteration (for counter runs "
                          "of those kernels alone)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the warm-up and timed batched solves, one short JSON line (for kernel traces of "
+                         "the timed step)")
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--tiles", type=str, default=None,
                     help="with --passes-only: fixed tiles instead of tuning, 'fp32:e,L,e,L,e,L;fp64:e,L,e,L,e,L' "
@@ -271,6 +274,13 @@ def main():
     t_b = time.perf_counter() - t0
     kps_b = args.nk * args.steps / t_b
     log(f"batched FP32+refine: {t_b:.2f}s for {args.steps} step(s) -> {kps_b:.3f} k-pts/s")
+    if args.batched_only:
+        S.close()
+        print(json.dumps({"metric": "k-pts/s (batched FP32+refine only)", "value": kps_b, "unit": "k-pts/s",
+                          "steps": args.steps, "warmup": args.warmup, "step_ms": [1000.0 * v for v in step_s],
+                          "qbq32_per_k": prof_b[0]["qbq32"], "qbq64_per_k": prof_b[0]["qbq64"],
+                          "es_iter_mean": float(np.mean([p["es_iter"] for p in prof_b]))}))
+        return
 
     # ---- the same batched solve with two inverse-iteration sweeps of the Ritz
     # pairs (fame_set_refinement), outside the timed steps: its cost and the field
