@@ -78,6 +78,20 @@ struct fame_ctx {
     int refine_max_sweeps = 8;
     double refine_reltol = 1e-6;
     fame_ls ls_last = {0, 0.0};
+    /* fame_eigs on a large batch: the wave vectors are split over child
+     * contexts of the same structure (kids[i] holds wave vectors kid_lo[i] ..
+     * kid_lo[i+1]-1), solved concurrently as by fame_eigs_many, with the
+     * parent's pass tiles; groups: 0 auto, 1 one batch, g >= 2 that many.
+     * eps_ver / k_ver count fame_set_eps / fame_set_wave_vectors calls, so the
+     * children are brought up to date only when the parent changed.  A child
+     * sizes its Gram partial sums for gram_batch problems (the parent's batch),
+     * so that its arithmetic is the parent's. */
+    int groups = 0;
+    bool owns_stream = false;     /* a child's stream, destroyed with it */
+    std::vector<fame_ctx*> kids;
+    std::vector<int> kid_lo;
+    int eps_ver = 0, k_ver = 0, kids_eps_ver = -1, kids_k_ver = -1;
+    int gram_batch = 0;
     long long cnt_qbq64 = 0, cnt_qbq32 = 0;
     int last_iters = 0;
     /* one CG iteration captured as a hipGraph per (precision, first problem,

@@ -93,7 +93,7 @@ fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
     CHK(al((void**)&b.hh, sizeof(double) * 2 * m * nk));
     CHK(al((void**)&b.nrm, sizeof(double) * nk));
     CHK(al((void**)&b.invb, sizeof(double) * nk));
-    size_t gp = std::max((size_t)GEMV_BLK_GS * 2 * m, (size_t)gram_blk(nk) * 2 * m * m);
+    size_t gp = std::max((size_t)GEMV_BLK_GS * 2 * m, (size_t)gram_blk(c->gram_batch > 0 ? c->gram_batch : nk) * 2 * m * m);
     CHK(al((void**)&b.gpart, sizeof(double) * gp * nk));
     CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
     CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
@@ -247,9 +247,148 @@ extern "C" fame_status fame_set_refinement(fame_ctx* c, int sweeps, double tol, 
     return FAME_OK;
 }
 
+/* Splitting a batch over concurrent child contexts.  The problems of one batch
+ * iterate in lockstep: each CG solve ends with launches in which only the
+ * slowest few problems still iterate, and the Lanczos vector work and host
+ * checks leave the passes' share of the GPU idle meanwhile.  Two halves of the
+ * batch, each a context on its own stream, fill those gaps with each other's
+ * passes (FCC 96, 30 wave vectors: 4.70 against 4.49 k-pts/s on one MI355X).
+ * Every problem's arithmetic is independent of the batch it runs in, and the
+ * children take the parent's tiles and Gram blocking, so the results are
+ * bit-identical to one batch.                                              */
+static int split_groups(const fame_ctx* c) {
+    const int g = c->groups > 0 ? c->groups : (c->nk >= 16 ? 2 : 1);
+    return std::max(1, std::min(g, c->nk));
+}
+
+/* children for G groups, in step with the parent; false (and no children)
+ * when the device lacks the memory for them next to the solve itself */
+static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok) {
+    *ok = false;
+    std::vector<int> lo(G + 1);
+    for (int i = 0; i <= G; ++i) lo[i] = (int)((long long)c->nk * i / G);
+    bool remake = (int)c->kids.size() != G;
+    for (int i = 0; !remake && i < G; ++i) remake = c->kids[i]->maxb < lo[i + 1] - lo[i];
+    if (remake) {
+        for (fame_ctx* k : c->kids) (void)fame_destroy(k);
+        c->kids.clear();
+        c->kid_lo.clear();
+        /* a child holds the per-problem work vectors of its wave vectors
+         * (about 400 N bytes each) and its own permittivity; the Lanczos
+         * buffers (U, W) move from the parent to the children */
+        size_t freeb = 0, totalb = 0;
+        HIPCHK(hipMemGetInfo(&freeb, &totalb));
+        const size_t N = c->N, n2 = 2 * N;
+        const size_t lz = 16 * n2 * ((size_t)nstep + 1 + (size_t)nwant + 1) * (size_t)c->nk;
+        const size_t need = 400 * N * (size_t)c->nk + 100 * N * (size_t)G + lz + ((size_t)1 << 30);
+        if (freeb < need) return FAME_OK;
+        for (int i = 0; i < G; ++i) {
+            fame_ctx* k = nullptr;
+            const fame_status st = fame_create(&c->lat, lo[i + 1] - lo[i], c->device, &k);
+            if (st != FAME_OK) {
+                for (fame_ctx* q : c->kids) (void)fame_destroy(q);
+                c->kids.clear();
+                return st == FAME_ERR_NOMEM || st == FAME_ERR_HIP ? FAME_OK : st;
+            }
+            c->kids.push_back(k);
+            /* its own stream for good (a new stream would drop its CG graphs) */
+            hipStream_t s = nullptr;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+                for (fame_ctx* q : c->kids) (void)fame_destroy(q);
+                c->kids.clear();
+                return FAME_OK;
+            }
+            k->stream = s;
+            k->owns_stream = true;
+        }
+        c->kids_eps_ver = c->kids_k_ver = -1;
+    }
+    if (c->kids_eps_ver != c->eps_ver) {
+        std::vector<double> e(3 * (size_t)c->N);
+        HIPCHK(hipMemcpyAsync(e.data(), c->eps_ref, sizeof(double) * e.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (fame_ctx* k : c->kids) CHK(fame_set_eps(k, e.data()));
+        c->kids_eps_ver = c->eps_ver;
+    }
+    if (c->kids_k_ver != c->k_ver || c->kid_lo != lo) {
+        for (int i = 0; i < G; ++i) CHK(fame_set_wave_vectors(c->kids[i], lo[i + 1] - lo[i], c->kvec.data() + 3 * lo[i], nullptr));
+        c->kid_lo = lo;
+        c->kids_k_ver = c->k_ver;
+    }
+    for (fame_ctx* k : c->kids) {
+        bool same = true;
+        for (int pi = 0; pi < 2; ++pi)
+            for (int a = 0; a < 3; ++a)
+                same = same && k->tile[a][pi].eng == c->tile[a][pi].eng && k->tile[a][pi].L == c->tile[a][pi].L;
+        if (!same) {
+            for (int pi = 0; pi < 2; ++pi) {
+                for (int a = 0; a < 3; ++a) k->tile[a][pi] = c->tile[a][pi];
+                k->nblk_in[pi] = c->nblk_in[pi];
+            }
+            k->drop_graphs();
+        }
+        k->tuned[0] = k->tuned[1] = true;    /* the parent's choice, tuned or fixed */
+        k->groups = 1;
+        k->gram_batch = c->nk;
+        k->use_graphs = c->use_graphs;
+        k->ritz_sweeps = c->ritz_sweeps; k->ritz_tol = c->ritz_tol; k->ritz_maxit = c->ritz_maxit;
+        k->refine_max_sweeps = c->refine_max_sweeps; k->refine_reltol = c->refine_reltol;
+    }
+    *ok = true;
+    return FAME_OK;
+}
+
 extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, const fame_ls* ls, double* freq,
                                  void* evr_dev, void* E_dev, fame_profile* prof) {
-    return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
+    if (!c || !es || !ls || !freq) FAIL(FAME_ERR_ARG, "fame_eigs: null argument");
+    const int G = (c->have_eps && c->nk >= 1) ? split_groups(c) : 1;
+    if (G <= 1 || es->nwant < 1 || es->nstep <= es->nwant || es->nstep + 1 > FAME_LZ_MAXCOL ||
+        (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE))
+        return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
+    HIPCHK(hipSetDevice(c->device));
+    /* the parent's tiles, tuned on the whole batch as a single solve would */
+    if (c->autotune) {
+        if (!c->tuned[0]) CHK(c->tune(8));
+        if (precision == FAME_PREC_F32_REFINE && !c->tuned[1]) CHK(c->tune(4));
+    }
+    bool ok = false;
+    CHK(sync_kids(c, G, es->nstep, es->nwant, &ok));
+    if (!ok) return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
+    /* the Lanczos buffers of an earlier single-batch solve are not needed now */
+    for (void* p : c->big) if (p) (void)hipFree(p);
+    c->big.clear();
+    HIPCHK(hipStreamSynchronize(c->stream));   /* work the caller queued on the parent */
+    c->ls_last = *ls;
+    const int nwant = es->nwant;
+    const size_t n2 = 2 * (size_t)c->N, n3 = 3 * (size_t)c->N;
+    std::vector<int> precs(G, precision);
+    std::vector<fame_es> ess(G, *es);
+    std::vector<fame_ls> lss(G, *ls);
+    std::vector<double*> fr(G);
+    std::vector<void*> ev(G), Ev(G);
+    std::vector<fame_profile*> pr(G);
+    long long q64 = 0, q32 = 0;
+    for (int i = 0; i < G; ++i) {
+        const size_t lo = c->kid_lo[i];
+        fr[i] = freq + lo * nwant;
+        ev[i] = evr_dev ? (void*)((double2*)evr_dev + lo * nwant * n2) : nullptr;
+        Ev[i] = E_dev ? (void*)((double2*)E_dev + lo * nwant * n3) : nullptr;
+        pr[i] = prof ? prof + lo : nullptr;
+        q64 -= c->kids[i]->cnt_qbq64; q32 -= c->kids[i]->cnt_qbq32;
+    }
+    CHK(fame_eigs_many(G, c->kids.data(), precs.data(), ess.data(), lss.data(), fr.data(), evr_dev ? ev.data() : nullptr,
+                       E_dev ? Ev.data() : nullptr, prof ? pr.data() : nullptr));
+    /* the children's streams are their own: their fields are complete when this returns */
+    for (fame_ctx* k : c->kids) HIPCHK(hipStreamSynchronize(k->stream));
+    for (fame_ctx* k : c->kids) { q64 += k->cnt_qbq64; q32 += k->cnt_qbq32; }
+    c->cnt_qbq64 += q64; c->cnt_qbq32 += q32;
+    return FAME_OK;
+}
+
+extern "C" fame_status fame_set_groups(fame_ctx* c, int groups) {
+    if (!c || groups < 0) FAIL(FAME_ERR_ARG, "fame_set_groups: need groups >= 0 (0: automatic)");
+    c->groups = groups;
+    return FAME_OK;
 }
 
 /* Several structures (contexts of any grid, batch and band count) solved in
@@ -349,8 +488,9 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
         /* Gram matrices of the bases that restart */
         HIPCHK(hipMemcpyAsync(b.mask, rmask.data(), sizeof(int) * nk, hipMemcpyHostToDevice, c->stream));
         const size_t ps = b.n2 * m;
-        HIPCHK(fame_gram(b.U, b.gpart, gram_blk(nk), m, b.n2, b.n2, ps, nk, b.mask, c->stream));
-        HIPCHK(fame_reduce_partials(b.gpart, gram_blk(nk), 2 * m * m, b.G, nk, c->stream));
+        const int gblk = gram_blk(c->gram_batch > 0 ? c->gram_batch : nk);
+        HIPCHK(fame_gram(b.U, b.gpart, gblk, m, b.n2, b.n2, ps, nk, b.mask, c->stream));
+        HIPCHK(fame_reduce_partials(b.gpart, gblk, 2 * m * m, b.G, nk, c->stream));
         HIPCHK(hipMemcpyAsync(G.data(), b.G, sizeof(double) * 2 * m * m * nk, hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
         for (int p = 0; p < nk; ++p) {

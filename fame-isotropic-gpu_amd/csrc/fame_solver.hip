@@ -397,6 +397,13 @@ extern "C" fame_status fame_create(const fame_lattice* lat, int max_batch, int d
 
 extern "C" fame_status fame_destroy(fame_ctx* c) {
     if (!c) return FAME_OK;
+    for (fame_ctx* k : c->kids) (void)fame_destroy(k);
+    c->kids.clear();
+    if (c->owns_stream && c->stream) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        c->stream = nullptr;
+    }
     if (c->sc_h) (void)hipHostFree(c->sc_h);
     delete c;
     return FAME_OK;
@@ -450,6 +457,7 @@ extern "C" fame_status fame_set_eps(fame_ctx* c, const double* eps) {
     HIPCHK(hipMemcpyAsync(c->invB32, ib32.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->have_eps = true;
+    ++c->eps_ver;
     return FAME_OK;
 }
 
@@ -499,6 +507,7 @@ extern "C" fame_status fame_set_wave_vectors(fame_ctx* c, int nk, const double* 
     HIPCHK(hipMemcpyAsync(c->prob_d, c->prob_h.data(), sizeof(ProbDev) * nk, hipMemcpyHostToDevice, c->stream));
     HIPCHK(fame_launch_sigma(c->g, c->prob_d, c->sig, c->sig_inv, nk, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    ++c->k_ver;
     return FAME_OK;
 }
 

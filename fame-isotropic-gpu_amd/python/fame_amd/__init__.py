@@ -118,6 +118,7 @@ def lib():
     L.fame_eig_grad.argtypes = [vp, i, vp, vp, vp, vp]
     L.fame_time_qbq_passes.argtypes = [vp, i, i, vp]
     L.fame_set_refinement.argtypes = [vp, i, d, i]
+    L.fame_set_groups.argtypes = [vp, i]
     L.fame_time_cg_passes.argtypes = [vp, i, i, vp]
     L.fame_time_qbq_passes.restype = C.c_int
     L.fame_set_stream.argtypes = [vp, vp]
@@ -365,6 +366,11 @@ class Solver:
         """Inverse-iteration sweeps that eigs applies to the converged Ritz pairs
         (0: the reference's algorithm); see fame_set_refinement."""
         _check(lib().fame_set_refinement(self._ctx, int(sweeps), float(tol), int(maxit)), "fame_set_refinement")
+
+    def set_groups(self, groups: int):
+        """How eigs runs the batch: 1 one batch, g >= 2 that many concurrent groups,
+        0 (default) two groups from 16 wave vectors on; see fame_set_groups."""
+        _check(lib().fame_set_groups(self._ctx, int(groups)), "fame_set_groups")
 
     def restore(self, evr):
         """E fields (nk, nwant, 3N) recomputed from reduced Ritz vectors (nk, nwant, 2N)."""

@@ -130,6 +130,19 @@ fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fam
  * solves are counted in the profile.  No reference counterpart.             */
 fame_status fame_set_refinement(fame_ctx* ctx, int sweeps, double tol, int maxit);
 
+/* How fame_eigs runs a batch: groups = 1 as one batch; g >= 2 split into g
+ * groups of consecutive wave vectors, each solved by a child context of the
+ * same structure on its own stream, concurrently (as fame_eigs_many does for
+ * several structures), so that one group's CG tails and vector work overlap
+ * another's passes; 0 (default) splits batches of 16 or more wave vectors in
+ * two.  The children use the parent's pass tiles and arithmetic, so the
+ * results are bit-identical to one batch.  They hold the per-problem work
+ * vectors of their wave vectors (about 400 N bytes each) until the context is
+ * destroyed; when the device lacks that memory, fame_eigs runs one batch.
+ * Replaces the reference's one-wave-vector-at-a-time loop in FAME_Main_Code
+ * (FAME_Main_Code.cu:79-131) as a batching policy; no reference counterpart. */
+fame_status fame_set_groups(fame_ctx* ctx, int groups);
+
 /* Restored E fields (device, nk*nwant*3N, reference scaling and Gamma rule, as
  * fame_eigs' E_dev) recomputed from the reduced Ritz vectors evr_dev (device,
  * nk*nwant*2N, as fame_eigs' evr_dev) with the permittivity and wave vectors

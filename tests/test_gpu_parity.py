@@ -280,6 +280,37 @@ def test_qbq_counts_are_applications(gpu):
     S.close()
 
 
+def test_split_batch_is_bit_identical(gpu):
+    """fame_eigs splits a batch of 16 or more wave vectors over concurrent child
+    contexts (fame_set_groups): one batch, the automatic two groups and three
+    groups give bit-identical eigenvalues, fields and iteration counts, in both
+    precisions, and the children follow the parent's new wave vectors (another
+    partition) and new permittivity."""
+    import torch
+    lo, fl, ol, eps = make("fcc")
+    rng = np.random.default_rng(11)
+    ks = np.vstack([np.zeros(3), rng.uniform(-0.4, 0.4, (17, 3))])
+    S = F.Solver(fl, max_batch=len(ks))
+    S.set_eps(eps)
+
+    def run(groups, prec):
+        S.set_groups(groups)
+        f, _, E, prof = S.eigs(8, 14, precision=prec, want_fields=True)
+        return f, E.cpu(), [p["ls_iter"] for p in prof]
+
+    for kk, e in ((ks, eps), (ks[:16], eps), (ks[:16], 1.0 + 0.5 * eps)):
+        S.set_eps(e)
+        S.set_wave_vectors(kk)
+        for prec in (F.PREC_F64, F.PREC_F32_REFINE):
+            f1, E1, it1 = run(1, prec)
+            for g in (0, 3):
+                f2, E2, it2 = run(g, prec)
+                assert np.array_equal(f1, f2), (len(kk), prec, g, np.max(np.abs(f1 - f2)))
+                assert torch.equal(E1, E2), (len(kk), prec, g)
+                assert it1 == it2, (len(kk), prec, g)
+    S.close()
+
+
 def _align_err(u, v):
     """|| u/|u| - phase * v/|v| || for the best unit phase."""
     u = u / np.linalg.norm(u)
