@@ -67,6 +67,7 @@ struct fame_ctx {
     CgScal* sc_h = nullptr;       /* pinned host copy of cgsc */
     std::vector<void*> allocs;
     std::vector<void*> big;       /* Lanczos basis etc., reallocated per solve shape */
+    std::vector<size_t> big_bytes;  /* their sizes (the shape they were allocated for) */
     /* tuning / state */
     int cg_predict[2] = {40, 40};
     /* inverse-iteration sweeps applied by fame_eigs to the converged Ritz pairs
@@ -79,19 +80,22 @@ struct fame_ctx {
     double refine_reltol = 1e-6;
     fame_ls ls_last = {0, 0.0};
     /* fame_eigs on a large batch: the wave vectors are split over child
-     * contexts of the same structure (kids[i] holds wave vectors kid_lo[i] ..
-     * kid_lo[i+1]-1), solved concurrently as by fame_eigs_many, with the
-     * parent's pass tiles; groups: 0 auto, 1 one batch, g >= 2 that many.
-     * eps_ver / k_ver count fame_set_eps / fame_set_wave_vectors calls, so the
-     * children are brought up to date only when the parent changed.  A child
-     * sizes its Gram partial sums for gram_batch problems (the parent's batch),
-     * so that its arithmetic is the parent's. */
+     * contexts of the same structure (kids[i] holds wave vectors i, i + G,
+     * i + 2G, ... of the parent's kid_nk), solved concurrently as by
+     * fame_eigs_many, with the parent's pass tiles; groups: 0 auto, 1 one
+     * batch, g >= 2 that many.  eps_ver / k_ver count fame_set_eps /
+     * fame_set_wave_vectors calls, so the children are brought up to date only
+     * when the parent changed.  A child sizes its Gram partial sums for
+     * gram_batch problems (the parent's batch), so that its arithmetic is the
+     * parent's, and writes its device outputs (reduced vectors, fields) with a
+     * problem stride of out_pmul problems, into its slots of the parent's. */
     int groups = 0;
     bool owns_stream = false;     /* a child's stream, destroyed with it */
     std::vector<fame_ctx*> kids;
-    std::vector<int> kid_lo;
+    int kid_nk = 0;
     int eps_ver = 0, k_ver = 0, kids_eps_ver = -1, kids_k_ver = -1;
     int gram_batch = 0;
+    int out_pmul = 1;
     long long cnt_qbq64 = 0, cnt_qbq32 = 0;
     int last_iters = 0;
     /* one CG iteration captured as a hipGraph per (precision, first problem,

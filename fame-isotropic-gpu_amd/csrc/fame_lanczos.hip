@@ -73,32 +73,38 @@ struct LzBuffers {
 };
 
 fame_status lz_alloc(fame_ctx* c, LzBuffers& b, int nk, int nstep, int nwant) {
-    for (void* p : c->big) if (p) (void)hipFree(p);
-    c->big.clear();
     b.n2 = 2 * (size_t)c->N; b.nk = nk; b.nstep = nstep; b.nwant = nwant;
     b.ls_time.assign(nk, 0.0);
     const size_t m = nstep + 1;
-    auto al = [&](void** p, size_t bytes) -> fame_status {
-        hipError_t e = hipMalloc(p, bytes ? bytes : 16);
-        if (e != hipSuccess) { *p = nullptr; FAIL(FAME_ERR_NOMEM, std::string("Lanczos allocation of ") + std::to_string(bytes) + " bytes failed"); }
-        c->big.push_back(*p);
+    const size_t gp = std::max((size_t)GEMV_BLK_GS * 2 * m, (size_t)gram_blk(c->gram_batch > 0 ? c->gram_batch : nk) * 2 * m * m);
+    void** ptr[] = {(void**)&b.U, (void**)&b.W, (void**)&b.T0d, (void**)&b.T1d, (void**)&b.hdot, (void**)&b.h,
+                    (void**)&b.hh, (void**)&b.nrm, (void**)&b.invb, (void**)&b.gpart, (void**)&b.G, (void**)&b.Q,
+                    (void**)&b.mask, (void**)&b.mask2};
+    const std::vector<size_t> bytes = {
+        16 * b.n2 * m * nk, 16 * b.n2 * (nwant + 1) * nk, sizeof(double) * m * nk, sizeof(double) * m * nk,
+        sizeof(double) * 2 * nk, sizeof(double) * 4 * nk, sizeof(double) * 2 * m * nk, sizeof(double) * nk,
+        sizeof(double) * nk, sizeof(double) * gp * nk, sizeof(double) * 2 * m * m * nk,
+        sizeof(double) * m * (nwant + 1) * nk, sizeof(int) * nk, sizeof(int) * nk};
+    /* the same shape as the last solve: keep its buffers (freeing and
+     * allocating the basis, gigabytes, costs tens of ms per solve) */
+    if (c->big_bytes == bytes && c->big.size() == bytes.size()) {
+        for (size_t i = 0; i < bytes.size(); ++i) *ptr[i] = c->big[i];
         return FAME_OK;
-    };
-    CHK(al((void**)&b.U, 16 * b.n2 * m * nk));
-    CHK(al((void**)&b.W, 16 * b.n2 * (nwant + 1) * nk));
-    CHK(al((void**)&b.T0d, sizeof(double) * m * nk));
-    CHK(al((void**)&b.T1d, sizeof(double) * m * nk));
-    CHK(al((void**)&b.hdot, sizeof(double) * 2 * nk));
-    CHK(al((void**)&b.h, sizeof(double) * 4 * nk));
-    CHK(al((void**)&b.hh, sizeof(double) * 2 * m * nk));
-    CHK(al((void**)&b.nrm, sizeof(double) * nk));
-    CHK(al((void**)&b.invb, sizeof(double) * nk));
-    size_t gp = std::max((size_t)GEMV_BLK_GS * 2 * m, (size_t)gram_blk(c->gram_batch > 0 ? c->gram_batch : nk) * 2 * m * m);
-    CHK(al((void**)&b.gpart, sizeof(double) * gp * nk));
-    CHK(al((void**)&b.G, sizeof(double) * 2 * m * m * nk));
-    CHK(al((void**)&b.Q, sizeof(double) * m * (nwant + 1) * nk));
-    CHK(al((void**)&b.mask, sizeof(int) * nk));
-    CHK(al((void**)&b.mask2, sizeof(int) * nk));
+    }
+    for (void* p : c->big) if (p) (void)hipFree(p);
+    c->big.clear();
+    c->big_bytes.clear();
+    for (size_t i = 0; i < bytes.size(); ++i) {
+        hipError_t e = hipMalloc(ptr[i], bytes[i] ? bytes[i] : 16);
+        if (e != hipSuccess) {
+            *ptr[i] = nullptr;
+            for (void* p : c->big) if (p) (void)hipFree(p);
+            c->big.clear();
+            FAIL(FAME_ERR_NOMEM, std::string("Lanczos allocation of ") + std::to_string(bytes[i]) + " bytes failed");
+        }
+        c->big.push_back(*ptr[i]);
+    }
+    c->big_bytes = bytes;
     return FAME_OK;
 }
 
@@ -261,18 +267,20 @@ static int split_groups(const fame_ctx* c) {
     return std::max(1, std::min(g, c->nk));
 }
 
+/* group i of G takes the wave vectors i, i + G, i + 2G, ...: neighbouring
+ * points of a path cost about the same, so the groups finish together */
+static int kid_count(int nk, int G, int i) { return (nk - i + G - 1) / G; }
+
 /* children for G groups, in step with the parent; false (and no children)
  * when the device lacks the memory for them next to the solve itself */
 static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok) {
     *ok = false;
-    std::vector<int> lo(G + 1);
-    for (int i = 0; i <= G; ++i) lo[i] = (int)((long long)c->nk * i / G);
     bool remake = (int)c->kids.size() != G;
-    for (int i = 0; !remake && i < G; ++i) remake = c->kids[i]->maxb < lo[i + 1] - lo[i];
+    for (int i = 0; !remake && i < G; ++i) remake = c->kids[i]->maxb < kid_count(c->nk, G, i);
     if (remake) {
         for (fame_ctx* k : c->kids) (void)fame_destroy(k);
         c->kids.clear();
-        c->kid_lo.clear();
+        c->kid_nk = 0;
         /* a child holds the per-problem work vectors of its wave vectors
          * (about 400 N bytes each) and its own permittivity; the Lanczos
          * buffers (U, W) move from the parent to the children */
@@ -284,7 +292,7 @@ static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok)
         if (freeb < need) return FAME_OK;
         for (int i = 0; i < G; ++i) {
             fame_ctx* k = nullptr;
-            const fame_status st = fame_create(&c->lat, lo[i + 1] - lo[i], c->device, &k);
+            const fame_status st = fame_create(&c->lat, kid_count(c->nk, G, i), c->device, &k);
             if (st != FAME_OK) {
                 for (fame_ctx* q : c->kids) (void)fame_destroy(q);
                 c->kids.clear();
@@ -310,9 +318,13 @@ static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok)
         for (fame_ctx* k : c->kids) CHK(fame_set_eps(k, e.data()));
         c->kids_eps_ver = c->eps_ver;
     }
-    if (c->kids_k_ver != c->k_ver || c->kid_lo != lo) {
-        for (int i = 0; i < G; ++i) CHK(fame_set_wave_vectors(c->kids[i], lo[i + 1] - lo[i], c->kvec.data() + 3 * lo[i], nullptr));
-        c->kid_lo = lo;
+    if (c->kids_k_ver != c->k_ver || c->kid_nk != c->nk) {
+        for (int i = 0; i < G; ++i) {
+            std::vector<double> kv;
+            for (int p = i; p < c->nk; p += G) kv.insert(kv.end(), c->kvec.begin() + 3 * p, c->kvec.begin() + 3 * p + 3);
+            CHK(fame_set_wave_vectors(c->kids[i], kid_count(c->nk, G, i), kv.data(), nullptr));
+        }
+        c->kid_nk = c->nk;
         c->kids_k_ver = c->k_ver;
     }
     for (fame_ctx* k : c->kids) {
@@ -330,6 +342,7 @@ static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok)
         k->tuned[0] = k->tuned[1] = true;    /* the parent's choice, tuned or fixed */
         k->groups = 1;
         k->gram_batch = c->nk;
+        k->out_pmul = G;
         k->use_graphs = c->use_graphs;
         k->ritz_sweeps = c->ritz_sweeps; k->ritz_tol = c->ritz_tol; k->ritz_maxit = c->ritz_maxit;
         k->refine_max_sweeps = c->refine_max_sweeps; k->refine_reltol = c->refine_reltol;
@@ -357,6 +370,7 @@ extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, 
     /* the Lanczos buffers of an earlier single-batch solve are not needed now */
     for (void* p : c->big) if (p) (void)hipFree(p);
     c->big.clear();
+    c->big_bytes.clear();
     HIPCHK(hipStreamSynchronize(c->stream));   /* work the caller queued on the parent */
     c->ls_last = *ls;
     const int nwant = es->nwant;
@@ -364,22 +378,34 @@ extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, 
     std::vector<int> precs(G, precision);
     std::vector<fame_es> ess(G, *es);
     std::vector<fame_ls> lss(G, *ls);
+    /* frequencies and profiles come back per group and are dealt out below; the
+     * device outputs are written in place, each child with a problem stride of
+     * G problems (out_pmul) from its first problem's slot */
+    std::vector<std::vector<double>> fk(G);
+    std::vector<std::vector<fame_profile>> pk(G);
     std::vector<double*> fr(G);
     std::vector<void*> ev(G), Ev(G);
     std::vector<fame_profile*> pr(G);
     long long q64 = 0, q32 = 0;
     for (int i = 0; i < G; ++i) {
-        const size_t lo = c->kid_lo[i];
-        fr[i] = freq + lo * nwant;
-        ev[i] = evr_dev ? (void*)((double2*)evr_dev + lo * nwant * n2) : nullptr;
-        Ev[i] = E_dev ? (void*)((double2*)E_dev + lo * nwant * n3) : nullptr;
-        pr[i] = prof ? prof + lo : nullptr;
+        const int n = kid_count(c->nk, G, i);
+        fk[i].assign((size_t)n * nwant, 0.0);
+        pk[i].assign(n, fame_profile());
+        fr[i] = fk[i].data();
+        pr[i] = pk[i].data();
+        ev[i] = evr_dev ? (void*)((double2*)evr_dev + (size_t)i * nwant * n2) : nullptr;
+        Ev[i] = E_dev ? (void*)((double2*)E_dev + (size_t)i * nwant * n3) : nullptr;
         q64 -= c->kids[i]->cnt_qbq64; q32 -= c->kids[i]->cnt_qbq32;
     }
     CHK(fame_eigs_many(G, c->kids.data(), precs.data(), ess.data(), lss.data(), fr.data(), evr_dev ? ev.data() : nullptr,
-                       E_dev ? Ev.data() : nullptr, prof ? pr.data() : nullptr));
+                       E_dev ? Ev.data() : nullptr, pr.data()));
     /* the children's streams are their own: their fields are complete when this returns */
     for (fame_ctx* k : c->kids) HIPCHK(hipStreamSynchronize(k->stream));
+    for (int i = 0; i < G; ++i)
+        for (int q = 0, p = i; p < c->nk; ++q, p += G) {
+            std::memcpy(freq + (size_t)p * nwant, fk[i].data() + (size_t)q * nwant, sizeof(double) * nwant);
+            if (prof) prof[p] = pk[i][q];
+        }
     for (fame_ctx* k : c->kids) { q64 += k->cnt_qbq64; q32 += k->cnt_qbq32; }
     c->cnt_qbq64 += q64; c->cnt_qbq32 += q32;
     return FAME_OK;
@@ -559,7 +585,8 @@ extern "C" fame_status fame_eigs_impl(fame_ctx* c, int precision, const fame_es*
         for (int p = 0; p < nk; ++p)
             for (int i = 0; i < nwant; ++i)
                 HIPCHK(fame_launch_pack(b.W + p * wps + (size_t)i * b.n2,
-                                        (double2*)evr_dev + ((size_t)p * nwant + i) * b.n2, c->N, c->nd[p], -1, 8, c->stream));
+                                        (double2*)evr_dev + ((size_t)p * c->out_pmul * nwant + i) * b.n2, c->N, c->nd[p], -1,
+                                        8, c->stream));
     }
     if (E_dev) CHK(fame_restore_impl(c, nwant, b.W, wps, E_dev));
     /* Gamma rule on the frequencies (FAME_Fast_Algorithms_Isotropic.cu:61-70) */
@@ -604,7 +631,7 @@ static fame_status restore_band(fame_ctx* c, int i, int nwant, const double2* ev
     HIPCHK(hipMemcpyAsync(c->scal, nr.data(), sizeof(double) * nk, hipMemcpyHostToDevice, c->stream));
     HIPCHK(fame_scale_real_mod(c->F64, c->F64, c->scal, N3, N3, 1, 1, nk, 8, c->stream));
     for (int p = 0; p < nk; ++p)
-        CHK(c->field_to_ref(c->F64 + (size_t)p * N3, (double2*)E_dev + ((size_t)p * nwant + i) * N3, 1, N3));
+        CHK(c->field_to_ref(c->F64 + (size_t)p * N3, (double2*)E_dev + ((size_t)p * c->out_pmul * nwant + i) * N3, 1, N3));
     return FAME_OK;
 }
 
@@ -614,7 +641,7 @@ static fame_status restore_gamma(fame_ctx* c, int nwant, void* E_dev) {
     const size_t N3 = 3 * (size_t)c->N;
     for (int p = 0; p < c->nk; ++p) {
         if (!c->prob_h[p].gamma) continue;
-        double2* E = (double2*)E_dev + (size_t)p * nwant * N3;
+        double2* E = (double2*)E_dev + (size_t)p * c->out_pmul * nwant * N3;
         for (int i = nwant - 1; i >= 2; --i)
             HIPCHK(hipMemcpyAsync(E + (size_t)i * N3, E + (size_t)(i - 2) * N3, 16 * N3, hipMemcpyDeviceToDevice, c->stream));
         std::vector<double2> cst(N3, make_double2(1.0 / std::sqrt((double)N3), 0.0));

@@ -56,10 +56,11 @@ typedef struct {
     int    restarts;       /* Lanczos_Isotropic return value                         */
     int    nd;             /* Nd (N-1 at Gamma)                                      */
     int    converged;      /* 1 if the Lanczos convergence test passed               */
-    int    qbq64;          /* FP64 QBQ applies                                       */
-    int    qbq32;          /* FP32 QBQ applies (refinement inner solves)             */
+    int    qbq64;          /* FP64 QBQ applications, mean per wave vector of the batch
+                              (or of its group, fame_set_groups) solved with it      */
+    int    qbq32;          /* FP32 QBQ applications (refinement inner solves), same   */
     int    pad;
-    double es_time;        /* seconds, eigensolve of the whole batch                 */
+    double es_time;        /* seconds, eigensolve of the whole batch (or group)      */
     double ls_time;        /* seconds inside invAr (CG) solves, this wave vector's share
                               of the batch's solves (the reference's Profile.ls_time,
                               FAME_Matrix_Vector_Production_Isotropic_invAr.cu:29-43) */
@@ -131,16 +132,18 @@ fame_status fame_eigs(fame_ctx* ctx, int precision, const fame_es* es, const fam
 fame_status fame_set_refinement(fame_ctx* ctx, int sweeps, double tol, int maxit);
 
 /* How fame_eigs runs a batch: groups = 1 as one batch; g >= 2 split into g
- * groups of consecutive wave vectors, each solved by a child context of the
- * same structure on its own stream, concurrently (as fame_eigs_many does for
- * several structures), so that one group's CG tails and vector work overlap
- * another's passes; 0 (default) splits batches of 16 or more wave vectors in
- * two.  The children use the parent's pass tiles and arithmetic, so the
- * results are bit-identical to one batch.  They hold the per-problem work
- * vectors of their wave vectors (about 400 N bytes each) until the context is
- * destroyed; when the device lacks that memory, fame_eigs runs one batch.
- * Replaces the reference's one-wave-vector-at-a-time loop in FAME_Main_Code
- * (FAME_Main_Code.cu:79-131) as a batching policy; no reference counterpart. */
+ * groups (wave vectors i, i + g, i + 2g, ... in group i, so that neighbouring
+ * points of a path, of similar cost, go to different groups), each solved by
+ * a child context of the same structure on its own stream, concurrently (as
+ * fame_eigs_many does for several structures), so that one group's CG tails
+ * and vector work overlap another's passes; 0 (default) splits batches of 16
+ * or more wave vectors in two.  The children use the parent's pass tiles and
+ * arithmetic, so the results are bit-identical to one batch.  They hold the
+ * per-problem work vectors of their wave vectors (about 400 N bytes each)
+ * until the context is destroyed; when the device lacks that memory,
+ * fame_eigs runs one batch.  Replaces the reference's one-wave-vector-at-a-time
+ * loop in FAME_Main_Code (FAME_Main_Code.cu:79-131) as a batching policy; no
+ * reference counterpart.                                                    */
 fame_status fame_set_groups(fame_ctx* ctx, int groups);
 
 /* Restored E fields (device, nk*nwant*3N, reference scaling and Gamma rule, as
