@@ -355,8 +355,9 @@ extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, 
                                  void* evr_dev, void* E_dev, fame_profile* prof) {
     if (!c || !es || !ls || !freq) FAIL(FAME_ERR_ARG, "fame_eigs: null argument");
     const int G = (c->have_eps && c->nk >= 1) ? split_groups(c) : 1;
-    if (G <= 1 || es->nwant < 1 || es->nstep <= es->nwant || es->nstep + 1 > FAME_LZ_MAXCOL ||
-        (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE))
+    /* one batch, or arguments that fame_eigs_impl rejects with its own message */
+    if (G <= 1 || es->nwant < 1 || es->nstep <= es->nwant || es->nstep + 1 > FAME_LZ_MAXCOL || es->maxit < 1 ||
+        ls->maxit < 1 || (precision != FAME_PREC_F64 && precision != FAME_PREC_F32_REFINE))
         return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
     HIPCHK(hipSetDevice(c->device));
     /* the parent's tiles, tuned on the whole batch as a single solve would */
