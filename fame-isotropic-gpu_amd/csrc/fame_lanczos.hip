@@ -258,12 +258,14 @@ extern "C" fame_status fame_set_refinement(fame_ctx* c, int sweeps, double tol, 
  * slowest few problems still iterate, and the Lanczos vector work and host
  * checks leave the passes' share of the GPU idle meanwhile.  Two halves of the
  * batch, each a context on its own stream, fill those gaps with each other's
- * passes (FCC 96, 30 wave vectors: 4.70 against 4.49 k-pts/s on one MI355X).
+ * passes (FCC 96 on one MI355X: 4.72 against 4.48 k-pts/s for 30 wave vectors,
+ * 4.21 against 3.88 for 8, 3.64 against 3.20 for 2; four groups were slower
+ * than two at 4, 8 and 16 wave vectors).
  * Every problem's arithmetic is independent of the batch it runs in, and the
  * children take the parent's tiles and Gram blocking, so the results are
  * bit-identical to one batch.                                              */
 static int split_groups(const fame_ctx* c) {
-    const int g = c->groups > 0 ? c->groups : (c->nk >= 16 ? 2 : 1);
+    const int g = c->groups > 0 ? c->groups : 2;
     return std::max(1, std::min(g, c->nk));
 }
 
@@ -328,6 +330,11 @@ static fame_status sync_kids(fame_ctx* c, int G, int nstep, int nwant, bool* ok)
         c->kids_k_ver = c->k_ver;
     }
     for (fame_ctx* k : c->kids) {
+        /* the parent's plans: a child built under another FAME_FFT setting may
+         * lack the matrix-core plan that the parent's tiles name */
+        k->eng_forced = c->eng_forced;
+        for (int a = 0; a < 3; ++a)
+            if (c->ax[a][1].n > 0 && k->ax[a][1].n == 0) CHK(k->ax[a][1].build(c->ax[a][1].n, FFT_ENGINE_MFMA));
         bool same = true;
         for (int pi = 0; pi < 2; ++pi)
             for (int a = 0; a < 3; ++a)

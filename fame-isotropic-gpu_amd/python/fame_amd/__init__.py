@@ -369,7 +369,7 @@ class Solver:
 
     def set_groups(self, groups: int):
         """How eigs runs the batch: 1 one batch, g >= 2 that many concurrent groups,
-        0 (default) two groups from 16 wave vectors on; see fame_set_groups."""
+        0 (default) two groups from two wave vectors on; see fame_set_groups."""
         _check(lib().fame_set_groups(self._ctx, int(groups)), "fame_set_groups")
 
     def restore(self, evr):

@@ -136,8 +136,8 @@ fame_status fame_set_refinement(fame_ctx* ctx, int sweeps, double tol, int maxit
  * points of a path, of similar cost, go to different groups), each solved by
  * a child context of the same structure on its own stream, concurrently (as
  * fame_eigs_many does for several structures), so that one group's CG tails
- * and vector work overlap another's passes; 0 (default) splits batches of 16
- * or more wave vectors in two.  The children use the parent's pass tiles and
+ * and vector work overlap another's passes; 0 (default) splits every batch of
+ * two or more wave vectors in two.  The children use the parent's pass tiles and
  * arithmetic, so the results are bit-identical to one batch.  They hold the
  * per-problem work vectors of their wave vectors (about 400 N bytes each)
  * until the context is destroyed; when the device lacks that memory,

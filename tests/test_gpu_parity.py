@@ -260,23 +260,30 @@ def test_qbq_counts_are_applications(gpu):
     in a batch whose problems finish their CG solves at different iterations,
     the FP64 count equals the CG iterations (ls_iter minus one per solve) and
     the FP32 count of the refined solve equals its FP32 iterations, averaged
-    over the batch's wave vectors as the profile reports them."""
+    over the batch's wave vectors as the profile reports them (over each
+    group's when the batch is split)."""
     lo, fl, ol, eps = make("fcc")
     ks = np.array([[0.21, -0.13, 0.07], [0.0, 0.0, 0.0], [0.4, 0.1, 0.2], [0.33, 0.33, 0.0]])
     S = F.Solver(fl, max_batch=len(ks))
     S.set_eps(eps)
     S.set_wave_vectors(ks)
+    n = len(ks)
     for prec in (F.PREC_F64, F.PREC_F32_REFINE):
+        S.set_groups(1)
         _, _, _, prof = S.eigs(8, 14, precision=prec, want_fields=False)
         cg_its = sum(p["ls_iter"] - p["es_iter"] for p in prof)
         solves = sum(p["es_iter"] for p in prof)
-        n = len(ks)
         if prec == F.PREC_F64:
             assert prof[0]["qbq64"] == cg_its // n and prof[0]["qbq32"] == 0, (prof[0], cg_its)
         else:
             assert prof[0]["qbq32"] == cg_its // n, (prof[0], cg_its)
             # one FP64 residual pass per sweep, at least one sweep per solve
             assert solves // n <= prof[0]["qbq64"] < prof[0]["qbq32"], (prof[0], solves)
+        # split in two groups, each entry holds its group's mean: same total
+        S.set_groups(2)
+        _, _, _, prof2 = S.eigs(8, 14, precision=prec, want_fields=False)
+        key = "qbq64" if prec == F.PREC_F64 else "qbq32"
+        assert abs(sum(p[key] for p in prof2) - cg_its) < n, (prof2, cg_its)
     S.close()
 
 
