@@ -61,7 +61,7 @@ def test_c1_sc32_vs_oracle(gpu):
         res = S.residuals(f, E)
         assert np.all(res[0] < 1e-6 * f[0] ** 2)
     # two inverse-iteration sweeps of the converged Ritz pairs reach the field
-    # residual the north star asks for, ||C^H C E - lambda B E|| / ||E|| <= 1e-10
+    # residual target ||C^H C E - lambda B E|| / ||E|| <= 1e-10
     # in FP64 (the FP32-refined solve stays within 100 eps ||C^H C||)
     S.set_refinement(2, 1e-16, 6000)
     for prec, bar in ((F.PREC_F64, 1e-10), (F.PREC_F32_REFINE, 100 * 2.2e-16 * _curlcurl_norm(fl))):

@@ -4,7 +4,8 @@ FCC diamond network at 64 -> 64x55x52 (12 bands, nstep 36) and 96 -> 96x83x78
 each.  The expected frequencies were computed by the FP64 CPU oracle and are
 committed in tests/golden/oracle_eigs_fcc.json (tests/golden/make_oracle_eigs.py).
 The GPU's FP64 and FP32+FP64-refinement solves are held to 1e-8 relative in
-lambda = omega^2, the north-star tolerance; the permittivity is checked to be
+lambda = omega^2, the eigenvalue tolerance of the benchmark workload; the
+permittivity is checked to be
 the one the fixture was made from."""
 import hashlib
 import json

@@ -4,7 +4,8 @@ Operators (Qr, Qrs, Pr, Prs, QBQ, invAr) on random vectors, CG, and the full
 restarted Lanczos eigensolve (FP64 and FP32+FP64-refinement), for Simple (SC)
 and General (FCC, BCC, hexagonal) lattices, at a generic k and at Gamma.
 Bars: operators 1e-11 relative (FP64 FFT rounding), eigenvalues 1e-8 relative
-residual ||A E - lambda B E||/||E|| small.
+(the eigenvalue tolerance of the benchmark workload), residual
+||A E - lambda B E||/||E|| small.
 Both line-transform engines are covered: Stockham/Bluestein passes and the
 direct DFT on the matrix cores (FAME_FFT selects the engine at context
 creation; the default picks the matrix cores for axis lengths 16..127).
