@@ -233,7 +233,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver shards over wave vectors with no communication, "
-              "and multi-GPU runs are not part of this build", file=sys.stderr)
+              "and runs on several GPUs are not implemented", file=sys.stderr)
         sys.exit(2)
 
     import torch
