@@ -372,13 +372,15 @@ extern "C" fame_status fame_eigs(fame_ctx* c, int precision, const fame_es* es, 
         if (!c->tuned[0]) CHK(c->tune(8));
         if (precision == FAME_PREC_F32_REFINE && !c->tuned[1]) CHK(c->tune(4));
     }
-    bool ok = false;
-    CHK(sync_kids(c, G, es->nstep, es->nwant, &ok));
-    if (!ok) return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
-    /* the Lanczos buffers of an earlier single-batch solve are not needed now */
+    /* the Lanczos buffers of an earlier single-batch solve are not needed by
+     * the children (freed before their memory is checked) */
+    HIPCHK(hipStreamSynchronize(c->stream));
     for (void* p : c->big) if (p) (void)hipFree(p);
     c->big.clear();
     c->big_bytes.clear();
+    bool ok = false;
+    CHK(sync_kids(c, G, es->nstep, es->nwant, &ok));
+    if (!ok) return fame_eigs_impl(c, precision, es, ls, freq, evr_dev, E_dev, prof);
     HIPCHK(hipStreamSynchronize(c->stream));   /* work the caller queued on the parent */
     c->ls_last = *ls;
     const int nwant = es->nwant;
