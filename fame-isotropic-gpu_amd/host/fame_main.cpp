@@ -141,11 +141,13 @@ extern "C" int fame_main_code_run(const PAR* ParP, PROFILE* Profile) {
     }
 
     fame_lattice L = lattice_of(Par);
-    /* batch size from the free device memory: Lanczos basis, fields and CG work */
+    /* batch size from the free device memory: Lanczos basis, fields and CG
+     * work, and the work vectors of the two groups fame_eigs splits a batch
+     * into (about 400 N bytes per wave vector) */
     if (hipSetDevice(Par.flag.device) != hipSuccess) return FAME_ERR_HIP;
     size_t freeb = 0, totb = 0;
     if (hipMemGetInfo(&freeb, &totb) != hipSuccess) return FAME_ERR_HIP;
-    const double per_k = 16.0 * 2 * N * (Nstep + Nwant + 16) + 24.0 * N3 * (Nwant + 2);
+    const double per_k = 16.0 * 2 * N * (Nstep + Nwant + 16) + 24.0 * N3 * (Nwant + 2) + 400.0 * N;
     int batch = (int)std::max(1.0, std::min((double)Nk, 0.7 * (double)freeb / per_k));
 
     fame_ctx* ctx = nullptr;
